@@ -1,0 +1,1360 @@
+/*
+ * lulesh_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY; see the header).
+ *
+ * A sequential restatement of the OpenMP-4.0 LULESH 2.0 step in the
+ * reference's own operation order, so that it reproduces the compiled
+ * reference bit for bit (checked in tests against the unmodified omp_4.0
+ * sources built under oracle/_ref).  Every floating-point expression keeps
+ * the reference's association order; the file must be compiled with
+ * -ffp-contract=off.  Citations are path:line into the reference tree.
+ *
+ * Multi-subdomain groups emulate the MPI exchange of lulesh-comm.cc:
+ * every subdomain packs (CommSend) before any subdomain unpacks
+ * (CommSBN / CommSyncPosVel / CommMonoQ), which is what the reference's
+ * Irecv/Isend/Wait ordering guarantees.
+ */
+#include "lulesh_oracle.h"
+
+#include <limits.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- constants: LULESH/omp_4.0/lulesh.h:60-103 (BC bits, tags) ---- */
+#define XI_M        0x00007
+#define XI_M_SYMM   0x00001
+#define XI_M_FREE   0x00002
+#define XI_M_COMM   0x00004
+#define XI_P        0x00038
+#define XI_P_SYMM   0x00008
+#define XI_P_FREE   0x00010
+#define XI_P_COMM   0x00020
+#define ETA_M       0x001c0
+#define ETA_M_SYMM  0x00040
+#define ETA_M_FREE  0x00080
+#define ETA_M_COMM  0x00100
+#define ETA_P       0x00e00
+#define ETA_P_SYMM  0x00200
+#define ETA_P_FREE  0x00400
+#define ETA_P_COMM  0x00800
+#define ZETA_M      0x07000
+#define ZETA_M_SYMM 0x01000
+#define ZETA_M_FREE 0x02000
+#define ZETA_M_COMM 0x04000
+#define ZETA_P      0x38000
+#define ZETA_P_SYMM 0x08000
+#define ZETA_P_FREE 0x10000
+#define ZETA_P_COMM 0x20000
+
+#define MSG_COMM_SBN     1024
+#define MSG_SYNC_POS_VEL 2048
+#define MSG_MONOQ        3072
+#define MAX_FIELDS_PER_MPI_COMM 6
+#define PAD_REAL 16                       /* 128 B / sizeof(double) */
+#define ALIGN_REAL(n) (((n) + (PAD_REAL - 1)) & ~(PAD_REAL - 1))
+
+/* field ids: same numbering as include/lulesh_amd.h */
+enum {
+    F_X = 0, F_Y, F_Z, F_XD, F_YD, F_ZD, F_XDD, F_YDD, F_ZDD,
+    F_FX, F_FY, F_FZ, F_NODAL_MASS,
+    F_E = 16, F_P, F_Q, F_QL, F_QQ, F_V, F_VOLO, F_DELV, F_VDOV, F_AREALG,
+    F_SS, F_ELEM_MASS, F_DELV_XI, F_DELV_ETA, F_DELV_ZETA,
+    F_DELX_XI, F_DELX_ETA, F_DELX_ZETA, F_VNEW
+};
+
+typedef struct {
+    int rank, col, row, plane, tp, nx;
+    int numElem, numNode, allElem;
+    int rowMin, rowMax, colMin, colMax, planeMin, planeMax;
+    int maxPlaneSize, maxEdgeSize, comBufSize;
+
+    double *x, *y, *z, *xd, *yd, *zd, *xdd, *ydd, *zdd, *fx, *fy, *fz, *nodalMass;
+    int *symmX, *symmY, *symmZ;            /* NULL when empty */
+
+    int *nodelist, *lxim, *lxip, *letam, *letap, *lzetam, *lzetap, *elemBC;
+    double *e, *p, *q, *ql, *qq, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass;
+    double *vnew, *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
+
+    int numReg, cost;
+    int *regElemSize, **regElemlist, *regNumList, *elemRep;
+    int *nodeElemStart, *nodeElemCornerList;
+
+    /* per-step scratch (the reference allocates these per call) */
+    double *sig, *determ, *fx_elem, *fy_elem, *fz_elem;
+    double *dvdx, *dvdy, *dvdz, *x8n, *y8n, *z8n;
+
+    double dtcourant, dthydro;
+    /* every rank keeps its own time state (lulesh.h:237-247 of omp_4.0);
+     * dt0 depends on the rank's own volo(0) (lulesh-init.cc:180) */
+    double time, deltatime;
+    int cycle;
+    double *commDataRecv;
+} lo_dom;
+
+struct lo_group {
+    int nx, tp, ndom, cbrt_mode;
+    lo_dom *d;
+    double stoptime, dtfixed, dtmax, deltatimemultlb, deltatimemultub;
+    int status;
+    /* physics constants: LULESH/omp_4.0/lulesh-init.cc:20-38 */
+    double e_cut, p_cut, q_cut, v_cut, u_cut, hgcoef, ss4o3, qstop,
+           monoq_max_slope, monoq_limiter_mult, qlc_monoq, qqc_monoq, qqc,
+           eosvmax, eosvmin, pmin, emin, dvovmax, refdens;
+};
+
+static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+static int    *ialloc(size_t n) { return (int *)calloc(n ? n : 1, sizeof(int)); }
+
+/* ================= element geometry helpers (omp lulesh.cc) ================ */
+
+/* CalcElemVolume, lulesh.cc:1627-1709 */
+static double elem_volume(const double x[8], const double y[8], const double z[8])
+{
+    const double twelveth = 1.0 / 12.0;
+    double dx61 = x[6] - x[1], dy61 = y[6] - y[1], dz61 = z[6] - z[1];
+    double dx70 = x[7] - x[0], dy70 = y[7] - y[0], dz70 = z[7] - z[0];
+    double dx63 = x[6] - x[3], dy63 = y[6] - y[3], dz63 = z[6] - z[3];
+    double dx20 = x[2] - x[0], dy20 = y[2] - y[0], dz20 = z[2] - z[0];
+    double dx50 = x[5] - x[0], dy50 = y[5] - y[0], dz50 = z[5] - z[0];
+    double dx64 = x[6] - x[4], dy64 = y[6] - y[4], dz64 = z[6] - z[4];
+    double dx31 = x[3] - x[1], dy31 = y[3] - y[1], dz31 = z[3] - z[1];
+    double dx72 = x[7] - x[2], dy72 = y[7] - y[2], dz72 = z[7] - z[2];
+    double dx43 = x[4] - x[3], dy43 = y[4] - y[3], dz43 = z[4] - z[3];
+    double dx57 = x[5] - x[7], dy57 = y[5] - y[7], dz57 = z[5] - z[7];
+    double dx14 = x[1] - x[4], dy14 = y[1] - y[4], dz14 = z[1] - z[4];
+    double dx25 = x[2] - x[5], dy25 = y[2] - y[5], dz25 = z[2] - z[5];
+    /* 3x3 determinant with rows (a,b,c), (d,e,f), (g,h,i):
+       a(e i - f h) + d(c h - b i) + g(b f - c e), reference operand order */
+#define DET3(a, b, c, d, e_, f, g, h, i) \
+    ((a) * ((e_) * (i) - (f) * (h)) + (d) * ((c) * (h) - (b) * (i)) + (g) * ((b) * (f) - (c) * (e_)))
+    double vol =
+        DET3(dx31 + dx72, dx63, dx20, dy31 + dy72, dy63, dy20, dz31 + dz72, dz63, dz20) +
+        DET3(dx43 + dx57, dx64, dx70, dy43 + dy57, dy64, dy70, dz43 + dz57, dz64, dz70) +
+        DET3(dx14 + dx25, dx61, dx50, dy14 + dy25, dy61, dy50, dz14 + dz25, dz61, dz50);
+#undef DET3
+    vol *= twelveth;
+    return vol;
+}
+
+/* CalcElemShapeFunctionDerivatives, lulesh.cc:342-428.  b may be NULL when
+ * only the Jacobian volume is wanted (IntegrateStressForElems overwrites B). */
+static double shape_derivs(const double x[8], const double y[8], const double z[8],
+                           double b[3][8])
+{
+    double fjxxi = .125 * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
+    double fjxet = .125 * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
+    double fjxze = .125 * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
+    double fjyxi = .125 * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
+    double fjyet = .125 * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
+    double fjyze = .125 * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
+    double fjzxi = .125 * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
+    double fjzet = .125 * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
+    double fjzze = .125 * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
+
+    double cjxxi = (fjyet * fjzze) - (fjzet * fjyze);
+    double cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
+    double cjxze = (fjyxi * fjzet) - (fjzxi * fjyet);
+    double cjyxi = -(fjxet * fjzze) + (fjzet * fjxze);
+    double cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
+    double cjyze = -(fjxxi * fjzet) + (fjzxi * fjxet);
+    double cjzxi = (fjxet * fjyze) - (fjyet * fjxze);
+    double cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
+    double cjzze = (fjxxi * fjyet) - (fjyxi * fjxet);
+
+    if (b) {
+        b[0][0] = -cjxxi - cjxet - cjxze;
+        b[0][1] = cjxxi - cjxet - cjxze;
+        b[0][2] = cjxxi + cjxet - cjxze;
+        b[0][3] = -cjxxi + cjxet - cjxze;
+        b[0][4] = -b[0][2]; b[0][5] = -b[0][3]; b[0][6] = -b[0][0]; b[0][7] = -b[0][1];
+        b[1][0] = -cjyxi - cjyet - cjyze;
+        b[1][1] = cjyxi - cjyet - cjyze;
+        b[1][2] = cjyxi + cjyet - cjyze;
+        b[1][3] = -cjyxi + cjyet - cjyze;
+        b[1][4] = -b[1][2]; b[1][5] = -b[1][3]; b[1][6] = -b[1][0]; b[1][7] = -b[1][1];
+        b[2][0] = -cjzxi - cjzet - cjzze;
+        b[2][1] = cjzxi - cjzet - cjzze;
+        b[2][2] = cjzxi + cjzet - cjzze;
+        b[2][3] = -cjzxi + cjzet - cjzze;
+        b[2][4] = -b[2][2]; b[2][5] = -b[2][3]; b[2][6] = -b[2][0]; b[2][7] = -b[2][1];
+    }
+    return 8. * (fjxet * cjxet + fjyet * cjyet + fjzet * cjzet);
+}
+
+/* SumElemFaceNormal + CalcElemNodeNormals, lulesh.cc:433-525.  Faces are
+ * visited in the reference order; each face adds its area vector to its
+ * four nodes. */
+static const int kFaceNodes[6][4] = {
+    {0, 1, 2, 3}, {0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3}, {3, 7, 4, 0}, {4, 7, 6, 5}
+};
+
+static void node_normals(double pf[3][8], const double x[8], const double y[8], const double z[8])
+{
+    for (int i = 0; i < 8; ++i) pf[0][i] = pf[1][i] = pf[2][i] = 0.0;
+    for (int f = 0; f < 6; ++f) {
+        const int *n = kFaceNodes[f];
+        double bx0 = 0.5 * (x[n[3]] + x[n[2]] - x[n[1]] - x[n[0]]);
+        double by0 = 0.5 * (y[n[3]] + y[n[2]] - y[n[1]] - y[n[0]]);
+        double bz0 = 0.5 * (z[n[3]] + z[n[2]] - z[n[1]] - z[n[0]]);
+        double bx1 = 0.5 * (x[n[2]] + x[n[1]] - x[n[3]] - x[n[0]]);
+        double by1 = 0.5 * (y[n[2]] + y[n[1]] - y[n[3]] - y[n[0]]);
+        double bz1 = 0.5 * (z[n[2]] + z[n[1]] - z[n[3]] - z[n[0]]);
+        double ax = 0.25 * (by0 * bz1 - bz0 * by1);
+        double ay = 0.25 * (bz0 * bx1 - bx0 * bz1);
+        double az = 0.25 * (bx0 * by1 - by0 * bx1);
+        for (int k = 0; k < 4; ++k) {
+            pf[0][n[k]] += ax;
+            pf[1][n[k]] += ay;
+            pf[2][n[k]] += az;
+        }
+    }
+}
+
+/* VoluDer + CalcElemVolumeDerivative, lulesh.cc:717-789.  Row j lists the
+ * six node arguments the reference passes for corner j. */
+static const int kVoluDerArgs[8][6] = {
+    {1, 2, 3, 4, 5, 7}, {2, 3, 0, 5, 6, 4}, {3, 0, 1, 6, 7, 5}, {0, 1, 2, 7, 4, 6},
+    {7, 6, 5, 0, 3, 1}, {4, 7, 6, 1, 0, 2}, {5, 4, 7, 2, 1, 3}, {6, 5, 4, 3, 2, 0}
+};
+
+static void volume_derivs(double dvdx[8], double dvdy[8], double dvdz[8],
+                          const double x[8], const double y[8], const double z[8])
+{
+    const double twelfth = 1.0 / 12.0;
+    for (int j = 0; j < 8; ++j) {
+        const int *a = kVoluDerArgs[j];
+        double x0 = x[a[0]], x1 = x[a[1]], x2 = x[a[2]], x3 = x[a[3]], x4 = x[a[4]], x5 = x[a[5]];
+        double y0 = y[a[0]], y1 = y[a[1]], y2 = y[a[2]], y3 = y[a[3]], y4 = y[a[4]], y5 = y[a[5]];
+        double z0 = z[a[0]], z1 = z[a[1]], z2 = z[a[2]], z3 = z[a[3]], z4 = z[a[4]], z5 = z[a[5]];
+        double dx = (y1 + y2) * (z0 + z1) - (y0 + y1) * (z1 + z2) +
+                    (y0 + y4) * (z3 + z4) - (y3 + y4) * (z0 + z4) -
+                    (y2 + y5) * (z3 + z5) + (y3 + y5) * (z2 + z5);
+        double dy = -(x1 + x2) * (z0 + z1) + (x0 + x1) * (z1 + z2) -
+                    (x0 + x4) * (z3 + z4) + (x3 + x4) * (z0 + z4) +
+                    (x2 + x5) * (z3 + z5) - (x3 + x5) * (z2 + z5);
+        double dz = -(y1 + y2) * (x0 + x1) + (y0 + y1) * (x1 + x2) -
+                    (y0 + y4) * (x3 + x4) + (y3 + y4) * (x0 + x4) +
+                    (y2 + y5) * (x3 + x5) - (y3 + y5) * (x2 + x5);
+        dvdx[j] = dx * twelfth;
+        dvdy[j] = dy * twelfth;
+        dvdz[j] = dz * twelfth;
+    }
+}
+
+/* AreaFace, lulesh.cc:1725-1744 */
+static double area_face(double x0, double x1, double x2, double x3,
+                        double y0, double y1, double y2, double y3,
+                        double z0, double z1, double z2, double z3)
+{
+    double fx = (x2 - x0) - (x3 - x1), fy = (y2 - y0) - (y3 - y1), fz = (z2 - z0) - (z3 - z1);
+    double gx = (x2 - x0) + (x3 - x1), gy = (y2 - y0) + (y3 - y1), gz = (z2 - z0) + (z3 - z1);
+    return (fx * fx + fy * fy + fz * fz) * (gx * gx + gy * gy + gz * gz) -
+           (fx * gx + fy * gy + fz * gz) * (fx * gx + fy * gy + fz * gz);
+}
+
+/* CalcElemCharacteristicLength, lulesh.cc:1751-1791 */
+static const int kLenFaces[6][4] = {
+    {0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}
+};
+
+static double char_length(const double x[8], const double y[8], const double z[8], double volume)
+{
+    double cl = 0.0;
+    for (int f = 0; f < 6; ++f) {
+        const int *n = kLenFaces[f];
+        double a = area_face(x[n[0]], x[n[1]], x[n[2]], x[n[3]],
+                             y[n[0]], y[n[1]], y[n[2]], y[n[3]],
+                             z[n[0]], z[n[1]], z[n[2]], z[n[3]]);
+        cl = a > cl ? a : cl;
+    }
+    return 4.0 * volume / sqrt(cl);
+}
+
+static void gather8(const int *nl, const double *src, double out[8])
+{
+    for (int i = 0; i < 8; ++i) out[i] = src[nl[i]];
+}
+
+/* ============================ nodal phase ============================== */
+
+/* IntegrateStressForElems, lulesh.cc:546-712 (threaded form: per-element
+ * corner forces, then an ordered per-node gather). */
+static int integrate_stress(lo_dom *d)
+{
+    int bad = 0;
+    for (int k = 0; k < d->numElem; ++k) {
+        const int *nl = &d->nodelist[8 * k];
+        double xl[8], yl[8], zl[8], B[3][8];
+        gather8(nl, d->x, xl); gather8(nl, d->y, yl); gather8(nl, d->z, zl);
+        d->determ[k] = shape_derivs(xl, yl, zl, NULL);
+        node_normals(B, xl, yl, zl);
+        double s = d->sig[k];
+        for (int i = 0; i < 8; ++i) {
+            d->fx_elem[8 * k + i] = -(s * B[0][i]);
+            d->fy_elem[8 * k + i] = -(s * B[1][i]);
+            d->fz_elem[8 * k + i] = -(s * B[2][i]);
+        }
+    }
+    for (int n = 0; n < d->numNode; ++n) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int c = d->nodeElemStart[n]; c < d->nodeElemStart[n + 1]; ++c) {
+            int ce = d->nodeElemCornerList[c];
+            fx += d->fx_elem[ce];
+            fy += d->fy_elem[ce];
+            fz += d->fz_elem[ce];
+        }
+        d->fx[n] = fx; d->fy[n] = fy; d->fz[n] = fz;
+    }
+    /* CalcVolumeForceForElems, lulesh.cc:1340-1349 */
+    for (int k = 0; k < d->numElem; ++k)
+        if (d->determ[k] <= 0.0) bad = 1;
+    return bad;
+}
+
+/* CalcHourglassControlForElems + CalcFBHourglassForceForElems,
+ * lulesh.cc:837-1297 */
+static const double kGamma[4][8] = {
+    { 1.,  1., -1., -1., -1., -1.,  1.,  1.},
+    { 1., -1., -1.,  1., -1.,  1.,  1., -1.},
+    { 1., -1.,  1., -1.,  1., -1.,  1., -1.},
+    {-1.,  1., -1.,  1.,  1., -1.,  1., -1.}
+};
+
+static int hourglass(lo_group *g, lo_dom *d)
+{
+    int bad = 0;
+    for (int i = 0; i < d->numElem; ++i) {
+        const int *nl = &d->nodelist[8 * i];
+        double x1[8], y1[8], z1[8];
+        gather8(nl, d->x, x1); gather8(nl, d->y, y1); gather8(nl, d->z, z1);
+        volume_derivs(&d->dvdx[8 * i], &d->dvdy[8 * i], &d->dvdz[8 * i], x1, y1, z1);
+        for (int j = 0; j < 8; ++j) {
+            d->x8n[8 * i + j] = x1[j]; d->y8n[8 * i + j] = y1[j]; d->z8n[8 * i + j] = z1[j];
+        }
+        d->determ[i] = d->volo[i] * d->v[i];
+        if (d->v[i] <= 0.0) bad = 1;
+    }
+    if (bad) return 1;
+    if (!(g->hgcoef > 0.0)) return 0;
+
+    const double hourg = g->hgcoef;
+    for (int i2 = 0; i2 < d->numElem; ++i2) {
+        const int *nl = &d->nodelist[8 * i2];
+        const int i3 = 8 * i2;
+        double hourgam[8][4];
+        double volinv = 1.0 / d->determ[i2];
+        for (int m = 0; m < 4; ++m) {
+            double hx = 0.0, hy = 0.0, hz = 0.0;
+            /* x8n[i3]*gamma[m][0] + ... + x8n[i3+7]*gamma[m][7], left to right */
+            hx = d->x8n[i3] * kGamma[m][0];
+            hy = d->y8n[i3] * kGamma[m][0];
+            hz = d->z8n[i3] * kGamma[m][0];
+            for (int j = 1; j < 8; ++j) {
+                hx = hx + d->x8n[i3 + j] * kGamma[m][j];
+                hy = hy + d->y8n[i3 + j] * kGamma[m][j];
+                hz = hz + d->z8n[i3 + j] * kGamma[m][j];
+            }
+            for (int j = 0; j < 8; ++j)
+                hourgam[j][m] = kGamma[m][j] -
+                    volinv * (d->dvdx[i3 + j] * hx + d->dvdy[i3 + j] * hy + d->dvdz[i3 + j] * hz);
+        }
+        double ss1 = d->ss[i2], mass1 = d->elemMass[i2];
+        double volume13 = (g->cbrt_mode == LO_CBRT_EXACT) ? lo_cbrt_exact(d->determ[i2])
+                                                          : cbrt(d->determ[i2]);
+        double coefficient = -hourg * 0.01 * ss1 * mass1 / volume13;
+
+        /* CalcElemFBHourglassForce, lulesh.cc:794-832 */
+        const double *vel[3] = {d->xd, d->yd, d->zd};
+        double *out[3] = {d->fx_elem, d->fy_elem, d->fz_elem};
+        for (int c = 0; c < 3; ++c) {
+            double vl[8], hxx[4];
+            gather8(nl, vel[c], vl);
+            for (int m = 0; m < 4; ++m) {
+                double acc = hourgam[0][m] * vl[0];
+                for (int j = 1; j < 8; ++j) acc = acc + hourgam[j][m] * vl[j];
+                hxx[m] = acc;
+            }
+            for (int j = 0; j < 8; ++j)
+                out[c][i3 + j] = coefficient *
+                    (hourgam[j][0] * hxx[0] + hourgam[j][1] * hxx[1] +
+                     hourgam[j][2] * hxx[2] + hourgam[j][3] * hxx[3]);
+        }
+    }
+    for (int n = 0; n < d->numNode; ++n) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int c = d->nodeElemStart[n]; c < d->nodeElemStart[n + 1]; ++c) {
+            int ce = d->nodeElemCornerList[c];
+            fx += d->fx_elem[ce];
+            fy += d->fy_elem[ce];
+            fz += d->fz_elem[ce];
+        }
+        d->fx[n] += fx; d->fy[n] += fy; d->fz[n] += fz;
+    }
+    return 0;
+}
+
+/* CalcForceForNodes (local part), lulesh.cc:1366-1397 */
+static int local_forces(lo_group *g, lo_dom *d)
+{
+    for (int i = 0; i < d->numNode; ++i) d->fx[i] = d->fy[i] = d->fz[i] = 0.0;
+    /* InitStressTermsForElems, lulesh.cc:315-337 */
+    for (int i = 0; i < d->numElem; ++i) d->sig[i] = -d->p[i] - d->q[i];
+    if (integrate_stress(d)) return -1;
+    if (hourglass(g, d)) return -1;
+    return 0;
+}
+
+/* CalcAccelerationForNodes .. CalcPositionForNodes, lulesh.cc:1415-1550 */
+static void node_update(lo_group *g, lo_dom *d)
+{
+    const double dt = d->deltatime, u_cut = g->u_cut;
+    for (int i = 0; i < d->numNode; ++i) {
+        double one_over_nMass = 1. / d->nodalMass[i];
+        d->xdd[i] = d->fx[i] * one_over_nMass;
+        d->ydd[i] = d->fy[i] * one_over_nMass;
+        d->zdd[i] = d->fz[i] * one_over_nMass;
+    }
+    int nbc = (d->nx + 1) * (d->nx + 1);
+    for (int i = 0; i < nbc; ++i) {
+        if (d->symmX) d->xdd[d->symmX[i]] = 0.0;
+        if (d->symmY) d->ydd[d->symmY[i]] = 0.0;
+        if (d->symmZ) d->zdd[d->symmZ[i]] = 0.0;
+    }
+    for (int i = 0; i < d->numNode; ++i) {
+        double t;
+        t = d->xd[i] + d->xdd[i] * dt; if (fabs(t) < u_cut) t = 0.0; d->xd[i] = t;
+        t = d->yd[i] + d->ydd[i] * dt; if (fabs(t) < u_cut) t = 0.0; d->yd[i] = t;
+        t = d->zd[i] + d->zdd[i] * dt; if (fabs(t) < u_cut) t = 0.0; d->zd[i] = t;
+    }
+    for (int i = 0; i < d->numNode; ++i) {
+        d->x[i] += d->xd[i] * dt;
+        d->y[i] += d->yd[i] * dt;
+        d->z[i] += d->zd[i] * dt;
+    }
+}
+
+/* =========================== element phase ============================= */
+
+/* CalcKinematicsForElems + CalcLagrangeElements, lulesh.cc:1861-2054 */
+static int kinematics(lo_group *g, lo_dom *d)
+{
+    int bad = 0;
+    const double dt2 = 0.5 * d->deltatime;
+    for (int k = 0; k < d->numElem; ++k) {
+        const int *nl = &d->nodelist[8 * k];
+        double xl[8], yl[8], zl[8], xdl[8], ydl[8], zdl[8], B[3][8];
+        gather8(nl, d->x, xl); gather8(nl, d->y, yl); gather8(nl, d->z, zl);
+        double volume = elem_volume(xl, yl, zl);
+        double rel = volume / d->volo[k];
+        d->vnew[k] = rel;
+        d->delv[k] = rel - d->v[k];
+        d->arealg[k] = char_length(xl, yl, zl, volume);
+        gather8(nl, d->xd, xdl); gather8(nl, d->yd, ydl); gather8(nl, d->zd, zdl);
+        for (int j = 0; j < 8; ++j) {
+            xl[j] -= dt2 * xdl[j];
+            yl[j] -= dt2 * ydl[j];
+            zl[j] -= dt2 * zdl[j];
+        }
+        double detJ = shape_derivs(xl, yl, zl, B);
+        /* CalcElemVelocityGradient, lulesh.cc:1796-1856 (diagonal terms) */
+        double inv_detJ = 1.0 / detJ;
+        double dxx = inv_detJ * (B[0][0] * (xdl[0] - xdl[6]) + B[0][1] * (xdl[1] - xdl[7]) +
+                                 B[0][2] * (xdl[2] - xdl[4]) + B[0][3] * (xdl[3] - xdl[5]));
+        double dyy = inv_detJ * (B[1][0] * (ydl[0] - ydl[6]) + B[1][1] * (ydl[1] - ydl[7]) +
+                                 B[1][2] * (ydl[2] - ydl[4]) + B[1][3] * (ydl[3] - ydl[5]));
+        double dzz = inv_detJ * (B[2][0] * (zdl[0] - zdl[6]) + B[2][1] * (zdl[1] - zdl[7]) +
+                                 B[2][2] * (zdl[2] - zdl[4]) + B[2][3] * (zdl[3] - zdl[5]));
+        d->vdov[k] = dxx + dyy + dzz;
+        if (d->vnew[k] <= 0.0) bad = 1;
+    }
+    return bad;
+}
+
+/* CalcMonotonicQGradientsForElems, lulesh.cc:2059-2244 */
+static void monoq_gradients(lo_dom *d)
+{
+    const double ptiny = 1.e-36;
+    for (int i = 0; i < d->numElem; ++i) {
+        const int *nl = &d->nodelist[8 * i];
+        double x[8], y[8], z[8], xv[8], yv[8], zv[8];
+        gather8(nl, d->x, x); gather8(nl, d->y, y); gather8(nl, d->z, z);
+        gather8(nl, d->xd, xv); gather8(nl, d->yd, yv); gather8(nl, d->zd, zv);
+        double vol = d->volo[i] * d->vnew[i];
+        double norm = 1.0 / (vol + ptiny);
+        double dxj = -0.25 * ((x[0] + x[1] + x[5] + x[4]) - (x[3] + x[2] + x[6] + x[7]));
+        double dyj = -0.25 * ((y[0] + y[1] + y[5] + y[4]) - (y[3] + y[2] + y[6] + y[7]));
+        double dzj = -0.25 * ((z[0] + z[1] + z[5] + z[4]) - (z[3] + z[2] + z[6] + z[7]));
+        double dxi = 0.25 * ((x[1] + x[2] + x[6] + x[5]) - (x[0] + x[3] + x[7] + x[4]));
+        double dyi = 0.25 * ((y[1] + y[2] + y[6] + y[5]) - (y[0] + y[3] + y[7] + y[4]));
+        double dzi = 0.25 * ((z[1] + z[2] + z[6] + z[5]) - (z[0] + z[3] + z[7] + z[4]));
+        double dxk = 0.25 * ((x[4] + x[5] + x[6] + x[7]) - (x[0] + x[1] + x[2] + x[3]));
+        double dyk = 0.25 * ((y[4] + y[5] + y[6] + y[7]) - (y[0] + y[1] + y[2] + y[3]));
+        double dzk = 0.25 * ((z[4] + z[5] + z[6] + z[7]) - (z[0] + z[1] + z[2] + z[3]));
+        double ax, ay, az, dxv, dyv, dzv;
+
+        ax = dyi * dzj - dzi * dyj; ay = dzi * dxj - dxi * dzj; az = dxi * dyj - dyi * dxj;
+        d->delx_zeta[i] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+        ax *= norm; ay *= norm; az *= norm;
+        dxv = 0.25 * ((xv[4] + xv[5] + xv[6] + xv[7]) - (xv[0] + xv[1] + xv[2] + xv[3]));
+        dyv = 0.25 * ((yv[4] + yv[5] + yv[6] + yv[7]) - (yv[0] + yv[1] + yv[2] + yv[3]));
+        dzv = 0.25 * ((zv[4] + zv[5] + zv[6] + zv[7]) - (zv[0] + zv[1] + zv[2] + zv[3]));
+        d->delv_zeta[i] = ax * dxv + ay * dyv + az * dzv;
+
+        ax = dyj * dzk - dzj * dyk; ay = dzj * dxk - dxj * dzk; az = dxj * dyk - dyj * dxk;
+        d->delx_xi[i] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+        ax *= norm; ay *= norm; az *= norm;
+        dxv = 0.25 * ((xv[1] + xv[2] + xv[6] + xv[5]) - (xv[0] + xv[3] + xv[7] + xv[4]));
+        dyv = 0.25 * ((yv[1] + yv[2] + yv[6] + yv[5]) - (yv[0] + yv[3] + yv[7] + yv[4]));
+        dzv = 0.25 * ((zv[1] + zv[2] + zv[6] + zv[5]) - (zv[0] + zv[3] + zv[7] + zv[4]));
+        d->delv_xi[i] = ax * dxv + ay * dyv + az * dzv;
+
+        ax = dyk * dzi - dzk * dyi; ay = dzk * dxi - dxk * dzi; az = dxk * dyi - dyk * dxi;
+        d->delx_eta[i] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+        ax *= norm; ay *= norm; az *= norm;
+        dxv = -0.25 * ((xv[0] + xv[1] + xv[5] + xv[4]) - (xv[3] + xv[2] + xv[6] + xv[7]));
+        dyv = -0.25 * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
+        dzv = -0.25 * ((zv[0] + zv[1] + zv[5] + zv[4]) - (zv[3] + zv[2] + zv[6] + zv[7]));
+        d->delv_eta[i] = ax * dxv + ay * dyv + az * dzv;
+    }
+}
+
+static void monoq_region(lo_group *g, lo_dom *d)
+{
+    const double ptiny = 1.e-36;
+    const double mult = g->monoq_limiter_mult, max_slope = g->monoq_max_slope;
+    const double qlc = g->qlc_monoq, qqc = g->qqc_monoq;
+    for (int i = 0; i < d->numElem; ++i) {
+        int bc = d->elemBC[i];
+        double delvm = 0.0, delvp = 0.0, norm, phixi, phieta, phizeta;
+
+        norm = 1. / (d->delv_xi[i] + ptiny);
+        switch (bc & XI_M) {
+        case XI_M_COMM: case 0: delvm = d->delv_xi[d->lxim[i]]; break;
+        case XI_M_SYMM: delvm = d->delv_xi[i]; break;
+        case XI_M_FREE: delvm = 0.0; break;
+        default: delvm = 0; break;
+        }
+        switch (bc & XI_P) {
+        case XI_P_COMM: case 0: delvp = d->delv_xi[d->lxip[i]]; break;
+        case XI_P_SYMM: delvp = d->delv_xi[i]; break;
+        case XI_P_FREE: delvp = 0.0; break;
+        default: delvp = 0; break;
+        }
+        delvm = delvm * norm; delvp = delvp * norm;
+        phixi = .5 * (delvm + delvp);
+        delvm *= mult; delvp *= mult;
+        if (delvm < phixi) phixi = delvm;
+        if (delvp < phixi) phixi = delvp;
+        if (phixi < 0.) phixi = 0.;
+        if (phixi > max_slope) phixi = max_slope;
+
+        norm = 1. / (d->delv_eta[i] + ptiny);
+        switch (bc & ETA_M) {
+        case ETA_M_COMM: case 0: delvm = d->delv_eta[d->letam[i]]; break;
+        case ETA_M_SYMM: delvm = d->delv_eta[i]; break;
+        case ETA_M_FREE: delvm = 0.0; break;
+        default: delvm = 0; break;
+        }
+        switch (bc & ETA_P) {
+        case ETA_P_COMM: case 0: delvp = d->delv_eta[d->letap[i]]; break;
+        case ETA_P_SYMM: delvp = d->delv_eta[i]; break;
+        case ETA_P_FREE: delvp = 0.0; break;
+        default: delvp = 0; break;
+        }
+        delvm = delvm * norm; delvp = delvp * norm;
+        phieta = .5 * (delvm + delvp);
+        delvm *= mult; delvp *= mult;
+        if (delvm < phieta) phieta = delvm;
+        if (delvp < phieta) phieta = delvp;
+        if (phieta < 0.) phieta = 0.;
+        if (phieta > max_slope) phieta = max_slope;
+
+        norm = 1. / (d->delv_zeta[i] + ptiny);
+        switch (bc & ZETA_M) {
+        case ZETA_M_COMM: case 0: delvm = d->delv_zeta[d->lzetam[i]]; break;
+        case ZETA_M_SYMM: delvm = d->delv_zeta[i]; break;
+        case ZETA_M_FREE: delvm = 0.0; break;
+        default: delvm = 0; break;
+        }
+        switch (bc & ZETA_P) {
+        case ZETA_P_COMM: case 0: delvp = d->delv_zeta[d->lzetap[i]]; break;
+        case ZETA_P_SYMM: delvp = d->delv_zeta[i]; break;
+        case ZETA_P_FREE: delvp = 0.0; break;
+        default: delvp = 0; break;
+        }
+        delvm = delvm * norm; delvp = delvp * norm;
+        phizeta = .5 * (delvm + delvp);
+        delvm *= mult; delvp *= mult;
+        if (delvm < phizeta) phizeta = delvm;
+        if (delvp < phizeta) phizeta = delvp;
+        if (phizeta < 0.) phizeta = 0.;
+        if (phizeta > max_slope) phizeta = max_slope;
+
+        double qlin, qquad;
+        if (d->vdov[i] > 0.) {
+            qlin = 0.; qquad = 0.;
+        } else {
+            double dxi = d->delv_xi[i] * d->delx_xi[i];
+            double deta = d->delv_eta[i] * d->delx_eta[i];
+            double dzeta = d->delv_zeta[i] * d->delx_zeta[i];
+            if (dxi > 0.) dxi = 0.;
+            if (deta > 0.) deta = 0.;
+            if (dzeta > 0.) dzeta = 0.;
+            double rho = d->elemMass[i] / (d->volo[i] * d->vnew[i]);
+            qlin = -qlc * rho * (dxi * (1. - phixi) + deta * (1. - phieta) + dzeta * (1. - phizeta));
+            qquad = qqc * rho * (dxi * dxi * (1. - phixi * phixi) +
+                                 deta * deta * (1. - phieta * phieta) +
+                                 dzeta * dzeta * (1. - phizeta * phizeta));
+        }
+        d->qq[i] = qquad;
+        d->ql[i] = qlin;
+    }
+}
+
+/* qstop scan of CalcQForElems, lulesh.cc:2531-2546 */
+static int qstop_check(lo_group *g, lo_dom *d)
+{
+    for (int i = 0; i < d->numElem; ++i)
+        if (d->q[i] > g->qstop) return 1;
+    return 0;
+}
+
+static double ss_from(double pbvc, double e_new, double vv, double bvc, double p_new, double rho0)
+{
+    double ssc = (pbvc * e_new + vv * vv * bvc * p_new) / rho0;
+    if (ssc <= .1111111e-36) ssc = .3333333e-18;
+    else ssc = sqrt(ssc);
+    return ssc;
+}
+
+/* EvalEOSForElems (+ the inlined CalcEnergyForElems / CalcPressure /
+ * CalcSoundSpeed / UpdateVolumes), lulesh.cc:2586-2836.  Every rep
+ * iteration recomputes the same values from the same inputs. */
+static void eval_eos(lo_group *g, lo_dom *d)
+{
+    const double e_cut = g->e_cut, p_cut = g->p_cut, q_cut = g->q_cut, v_cut = g->v_cut;
+    const double eosvmax = g->eosvmax, eosvmin = g->eosvmin, pmin = g->pmin, emin = g->emin;
+    const double rho0 = g->refdens;
+    const double c1s = 2.0 / 3.0, sixth = 1.0 / 6.0;
+    for (int elem = 0; elem < d->numElem; ++elem) {
+        int rep = d->elemRep[elem];
+        if (eosvmin != 0.) { if (d->vnew[elem] < eosvmin) d->vnew[elem] = eosvmin; }
+        if (eosvmax != 0.) { if (d->vnew[elem] > eosvmax) d->vnew[elem] = eosvmax; }
+        double vnewc = d->vnew[elem];
+        double e_t = d->e[elem], dv_t = d->delv[elem], p_t = d->p[elem], q_t = d->q[elem];
+        double qq_t = d->qq[elem], ql_t = d->ql[elem];
+        double p_new = 0, q_new = 0, e_new = 0, bvc = 0, pbvc = 0;
+        for (int j = 0; j < rep; ++j) {
+            double e_old = e_t, delvc = dv_t, p_old = p_t, q_old = q_t, qq_old = qq_t, ql_old = ql_t;
+            double compression = 1. / vnewc - 1.;
+            double vchalf = vnewc - delvc * .5;
+            double compHalfStep = 1. / vchalf - 1.;
+            if (vnewc <= eosvmin) compHalfStep = compression;
+            if (vnewc >= eosvmax) { p_old = 0.; compression = 0.; compHalfStep = 0.; }
+            double work = 0.;
+            e_new = e_old - 0.5 * delvc * (p_old + q_old) + 0.5 * work;
+            if (e_new < emin) e_new = emin;
+            bvc = c1s * (compHalfStep + 1.);
+            pbvc = c1s;
+            double pHalfStep = bvc * e_new;
+            if (fabs(pHalfStep) < p_cut) pHalfStep = 0.0;
+            if (vnewc >= eosvmax) pHalfStep = 0.0;
+            if (pHalfStep < pmin) pHalfStep = pmin;
+            double vhalf = 1. / (1. + compHalfStep);
+            if (delvc > 0.) {
+                q_new = 0.;
+            } else {
+                double ssc = (pbvc * e_new + vhalf * vhalf * bvc * pHalfStep) / rho0;
+                if (ssc <= .1111111e-36) ssc = .3333333e-18;
+                else ssc = sqrt(ssc);
+                q_new = (ssc * ql_old + qq_old);
+            }
+            e_new = e_new + 0.5 * delvc * (3.0 * (p_old + q_old) - 4.0 * (pHalfStep + q_new));
+            e_new += 0.5 * work;
+            if (fabs(e_new) < e_cut) e_new = 0.;
+            if (e_new < emin) e_new = emin;
+            bvc = c1s * (compression + 1.);
+            pbvc = c1s;
+            p_new = bvc * e_new;
+            if (fabs(p_new) < p_cut) p_new = 0.0;
+            if (vnewc >= eosvmax) p_new = 0.0;
+            if (p_new < pmin) p_new = pmin;
+            double q_tilde;
+            if (delvc > 0.) {
+                q_tilde = 0.;
+            } else {
+                q_tilde = ss_from(pbvc, e_new, vnewc, bvc, p_new, rho0) * ql_old + qq_old;
+            }
+            e_new = e_new - (7.0 * (p_old + q_old) - 8.0 * (pHalfStep + q_new) + (p_new + q_tilde)) * delvc * sixth;
+            if (fabs(e_new) < e_cut) e_new = 0.;
+            if (e_new < emin) e_new = emin;
+            bvc = c1s * (compression + 1.);
+            pbvc = c1s;
+            p_new = bvc * e_new;
+            if (fabs(p_new) < p_cut) p_new = 0.0;
+            if (vnewc >= eosvmax) p_new = 0.0;
+            if (p_new < pmin) p_new = pmin;
+            if (delvc <= 0.) {
+                q_new = ss_from(pbvc, e_new, vnewc, bvc, p_new, rho0) * ql_old + qq_old;
+                if (fabs(q_new) < q_cut) q_new = 0.;
+            }
+        }
+        d->p[elem] = p_new;
+        d->e[elem] = e_new;
+        d->q[elem] = q_new;
+        d->ss[elem] = ss_from(pbvc, e_new, vnewc, bvc, p_new, rho0);
+        if (fabs(vnewc - 1.0) < v_cut) vnewc = 1.0;
+        d->v[elem] = vnewc;
+    }
+}
+
+/* CalcCourantConstraintForElems + CalcHydroConstraintForElems over every
+ * region, lulesh.cc:2874-3035.  The per-region, per-thread minima of the
+ * reference reduce to the minimum over all elements with vdov != 0. */
+static void time_constraints(lo_group *g, lo_dom *d)
+{
+    d->dtcourant = 1.0e+20;
+    d->dthydro = 1.0e+20;
+    const double qqc2 = 64.0 * g->qqc * g->qqc;
+    for (int r = 0; r < d->numReg; ++r) {
+        double dtc = d->dtcourant, dth = d->dthydro;
+        for (int t = 0; t < d->regElemSize[r]; ++t) {
+            int i = d->regElemlist[r][t];
+            double dtf = d->ss[i] * d->ss[i];
+            if (d->vdov[i] < 0.)
+                dtf = dtf + qqc2 * d->arealg[i] * d->arealg[i] * d->vdov[i] * d->vdov[i];
+            dtf = sqrt(dtf);
+            dtf = d->arealg[i] / dtf;
+            if (d->vdov[i] != 0.) { if (dtf < dtc) dtc = dtf; }
+        }
+        if (dtc < d->dtcourant) d->dtcourant = dtc;
+        for (int t = 0; t < d->regElemSize[r]; ++t) {
+            int i = d->regElemlist[r][t];
+            if (d->vdov[i] != 0.) {
+                double dtdvov = g->dvovmax / (fabs(d->vdov[i]) + 1.e-20);
+                if (dth > dtdvov) dth = dtdvov;
+            }
+        }
+        if (dth < d->dthydro) d->dthydro = dth;
+    }
+}
+
+/* ============================ halo exchange ============================ */
+
+typedef struct { int dc, dr, dp; int gate; } lo_dir;  /* gate: 1 = needs doRecv/doSend */
+
+/* Receive slot order of CommRecv (lulesh-comm.cc:59-353) and the
+ * receiver-offset helper shmRecvOffset (cuda comm/comm_backend.h:75-138):
+ * 6 faces, 12 edges, 8 corners.  A direction is the sender's position
+ * relative to the receiver. */
+static const lo_dir kRecvOrder[26] = {
+    {0, 0, -1, 1}, {0, 0, 1, 0}, {0, -1, 0, 1}, {0, 1, 0, 0}, {-1, 0, 0, 1}, {1, 0, 0, 0},
+    {-1, -1, 0, 1}, {0, -1, -1, 1}, {-1, 0, -1, 1}, {1, 1, 0, 0}, {0, 1, 1, 0}, {1, 0, 1, 0},
+    {-1, 1, 0, 0}, {0, -1, 1, 0}, {-1, 0, 1, 0}, {1, -1, 0, 1}, {0, 1, -1, 1}, {1, 0, -1, 1},
+    {-1, -1, -1, 1}, {-1, -1, 1, 0}, {1, -1, -1, 1}, {1, -1, 1, 0},
+    {-1, 1, -1, 1}, {-1, 1, 1, 0}, {1, 1, -1, 1}, {1, 1, 1, 0}
+};
+
+/* Send order of CommSend (lulesh-comm.cc:357-844); direction = receiver's
+ * position relative to the sender; gate = needs doSend. */
+static const lo_dir kSendOrder[26] = {
+    {0, 0, -1, 0}, {0, 0, 1, 1}, {0, -1, 0, 0}, {0, 1, 0, 1}, {-1, 0, 0, 0}, {1, 0, 0, 1},
+    {-1, -1, 0, 0}, {0, -1, -1, 0}, {-1, 0, -1, 0}, {1, 1, 0, 1}, {0, 1, 1, 1}, {1, 0, 1, 1},
+    {-1, 1, 0, 1}, {0, -1, 1, 1}, {-1, 0, 1, 1}, {1, -1, 0, 0}, {0, 1, -1, 0}, {1, 0, -1, 0},
+    {-1, -1, -1, 0}, {-1, -1, 1, 1}, {1, -1, -1, 0}, {1, -1, 1, 1},
+    {-1, 1, -1, 0}, {-1, 1, 1, 1}, {1, 1, -1, 0}, {1, 1, 1, 1}
+};
+
+static int has_nbr(const lo_dom *d, int dc, int dr, int dp)
+{
+    if (dc < 0 && !d->colMin) return 0;
+    if (dc > 0 && !d->colMax) return 0;
+    if (dr < 0 && !d->rowMin) return 0;
+    if (dr > 0 && !d->rowMax) return 0;
+    if (dp < 0 && !d->planeMin) return 0;
+    if (dp > 0 && !d->planeMax) return 0;
+    return 1;
+}
+
+static int ndims(const lo_dir *o) { return (o->dc != 0) + (o->dr != 0) + (o->dp != 0); }
+
+/* Indices (ascending) of the block boundary facing direction (dc,dr,dp) in
+ * a dx*dy*dz index box, i.e. the index maps of lulesh-comm.cc (e.g. face
+ * rowMin: i*dx*dy + j; edge rowMax&colMax: dx*dy-1 + i*dx*dy). */
+static int boundary_set(int dx, int dy, int dz, int dc, int dr, int dp, int *out)
+{
+    int c0 = dc < 0 ? 0 : dc > 0 ? dx - 1 : 0, c1 = dc ? c0 + 1 : dx;
+    int r0 = dr < 0 ? 0 : dr > 0 ? dy - 1 : 0, r1 = dr ? r0 + 1 : dy;
+    int p0 = dp < 0 ? 0 : dp > 0 ? dz - 1 : 0, p1 = dp ? p0 + 1 : dz;
+    int n = 0;
+    for (int p = p0; p < p1; ++p)
+        for (int r = r0; r < r1; ++r)
+            for (int c = c0; c < c1; ++c) out[n++] = p * dx * dy + r * dx + c;
+    return n;
+}
+
+/* shmRecvOffset: where, in recv_dom's receive buffer, the message from
+ * direction (dc,dr,dp) lands; -1 if none is expected. */
+int lo_recv_offset(const lo_group *g, int recv_dom, int dc, int dr, int dp,
+                   int xfer, int doRecv, int planeOnly)
+{
+    const lo_dom *d = &g->d[recv_dom];
+    int maxPlaneComm = xfer * d->maxPlaneSize, maxEdgeComm = xfer * d->maxEdgeSize;
+    int pmsg = 0, emsg = 0, cmsg = 0;
+    for (int s = 0; s < 26; ++s) {
+        const lo_dir *o = &kRecvOrder[s];
+        int nd = ndims(o);
+        if (nd > 1 && planeOnly) break;
+        if (!has_nbr(d, o->dc, o->dr, o->dp) || (o->gate && !doRecv)) continue;
+        int off = pmsg * maxPlaneComm + emsg * maxEdgeComm + cmsg * PAD_REAL;
+        if (o->dc == dc && o->dr == dr && o->dp == dp) return off;
+        if (nd == 1) ++pmsg; else if (nd == 2) ++emsg; else ++cmsg;
+    }
+    return -1;
+}
+
+int lo_com_buf_size(const lo_group *g, int dom) { return g->d[dom].comBufSize; }
+
+static const double *field_ptr(const lo_dom *d, int f);
+static double *field_ptr_mut(lo_dom *d, int f) { return (double *)field_ptr(d, f); }
+
+static int g_scratch_idx[1 << 20];
+
+/* CommSend for every subdomain (pack), then the unpack of every subdomain. */
+int lo_exchange(lo_group *g, int msg_type, int nfields, const int *fields)
+{
+    if (g->ndom == 1) return 0;
+    int doFlag = (msg_type == MSG_COMM_SBN || msg_type == MSG_MONOQ);
+    int planeOnly = (msg_type == MSG_MONOQ);
+    int elemMsg = (msg_type == MSG_MONOQ);
+    int n1 = elemMsg ? g->nx : g->nx + 1;        /* dx = dy = dz */
+    int tp = g->tp;
+    int *idx = g_scratch_idx;
+
+    /* pack: CommSend, lulesh-comm.cc:357-844 */
+    for (int s = 0; s < g->ndom; ++s) {
+        lo_dom *src = &g->d[s];
+        for (int k = 0; k < 26; ++k) {
+            const lo_dir *o = &kSendOrder[k];
+            if (planeOnly && ndims(o) > 1) break;
+            if (!has_nbr(src, o->dc, o->dr, o->dp) || (o->gate && !doFlag)) continue;
+            int r = s + o->dc + o->dr * tp + o->dp * tp * tp;
+            int off = lo_recv_offset(g, r, -o->dc, -o->dr, -o->dp, nfields, doFlag, planeOnly);
+            if (off < 0) return -3;   /* shmRecvOffset abort */
+            int cnt = boundary_set(n1, n1, n1, o->dc, o->dr, o->dp, idx);
+            double *buf = g->d[r].commDataRecv + off;
+            for (int f = 0; f < nfields; ++f) {
+                const double *fp = field_ptr(src, fields[f]);
+                if (cnt == 1) { buf[f] = fp[idx[0]]; continue; }  /* corner */
+                for (int i = 0; i < cnt; ++i) buf[f * cnt + i] = fp[idx[i]];
+            }
+        }
+    }
+
+    /* unpack: CommSBN / CommSyncPosVel / CommMonoQ, lulesh-comm.cc:848-1837 */
+    for (int r = 0; r < g->ndom; ++r) {
+        lo_dom *dst = &g->d[r];
+        int ghost = dst->numElem;
+        for (int k = 0; k < 26; ++k) {
+            const lo_dir *o = &kRecvOrder[k];
+            if (planeOnly && ndims(o) > 1) break;
+            if (!has_nbr(dst, o->dc, o->dr, o->dp) || (o->gate && !doFlag)) continue;
+            int off = lo_recv_offset(g, r, o->dc, o->dr, o->dp, nfields, doFlag, planeOnly);
+            int cnt = boundary_set(n1, n1, n1, o->dc, o->dr, o->dp, idx);
+            const double *buf = dst->commDataRecv + off;
+            for (int f = 0; f < nfields; ++f) {
+                double *fp = field_ptr_mut(dst, fields[f]);
+                const double *src = (cnt == 1) ? buf + f : buf + f * cnt;
+                if (msg_type == MSG_COMM_SBN) {
+                    for (int i = 0; i < cnt; ++i) fp[idx[i]] += src[i];
+                } else if (msg_type == MSG_SYNC_POS_VEL) {
+                    for (int i = 0; i < cnt; ++i) fp[idx[i]] = src[i];
+                } else {
+                    for (int i = 0; i < cnt; ++i) fp[ghost + i] = src[i];
+                }
+            }
+            if (msg_type == MSG_MONOQ) ghost += cnt;
+        }
+    }
+    return 0;
+}
+
+int lo_get_recv_buffer(const lo_group *g, int dom, double *out)
+{
+    memcpy(out, g->d[dom].commDataRecv, sizeof(double) * (size_t)g->d[dom].comBufSize);
+    return g->d[dom].comBufSize;
+}
+
+/* ================================ set-up ================================ */
+
+/* Domain::BuildMesh, lulesh-init.cc:189-238 */
+static void build_mesh(lo_dom *d)
+{
+    int nx = d->nx, en = nx + 1, meshEdgeElems = d->tp * nx;
+    int nidx = 0;
+    double tz = 1.125 * (double)(d->plane * nx) / (double)meshEdgeElems;
+    for (int plane = 0; plane < en; ++plane) {
+        double ty = 1.125 * (double)(d->row * nx) / (double)meshEdgeElems;
+        for (int row = 0; row < en; ++row) {
+            double tx = 1.125 * (double)(d->col * nx) / (double)meshEdgeElems;
+            for (int col = 0; col < en; ++col) {
+                d->x[nidx] = tx; d->y[nidx] = ty; d->z[nidx] = tz;
+                ++nidx;
+                tx = 1.125 * (double)(d->col * nx + col + 1) / (double)meshEdgeElems;
+            }
+            ty = 1.125 * (double)(d->row * nx + row + 1) / (double)meshEdgeElems;
+        }
+        tz = 1.125 * (double)(d->plane * nx + plane + 1) / (double)meshEdgeElems;
+    }
+    int zidx = 0;
+    nidx = 0;
+    for (int plane = 0; plane < nx; ++plane) {
+        for (int row = 0; row < nx; ++row) {
+            for (int col = 0; col < nx; ++col) {
+                int *ln = &d->nodelist[8 * zidx];
+                ln[0] = nidx;
+                ln[1] = nidx + 1;
+                ln[2] = nidx + en + 1;
+                ln[3] = nidx + en;
+                ln[4] = nidx + en * en;
+                ln[5] = nidx + en * en + 1;
+                ln[6] = nidx + en * en + en + 1;
+                ln[7] = nidx + en * en + en;
+                ++zidx; ++nidx;
+            }
+            ++nidx;
+        }
+        nidx += en;
+    }
+}
+
+/* Domain::SetupThreadSupportStructures, lulesh-init.cc:243-313 */
+static void setup_node_elem(lo_dom *d)
+{
+    int *cnt = ialloc(d->numNode);
+    for (int i = 0; i < d->numElem; ++i)
+        for (int j = 0; j < 8; ++j) ++cnt[d->nodelist[8 * i + j]];
+    d->nodeElemStart = ialloc(d->numNode + 1);
+    for (int i = 1; i <= d->numNode; ++i) d->nodeElemStart[i] = d->nodeElemStart[i - 1] + cnt[i - 1];
+    d->nodeElemCornerList = ialloc(d->nodeElemStart[d->numNode]);
+    memset(cnt, 0, sizeof(int) * d->numNode);
+    for (int i = 0; i < d->numElem; ++i)
+        for (int j = 0; j < 8; ++j) {
+            int m = d->nodelist[8 * i + j];
+            d->nodeElemCornerList[d->nodeElemStart[m] + cnt[m]] = 8 * i + j;
+            ++cnt[m];
+        }
+    free(cnt);
+}
+
+/* Domain::CreateRegionIndexSets, lulesh-init.cc:377-484 (glibc rand()),
+ * plus the elemRep assignment of main, lulesh.cc:3171-3192. */
+static void create_regions(lo_dom *d, int nr, int balance, int cost)
+{
+    srand(d->rank);
+    int myRank = d->rank;
+    d->numReg = nr;
+    d->cost = cost;
+    d->regElemSize = ialloc(nr);
+    d->regElemlist = (int **)calloc(nr, sizeof(int *));
+    d->regNumList = ialloc(d->numElem);
+    int nextIndex = 0;
+    if (nr == 1) {
+        while (nextIndex < d->numElem) d->regNumList[nextIndex++] = 1;
+        d->regElemSize[0] = 0;
+    } else {
+        int lastReg = -1, costDenominator = 0, runto, elements;
+        int *regBinEnd = ialloc(nr);
+        for (int i = 0; i < nr; ++i) {
+            d->regElemSize[i] = 0;
+            costDenominator += pow((i + 1), balance);
+            regBinEnd[i] = costDenominator;
+        }
+        while (nextIndex < d->numElem) {
+            int regionVar = rand() % costDenominator;
+            int i = 0;
+            while (regionVar >= regBinEnd[i]) i++;
+            int regionNum = ((i + myRank) % nr) + 1;
+            while (regionNum == lastReg) {
+                regionVar = rand() % costDenominator;
+                i = 0;
+                while (regionVar >= regBinEnd[i]) i++;
+                regionNum = ((i + myRank) % nr) + 1;
+            }
+            int binSize = rand() % 1000;
+            if (binSize < 773) elements = rand() % 15 + 1;
+            else if (binSize < 937) elements = rand() % 16 + 16;
+            else if (binSize < 970) elements = rand() % 32 + 32;
+            else if (binSize < 974) elements = rand() % 64 + 64;
+            else if (binSize < 978) elements = rand() % 128 + 128;
+            else if (binSize < 981) elements = rand() % 256 + 256;
+            else elements = rand() % 1537 + 512;
+            runto = elements + nextIndex;
+            while (nextIndex < runto && nextIndex < d->numElem) d->regNumList[nextIndex++] = regionNum;
+            lastReg = regionNum;
+        }
+        free(regBinEnd);
+    }
+    for (int i = 0; i < d->numElem; ++i) d->regElemSize[d->regNumList[i] - 1]++;
+    for (int i = 0; i < nr; ++i) {
+        d->regElemlist[i] = ialloc(d->regElemSize[i]);
+        d->regElemSize[i] = 0;
+    }
+    for (int i = 0; i < d->numElem; ++i) {
+        int r = d->regNumList[i] - 1;
+        d->regElemlist[r][d->regElemSize[r]++] = i;
+    }
+    d->elemRep = ialloc(d->numElem);
+    for (int r = 0; r < nr; ++r) {
+        int rep;
+        if (r < nr / 2) rep = 1;
+        else if (r < (nr - (nr + 15) / 20)) rep = 1 + cost;
+        else rep = 10 * (1 + cost);
+        for (int e = 0; e < d->regElemSize[r]; ++e) d->elemRep[d->regElemlist[r][e]] = rep;
+    }
+}
+
+/* SetupSymmetryPlanes / SetupElementConnectivities / SetupBoundaryConditions,
+ * lulesh-init.cc:487-631 */
+static void setup_bc(lo_dom *d)
+{
+    int nx = d->nx, en = nx + 1, ne = d->numElem;
+    int nidx = 0;
+    for (int i = 0; i < en; ++i) {
+        int planeInc = i * en * en, rowInc = i * en;
+        for (int j = 0; j < en; ++j) {
+            if (d->symmZ) d->symmZ[nidx] = rowInc + j;
+            if (d->symmY) d->symmY[nidx] = planeInc + j;
+            if (d->symmX) d->symmX[nidx] = planeInc + j * en;
+            ++nidx;
+        }
+    }
+    d->lxim[0] = 0;
+    for (int i = 1; i < ne; ++i) { d->lxim[i] = i - 1; d->lxip[i - 1] = i; }
+    d->lxip[ne - 1] = ne - 1;
+    for (int i = 0; i < nx; ++i) { d->letam[i] = i; d->letap[ne - nx + i] = ne - nx + i; }
+    for (int i = nx; i < ne; ++i) { d->letam[i] = i - nx; d->letap[i - nx] = i; }
+    for (int i = 0; i < nx * nx; ++i) { d->lzetam[i] = i; d->lzetap[ne - nx * nx + i] = ne - nx * nx + i; }
+    for (int i = nx * nx; i < ne; ++i) { d->lzetam[i] = i - nx * nx; d->lzetap[i - nx * nx] = i; }
+
+    int ghostIdx[6];
+    for (int i = 0; i < ne; ++i) d->elemBC[i] = 0;
+    for (int i = 0; i < 6; ++i) ghostIdx[i] = INT_MIN;
+    int pidx = ne;
+    if (d->planeMin) { ghostIdx[0] = pidx; pidx += nx * nx; }
+    if (d->planeMax) { ghostIdx[1] = pidx; pidx += nx * nx; }
+    if (d->rowMin) { ghostIdx[2] = pidx; pidx += nx * nx; }
+    if (d->rowMax) { ghostIdx[3] = pidx; pidx += nx * nx; }
+    if (d->colMin) { ghostIdx[4] = pidx; pidx += nx * nx; }
+    if (d->colMax) { ghostIdx[5] = pidx; }
+    for (int i = 0; i < nx; ++i) {
+        int planeInc = i * nx * nx, rowInc = i * nx;
+        for (int j = 0; j < nx; ++j) {
+            if (d->plane == 0) d->elemBC[rowInc + j] |= ZETA_M_SYMM;
+            else { d->elemBC[rowInc + j] |= ZETA_M_COMM; d->lzetam[rowInc + j] = ghostIdx[0] + rowInc + j; }
+            if (d->plane == d->tp - 1) d->elemBC[rowInc + j + ne - nx * nx] |= ZETA_P_FREE;
+            else { d->elemBC[rowInc + j + ne - nx * nx] |= ZETA_P_COMM;
+                   d->lzetap[rowInc + j + ne - nx * nx] = ghostIdx[1] + rowInc + j; }
+            if (d->row == 0) d->elemBC[planeInc + j] |= ETA_M_SYMM;
+            else { d->elemBC[planeInc + j] |= ETA_M_COMM; d->letam[planeInc + j] = ghostIdx[2] + rowInc + j; }
+            if (d->row == d->tp - 1) d->elemBC[planeInc + j + nx * nx - nx] |= ETA_P_FREE;
+            else { d->elemBC[planeInc + j + nx * nx - nx] |= ETA_P_COMM;
+                   d->letap[planeInc + j + nx * nx - nx] = ghostIdx[3] + rowInc + j; }
+            if (d->col == 0) d->elemBC[planeInc + j * nx] |= XI_M_SYMM;
+            else { d->elemBC[planeInc + j * nx] |= XI_M_COMM; d->lxim[planeInc + j * nx] = ghostIdx[4] + rowInc + j; }
+            if (d->col == d->tp - 1) d->elemBC[planeInc + j * nx + nx - 1] |= XI_P_FREE;
+            else { d->elemBC[planeInc + j * nx + nx - 1] |= XI_P_COMM;
+                   d->lxip[planeInc + j * nx + nx - 1] = ghostIdx[5] + rowInc + j; }
+        }
+    }
+}
+
+static void init_dom(lo_group *g, lo_dom *d, int rank, int nr, int balance, int cost)
+{
+    int nx = g->nx, tp = g->tp;
+    d->rank = rank; d->tp = tp; d->nx = nx;
+    /* InitMeshDecomp, lulesh-init.cc:635-712 (one domain per rank) */
+    d->col = rank % tp; d->row = (rank / tp) % tp; d->plane = rank / (tp * tp);
+    d->numElem = nx * nx * nx;
+    d->numNode = (nx + 1) * (nx + 1) * (nx + 1);
+    d->allElem = d->numElem + 6 * nx * nx;
+    int ne = d->numElem, nn = d->numNode;
+
+    d->x = dalloc(nn); d->y = dalloc(nn); d->z = dalloc(nn);
+    d->xd = dalloc(nn); d->yd = dalloc(nn); d->zd = dalloc(nn);
+    d->xdd = dalloc(nn); d->ydd = dalloc(nn); d->zdd = dalloc(nn);
+    d->fx = dalloc(nn); d->fy = dalloc(nn); d->fz = dalloc(nn);
+    d->nodalMass = dalloc(nn);
+    d->nodelist = ialloc(8 * (size_t)ne);
+    d->lxim = ialloc(ne); d->lxip = ialloc(ne); d->letam = ialloc(ne);
+    d->letap = ialloc(ne); d->lzetam = ialloc(ne); d->lzetap = ialloc(ne);
+    d->elemBC = ialloc(ne);
+    d->e = dalloc(ne); d->p = dalloc(ne); d->q = dalloc(ne); d->ql = dalloc(ne); d->qq = dalloc(ne);
+    d->v = dalloc(ne); d->volo = dalloc(ne); d->delv = dalloc(ne); d->vdov = dalloc(ne);
+    d->arealg = dalloc(ne); d->ss = dalloc(ne); d->elemMass = dalloc(ne); d->vnew = dalloc(ne);
+    d->delv_xi = dalloc(d->allElem); d->delv_eta = dalloc(d->allElem); d->delv_zeta = dalloc(d->allElem);
+    d->delx_xi = dalloc(ne); d->delx_eta = dalloc(ne); d->delx_zeta = dalloc(ne);
+    d->sig = dalloc(ne); d->determ = dalloc(ne);
+    d->fx_elem = dalloc(8 * (size_t)ne); d->fy_elem = dalloc(8 * (size_t)ne); d->fz_elem = dalloc(8 * (size_t)ne);
+    d->dvdx = dalloc(8 * (size_t)ne); d->dvdy = dalloc(8 * (size_t)ne); d->dvdz = dalloc(8 * (size_t)ne);
+    d->x8n = dalloc(8 * (size_t)ne); d->y8n = dalloc(8 * (size_t)ne); d->z8n = dalloc(8 * (size_t)ne);
+
+    /* Domain::SetupCommBuffers, lulesh-init.cc:317-372 */
+    int maxEdge = nx + 1;
+    d->maxPlaneSize = ALIGN_REAL(maxEdge * maxEdge);
+    d->maxEdgeSize = ALIGN_REAL(maxEdge);
+    d->rowMin = d->row != 0; d->rowMax = d->row != tp - 1;
+    d->colMin = d->col != 0; d->colMax = d->col != tp - 1;
+    d->planeMin = d->plane != 0; d->planeMax = d->plane != tp - 1;
+    int comBufSize = (d->rowMin + d->rowMax + d->colMin + d->colMax + d->planeMin + d->planeMax) *
+                     d->maxPlaneSize * MAX_FIELDS_PER_MPI_COMM;
+    comBufSize += ((d->rowMin & d->colMin) + (d->rowMin & d->planeMin) + (d->colMin & d->planeMin) +
+                   (d->rowMax & d->colMax) + (d->rowMax & d->planeMax) + (d->colMax & d->planeMax) +
+                   (d->rowMax & d->colMin) + (d->rowMin & d->planeMax) + (d->colMin & d->planeMax) +
+                   (d->rowMin & d->colMax) + (d->rowMax & d->planeMin) + (d->colMax & d->planeMin)) *
+                  d->maxEdgeSize * MAX_FIELDS_PER_MPI_COMM;
+    comBufSize += ((d->rowMin & d->colMin & d->planeMin) + (d->rowMin & d->colMin & d->planeMax) +
+                   (d->rowMin & d->colMax & d->planeMin) + (d->rowMin & d->colMax & d->planeMax) +
+                   (d->rowMax & d->colMin & d->planeMin) + (d->rowMax & d->colMin & d->planeMax) +
+                   (d->rowMax & d->colMax & d->planeMin) + (d->rowMax & d->colMax & d->planeMax)) *
+                  PAD_REAL;
+    d->comBufSize = comBufSize;
+    d->commDataRecv = dalloc(comBufSize);
+    if (d->col == 0) d->symmX = ialloc((size_t)(nx + 1) * (nx + 1));
+    if (d->row == 0) d->symmY = ialloc((size_t)(nx + 1) * (nx + 1));
+    if (d->plane == 0) d->symmZ = ialloc((size_t)(nx + 1) * (nx + 1));
+
+    /* Domain::Domain field initialisation, lulesh-init.cc:71-101 */
+    for (int i = 0; i < ne; ++i) d->v[i] = 1.0;
+
+    build_mesh(d);
+    setup_node_elem(d);
+    create_regions(d, nr, balance, cost);
+    setup_bc(d);
+
+    /* initial volumes and masses, lulesh-init.cc:151-168 */
+    for (int i = 0; i < ne; ++i) {
+        double xl[8], yl[8], zl[8];
+        const int *nl = &d->nodelist[8 * i];
+        gather8(nl, d->x, xl); gather8(nl, d->y, yl); gather8(nl, d->z, zl);
+        double volume = elem_volume(xl, yl, zl);
+        d->volo[i] = volume;
+        d->elemMass[i] = volume;
+        for (int j = 0; j < 8; ++j) d->nodalMass[nl[j]] += volume / 8.0;
+    }
+    /* initial energy, lulesh-init.cc:170-180 */
+    const double ebase = 3.948746e+7;
+    double scale = (nx * tp) / 45.0;
+    double einit = ebase * scale * scale * scale;
+    if (d->row + d->col + d->plane == 0) d->e[0] = einit;
+    d->deltatime = (.5 * cbrt(d->volo[0])) / sqrt(2.0 * einit);
+    d->time = 0.;
+    d->cycle = 0;
+    d->dtcourant = 1.0e+20;
+    d->dthydro = 1.0e+20;
+}
+
+lo_group *lo_create(int nx, int tp, int nr, int balance, int cost, int cbrt_mode)
+{
+    if (nx < 1 || tp < 1 || nr < 1) return NULL;
+    lo_group *g = (lo_group *)calloc(1, sizeof(lo_group));
+    g->nx = nx; g->tp = tp; g->ndom = tp * tp * tp; g->cbrt_mode = cbrt_mode;
+    g->e_cut = 1.0e-7; g->p_cut = 1.0e-7; g->q_cut = 1.0e-7; g->v_cut = 1.0e-10; g->u_cut = 1.0e-7;
+    g->hgcoef = 3.0; g->ss4o3 = 4.0 / 3.0; g->qstop = 1.0e+12; g->monoq_max_slope = 1.0;
+    g->monoq_limiter_mult = 2.0; g->qlc_monoq = 0.5; g->qqc_monoq = 2.0 / 3.0; g->qqc = 2.0;
+    g->eosvmax = 1.0e+9; g->eosvmin = 1.0e-9; g->pmin = 0.; g->emin = -1.0e+15;
+    g->dvovmax = 0.1; g->refdens = 1.0;
+    /* lulesh-init.cc:138-148 */
+    g->dtfixed = -1.0e-6; g->stoptime = 1.0e-2; g->deltatimemultlb = 1.1;
+    g->deltatimemultub = 1.2; g->dtmax = 1.0e-2;
+    g->d = (lo_dom *)calloc(g->ndom, sizeof(lo_dom));
+    for (int r = 0; r < g->ndom; ++r) init_dom(g, &g->d[r], r, nr, balance, cost);
+    /* initial nodalMass exchange, lulesh.cc:3147-3161 */
+    int fm = F_NODAL_MASS;
+    lo_exchange(g, MSG_COMM_SBN, 1, &fm);
+    return g;
+}
+
+void lo_destroy(lo_group *g)
+{
+    if (!g) return;
+    for (int r = 0; r < g->ndom; ++r) {
+        lo_dom *d = &g->d[r];
+        double *dp[] = {d->x, d->y, d->z, d->xd, d->yd, d->zd, d->xdd, d->ydd, d->zdd, d->fx, d->fy,
+                        d->fz, d->nodalMass, d->e, d->p, d->q, d->ql, d->qq, d->v, d->volo, d->delv,
+                        d->vdov, d->arealg, d->ss, d->elemMass, d->vnew, d->delv_xi, d->delv_eta,
+                        d->delv_zeta, d->delx_xi, d->delx_eta, d->delx_zeta, d->sig, d->determ,
+                        d->fx_elem, d->fy_elem, d->fz_elem, d->dvdx, d->dvdy, d->dvdz, d->x8n,
+                        d->y8n, d->z8n, d->commDataRecv};
+        for (size_t i = 0; i < sizeof(dp) / sizeof(dp[0]); ++i) free(dp[i]);
+        int *ip[] = {d->symmX, d->symmY, d->symmZ, d->nodelist, d->lxim, d->lxip, d->letam,
+                     d->letap, d->lzetam, d->lzetap, d->elemBC, d->regElemSize, d->regNumList,
+                     d->elemRep, d->nodeElemStart, d->nodeElemCornerList};
+        for (size_t i = 0; i < sizeof(ip) / sizeof(ip[0]); ++i) free(ip[i]);
+        for (int i = 0; i < d->numReg; ++i) free(d->regElemlist[i]);
+        free(d->regElemlist);
+    }
+    free(g->d);
+    free(g);
+}
+
+/* ============================== time loop ============================== */
+
+/* TimeIncrement, lulesh.cc:207-263, on every rank; the MPI_Allreduce MIN
+ * of gnewdt is taken over the subdomains. */
+static void time_increment(lo_group *g)
+{
+    double newdt = 1.0e+20;
+    for (int r = 0; r < g->ndom; ++r) {
+        double gnewdt = 1.0e+20;
+        if (g->d[r].dtcourant < gnewdt) gnewdt = g->d[r].dtcourant / 2.0;
+        if (g->d[r].dthydro < gnewdt) gnewdt = g->d[r].dthydro * 2.0 / 3.0;
+        if (r == 0 || gnewdt < newdt) newdt = gnewdt;
+    }
+    for (int r = 0; r < g->ndom; ++r) {
+        lo_dom *d = &g->d[r];
+        double targetdt = g->stoptime - d->time;
+        if ((g->dtfixed <= 0.0) && (d->cycle != 0)) {
+            double olddt = d->deltatime, dt = newdt;
+            double ratio = dt / olddt;
+            if (ratio >= 1.0) {
+                if (ratio < g->deltatimemultlb) dt = olddt;
+                else if (ratio > g->deltatimemultub) dt = olddt * g->deltatimemultub;
+            }
+            if (dt > g->dtmax) dt = g->dtmax;
+            d->deltatime = dt;
+        }
+        if ((targetdt > d->deltatime) && (targetdt < (4.0 * d->deltatime / 3.0)))
+            targetdt = 2.0 * d->deltatime / 3.0;
+        if (targetdt < d->deltatime) d->deltatime = targetdt;
+        d->time += d->deltatime;
+        ++d->cycle;
+    }
+}
+
+/* LagrangeLeapFrog, lulesh.cc:3040-3084 (SEDOV_SYNC_POS_VEL_EARLY) */
+static int leapfrog(lo_group *g)
+{
+    const int f3[3] = {F_FX, F_FY, F_FZ};
+    const int f6[6] = {F_X, F_Y, F_Z, F_XD, F_YD, F_ZD};
+    const int fq[3] = {F_DELV_XI, F_DELV_ETA, F_DELV_ZETA};
+    for (int r = 0; r < g->ndom; ++r)
+        if (local_forces(g, &g->d[r])) return -1;
+    lo_exchange(g, MSG_COMM_SBN, 3, f3);
+    for (int r = 0; r < g->ndom; ++r) node_update(g, &g->d[r]);
+    lo_exchange(g, MSG_SYNC_POS_VEL, 6, f6);
+    for (int r = 0; r < g->ndom; ++r) {
+        if (kinematics(g, &g->d[r])) return -1;
+        monoq_gradients(&g->d[r]);
+    }
+    lo_exchange(g, MSG_MONOQ, 3, fq);
+    for (int r = 0; r < g->ndom; ++r) {
+        monoq_region(g, &g->d[r]);
+        if (qstop_check(g, &g->d[r])) return -2;
+    }
+    for (int r = 0; r < g->ndom; ++r) eval_eos(g, &g->d[r]);
+    for (int r = 0; r < g->ndom; ++r) time_constraints(g, &g->d[r]);
+    return 0;
+}
+
+int lo_run(lo_group *g, int max_cycles)
+{
+    int taken = 0;
+    /* the loop test of main (lulesh.cc:3222) as evaluated on rank 0 */
+    while (g->status == 0 && g->d[0].time < g->stoptime && (max_cycles < 0 || taken < max_cycles)) {
+        time_increment(g);
+        g->status = leapfrog(g);
+        ++taken;
+    }
+    return g->status;
+}
+
+int    lo_cycle(const lo_group *g) { return g->d[0].cycle; }
+double lo_time(const lo_group *g) { return g->d[0].time; }
+double lo_deltatime(const lo_group *g) { return g->d[0].deltatime; }
+double lo_stoptime(const lo_group *g) { return g->stoptime; }
+int    lo_num_elem(const lo_group *g) { return g->d[0].numElem; }
+int    lo_num_node(const lo_group *g) { return g->d[0].numNode; }
+int    lo_num_domains(const lo_group *g) { return g->ndom; }
+void   lo_set_qstop(lo_group *g, double qstop) { g->qstop = qstop; }
+void   lo_set_stoptime(lo_group *g, double t) { g->stoptime = t; }
+
+static const double *field_ptr(const lo_dom *d, int f)
+{
+    switch (f) {
+    case F_X: return d->x; case F_Y: return d->y; case F_Z: return d->z;
+    case F_XD: return d->xd; case F_YD: return d->yd; case F_ZD: return d->zd;
+    case F_XDD: return d->xdd; case F_YDD: return d->ydd; case F_ZDD: return d->zdd;
+    case F_FX: return d->fx; case F_FY: return d->fy; case F_FZ: return d->fz;
+    case F_NODAL_MASS: return d->nodalMass;
+    case F_E: return d->e; case F_P: return d->p; case F_Q: return d->q;
+    case F_QL: return d->ql; case F_QQ: return d->qq; case F_V: return d->v;
+    case F_VOLO: return d->volo; case F_DELV: return d->delv; case F_VDOV: return d->vdov;
+    case F_AREALG: return d->arealg; case F_SS: return d->ss; case F_ELEM_MASS: return d->elemMass;
+    case F_DELV_XI: return d->delv_xi; case F_DELV_ETA: return d->delv_eta;
+    case F_DELV_ZETA: return d->delv_zeta; case F_DELX_XI: return d->delx_xi;
+    case F_DELX_ETA: return d->delx_eta; case F_DELX_ZETA: return d->delx_zeta;
+    case F_VNEW: return d->vnew;
+    default: return NULL;
+    }
+}
+
+static int field_len(const lo_dom *d, int f)
+{
+    if (f < 16) return d->numNode;
+    if (f == F_DELV_XI || f == F_DELV_ETA || f == F_DELV_ZETA) return d->allElem;
+    return d->numElem;
+}
+
+int lo_get_field(const lo_group *g, int dom, int f, double *out)
+{
+    if (dom < 0 || dom >= g->ndom) return -1;
+    const double *p = field_ptr(&g->d[dom], f);
+    if (!p) return -1;
+    int n = field_len(&g->d[dom], f);
+    memcpy(out, p, sizeof(double) * (size_t)n);
+    return n;
+}
+
+int lo_set_field(lo_group *g, int dom, int f, const double *in)
+{
+    if (dom < 0 || dom >= g->ndom) return -1;
+    double *p = field_ptr_mut(&g->d[dom], f);
+    if (!p) return -1;
+    int n = field_len(&g->d[dom], f);
+    memcpy(p, in, sizeof(double) * (size_t)n);
+    return n;
+}
+
+int lo_get_elem_rep(const lo_group *g, int dom, int *out)
+{
+    memcpy(out, g->d[dom].elemRep, sizeof(int) * (size_t)g->d[dom].numElem);
+    return g->d[dom].numElem;
+}
+
+int lo_get_region(const lo_group *g, int dom, int *out)
+{
+    for (int i = 0; i < g->d[dom].numElem; ++i) out[i] = g->d[dom].regNumList[i] - 1;
+    return g->d[dom].numElem;
+}
+
+/* VerifyAndWriteFinalOutput figures, lulesh-util.cc:177-224 */
+void lo_verify(const lo_group *g, int dom, double *origin_energy, double *max_abs_diff,
+               double *total_abs_diff, double *max_rel_diff)
+{
+    const lo_dom *d = &g->d[dom];
+    int nx = g->nx;
+    double MaxAbsDiff = 0.0, TotalAbsDiff = 0.0, MaxRelDiff = 0.0;
+    for (int j = 0; j < nx; ++j)
+        for (int k = j + 1; k < nx; ++k) {
+            double AbsDiff = fabs(d->e[j * nx + k] - d->e[k * nx + j]);
+            TotalAbsDiff += AbsDiff;
+            if (MaxAbsDiff < AbsDiff) MaxAbsDiff = AbsDiff;
+            double RelDiff = AbsDiff / d->e[k * nx + j];
+            if (MaxRelDiff < RelDiff) MaxRelDiff = RelDiff;
+        }
+    *origin_energy = d->e[0];
+    *max_abs_diff = MaxAbsDiff;
+    *total_abs_diff = TotalAbsDiff;
+    *max_rel_diff = MaxRelDiff;
+}
