@@ -1,0 +1,782 @@
+// lulesh_api.cpp -- groups, subdomains, the device_local comm backend, the
+// stepping engine (HIP graphs) and the C ABI of include/lulesh_amd.h.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "lulesh_amd.h"
+#include "lulesh_comm.h"
+#include "lulesh_dev.h"
+#include "lulesh_engine.h"
+#include "lulesh_kernels.h"
+#include "lulesh_setup.h"
+
+using namespace lulesh;
+
+#define HIPCHK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            fprintf(stderr, "lulesh_amd: %s:%d: %s: %s\n", __FILE__, __LINE__, #call,    \
+                    hipGetErrorString(e_));                                              \
+            return LULESH_EHIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+namespace lulesh {
+
+// ---------------------------------------------------------------------------
+// subdomain storage
+// ---------------------------------------------------------------------------
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+int domain_alloc(lulesh_domain *d)
+{
+    const HostDomain &h = d->host;
+    const size_t nn = h.numNode, ne = h.numElem, all = h.allElem;
+    struct Slot { double **p; size_t n; };
+    DomDev &D = d->dev;
+    std::vector<Slot> slots = {
+        {&D.x, nn}, {&D.y, nn}, {&D.z, nn}, {&D.xd, nn}, {&D.yd, nn}, {&D.zd, nn},
+        {&D.fx, nn}, {&D.fy, nn}, {&D.fz, nn}, {&D.nodalMass, nn},
+        {&D.e, ne}, {&D.p, ne}, {&D.q, ne}, {&D.v, ne}, {&D.volo, ne}, {&D.delv, ne},
+        {&D.vdov, ne}, {&D.arealg, ne}, {&D.ss, ne}, {&D.elemMass, ne}, {&D.vnew, ne},
+        {&D.delv_xi, all}, {&D.delv_eta, all}, {&D.delv_zeta, all},
+        {&D.delx_xi, ne}, {&D.delx_eta, ne}, {&D.delx_zeta, ne},
+        {&D.corner, (size_t)kCornerComps * ne}};
+    size_t bytes = 0;
+    for (auto &s : slots) bytes += align256(s.n * sizeof(double));
+    bytes += align256(ne * sizeof(int));
+    void *arena = nullptr;
+    if (hipMalloc(&arena, bytes) != hipSuccess) return LULESH_ENOMEM;
+    d->arena = arena;
+    d->arena_bytes = bytes;
+    char *p = (char *)arena;
+    for (auto &s : slots) {
+        *s.p = (double *)p;
+        p += align256(s.n * sizeof(double));
+    }
+    D.elemRep = (const int *)p;
+    HIPCHK(hipMemset(arena, 0, bytes));
+    return LULESH_OK;
+}
+
+int domain_upload(lulesh_domain *d)
+{
+    const HostDomain &h = d->host;
+    DomDev &D = d->dev;
+    const size_t nn = h.numNode, ne = h.numElem;
+    HIPCHK(hipMemcpy(D.x, h.x.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.y, h.y.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.z, h.z.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.nodalMass, h.nodalMass.data(), nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.volo, h.volo.data(), ne * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.elemMass, h.volo.data(), ne * 8, hipMemcpyHostToDevice));
+    std::vector<double> ones(ne, 1.0);
+    HIPCHK(hipMemcpy(D.v, ones.data(), ne * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.e, &h.e0, 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void *)D.elemRep, h.elemRep.data(), ne * 4, hipMemcpyHostToDevice));
+    return LULESH_OK;
+}
+
+double *field_dev_ptr(lulesh_domain *d, int f, size_t *len)
+{
+    DomDev &D = d->dev;
+    const size_t nn = d->host.numNode, ne = d->host.numElem, all = d->host.allElem;
+    switch (f) {
+    case LULESH_X: *len = nn; return D.x;
+    case LULESH_Y: *len = nn; return D.y;
+    case LULESH_Z: *len = nn; return D.z;
+    case LULESH_XD: *len = nn; return D.xd;
+    case LULESH_YD: *len = nn; return D.yd;
+    case LULESH_ZD: *len = nn; return D.zd;
+    case LULESH_FX: *len = nn; return D.fx;
+    case LULESH_FY: *len = nn; return D.fy;
+    case LULESH_FZ: *len = nn; return D.fz;
+    case LULESH_NODAL_MASS: *len = nn; return D.nodalMass;
+    case LULESH_E: *len = ne; return D.e;
+    case LULESH_P: *len = ne; return D.p;
+    case LULESH_Q: *len = ne; return D.q;
+    case LULESH_V: *len = ne; return D.v;
+    case LULESH_VOLO: *len = ne; return D.volo;
+    case LULESH_DELV: *len = ne; return D.delv;
+    case LULESH_VDOV: *len = ne; return D.vdov;
+    case LULESH_AREALG: *len = ne; return D.arealg;
+    case LULESH_SS: *len = ne; return D.ss;
+    case LULESH_ELEM_MASS: *len = ne; return D.elemMass;
+    case LULESH_DELV_XI: *len = all; return D.delv_xi;
+    case LULESH_DELV_ETA: *len = all; return D.delv_eta;
+    case LULESH_DELV_ZETA: *len = all; return D.delv_zeta;
+    case LULESH_DELX_XI: *len = ne; return D.delx_xi;
+    case LULESH_DELX_ETA: *len = ne; return D.delx_eta;
+    case LULESH_DELX_ZETA: *len = ne; return D.delx_zeta;
+    case LULESH_VNEW: *len = ne; return D.vnew;
+    default: *len = 0; return nullptr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device_local backend: receive buffers in HBM; senders write straight into
+// them (remote pack), so there is nothing to send and nothing to wait on
+// beyond stream order.
+// ---------------------------------------------------------------------------
+static int dl_alloc_recv(lulesh_domain *, size_t n, double **recv, void *)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, (n ? n : 1) * sizeof(double)) != hipSuccess) return LULESH_ENOMEM;
+    if (hipMemset(p, 0, (n ? n : 1) * sizeof(double)) != hipSuccess) return LULESH_EHIP;
+    *recv = (double *)p;
+    return LULESH_OK;
+}
+
+static int dl_teardown(lulesh_domain *, double *recv, void *)
+{
+    if (recv) (void)hipFree(recv);
+    return LULESH_OK;
+}
+
+static double *dl_pack_dest(lulesh_domain *, lulesh_domain *to, size_t offset, void *)
+{
+    return to->recv + offset;
+}
+
+static int dl_send_finish(lulesh_domain *, lulesh_stream_t, void *) { return LULESH_OK; }
+
+static double *dl_recv_base(lulesh_domain *d, void *) { return d->recv; }
+
+static const lulesh_comm_backend_ops kDeviceLocal = {
+    "device_local", dl_alloc_recv, dl_teardown, dl_pack_dest, dl_send_finish, dl_recv_base};
+
+// ---------------------------------------------------------------------------
+// comm plans
+// ---------------------------------------------------------------------------
+static std::string plan_key(int msgType, int nf, const int *fields, bool doFlag, bool planeOnly)
+{
+    std::string k = std::to_string(msgType) + ":" + std::to_string(doFlag) + std::to_string(planeOnly);
+    for (int i = 0; i < nf; ++i) k += "," + std::to_string(fields[i]);
+    return k;
+}
+
+DevPlan::~DevPlan()
+{
+    if (pack) (void)hipFree(pack);
+    if (tgt) (void)hipFree(tgt);
+    if (src) (void)hipFree(src);
+    if (fptr) (void)hipFree(fptr);
+}
+
+int group_plan(lulesh_group *g, int msgType, int nf, const int *fields, bool doFlag, bool planeOnly,
+               DevPlan **out)
+{
+    if (nf < 1 || nf > 6) return LULESH_EINVAL;
+    const std::string key = plan_key(msgType, nf, fields, doFlag, planeOnly);
+    auto it = g->plans.find(key);
+    if (it != g->plans.end()) { *out = it->second.get(); return LULESH_OK; }
+
+    PlanInput in;
+    in.tp = g->tp;
+    for (auto *d : g->doms) in.doms.push_back(&d->host);
+    in.nfields = nf;
+    in.doFlag = doFlag;
+    in.planeOnly = planeOnly;
+    in.pack_dest = [g](int from, int to, size_t off) {
+        return g->backend->pack_dest(g->doms[from], g->doms[to], off, g->backend_ctx);
+    };
+    in.recv_base = [g](int dom) -> const double * {
+        return g->backend->recv_base(g->doms[dom], g->backend_ctx);
+    };
+    HostPlan hp;
+    const int rc = build_plan(in, hp);
+    if (rc != 0) return LULESH_EDIRECTION;
+
+    std::unique_ptr<DevPlan> p(new DevPlan());
+    p->msgType = msgType;
+    p->nfields = nf;
+    p->mode = msgType == LULESH_MSG_COMM_SBN ? 0 : msgType == LULESH_MSG_SYNC_POS_VEL ? 1 : 2;
+    p->npack = (int)hp.pack.size();
+    p->ntgt = (int)hp.tgt.size();
+    p->pack_start = hp.pack_start;
+    p->tgt_start = hp.tgt_start;
+    HIPCHK(hipMalloc((void **)&p->pack, std::max<size_t>(1, hp.pack.size()) * sizeof(PackEnt)));
+    HIPCHK(hipMalloc((void **)&p->tgt, std::max<size_t>(1, hp.tgt.size()) * sizeof(UnpackTgt)));
+    HIPCHK(hipMalloc((void **)&p->src, std::max<size_t>(1, hp.src.size()) * sizeof(UnpackSrc)));
+    if (!hp.pack.empty())
+        HIPCHK(hipMemcpy(p->pack, hp.pack.data(), hp.pack.size() * sizeof(PackEnt), hipMemcpyHostToDevice));
+    if (!hp.tgt.empty())
+        HIPCHK(hipMemcpy(p->tgt, hp.tgt.data(), hp.tgt.size() * sizeof(UnpackTgt), hipMemcpyHostToDevice));
+    if (!hp.src.empty())
+        HIPCHK(hipMemcpy(p->src, hp.src.data(), hp.src.size() * sizeof(UnpackSrc), hipMemcpyHostToDevice));
+    std::vector<double *> fp((size_t)g->ndom * nf);
+    for (int r = 0; r < g->ndom; ++r)
+        for (int f = 0; f < nf; ++f) {
+            size_t len;
+            fp[(size_t)r * nf + f] = field_dev_ptr(g->doms[r], fields[f], &len);
+            if (!fp[(size_t)r * nf + f]) return LULESH_EINVAL;
+        }
+    HIPCHK(hipMalloc((void **)&p->fptr, fp.size() * sizeof(double *)));
+    HIPCHK(hipMemcpy(p->fptr, fp.data(), fp.size() * sizeof(double *), hipMemcpyHostToDevice));
+    *out = p.get();
+    g->plans[key] = std::move(p);
+    return LULESH_OK;
+}
+
+int plan_pack(lulesh_group *g, DevPlan *p, int rank, hipStream_t s, bool gated = false)
+{
+    const int a = rank < 0 ? 0 : p->pack_start[rank];
+    const int b = rank < 0 ? p->npack : p->pack_start[rank + 1];
+    HIPCHK(launch_pack(p->pack + a, b - a, p->fptr, p->nfields, gated ? g->gctl : nullptr, s));
+    return LULESH_OK;
+}
+
+int plan_unpack(lulesh_group *g, DevPlan *p, int rank, hipStream_t s, bool gated = false)
+{
+    const int a = rank < 0 ? 0 : p->tgt_start[rank];
+    const int b = rank < 0 ? p->ntgt : p->tgt_start[rank + 1];
+    HIPCHK(launch_unpack(p->tgt + a, b - a, p->src, p->fptr, p->nfields, p->mode,
+                         gated ? g->gctl : nullptr, s));
+    return LULESH_OK;
+}
+
+// the three exchange steps of a cycle and their field lists
+// (lulesh.cc:1400-1408, 1607-1617, 2511-2519)
+static const int kPosVelFields[6] = {LULESH_X, LULESH_Y, LULESH_Z, LULESH_XD, LULESH_YD, LULESH_ZD};
+static const int kMonoQFields[3] = {LULESH_DELV_XI, LULESH_DELV_ETA, LULESH_DELV_ZETA};
+
+int group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, hipStream_t s, bool gated)
+{
+    if (g->ndom == 1) return LULESH_OK;
+    const bool doFlag = msgType != LULESH_MSG_SYNC_POS_VEL;
+    const bool planeOnly = msgType == LULESH_MSG_MONOQ;
+    DevPlan *p = nullptr;
+    int rc = group_plan(g, msgType, nf, fields, doFlag, planeOnly, &p);
+    if (rc) return rc;
+    rc = plan_pack(g, p, -1, s, gated);
+    if (rc) return rc;
+    for (auto *d : g->doms) {
+        rc = g->backend->send_finish(d, s, g->backend_ctx);
+        if (rc) return rc;
+    }
+    return plan_unpack(g, p, -1, s, gated);
+}
+
+// ---------------------------------------------------------------------------
+// groups
+// ---------------------------------------------------------------------------
+int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **out)
+{
+    if (nx < 1 || tp < 1 || tp > 4 || nr < 1 || balance < 0 || cost < 0 || !out) return LULESH_EINVAL;
+    if ((long long)nx * nx * nx * 8 > 2147483647LL) return LULESH_EINVAL;
+    std::unique_ptr<lulesh_group> g(new lulesh_group());
+    g->nx = nx;
+    g->tp = tp;
+    g->ndom = tp * tp * tp;
+    g->backend = &kDeviceLocal;
+    HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    for (int r = 0; r < g->ndom; ++r) {
+        lulesh_domain *d = new lulesh_domain();
+        d->g = g.get();
+        d->rank = r;
+        g->doms.push_back(d);
+        build_host_domain(d->host, nx, tp, r, nr, balance, cost);
+        int rc = domain_alloc(d);
+        if (rc) return rc;
+        rc = domain_upload(d);
+        if (rc) return rc;
+        rc = g->backend->alloc_recv(d, (size_t)d->host.comBufSize, &d->recv, g->backend_ctx);
+        if (rc) return rc;
+    }
+    // control blocks: GroupCtl then ndom DomCtl in one allocation
+    const size_t cbytes = sizeof(GroupCtl) + (size_t)g->ndom * sizeof(DomCtl);
+    HIPCHK(hipMalloc(&g->ctl_mem, cbytes));
+    g->gctl = (GroupCtl *)g->ctl_mem;
+    g->dctl = (DomCtl *)((char *)g->ctl_mem + sizeof(GroupCtl));
+    g->stoptime = 1.0e-2;   // lulesh-init.cc:139
+    g->qstop = 1.0e+12;
+    g->max_cycles = -1;
+    int rc = group_reset_ctl(g.get());
+    if (rc) return rc;
+    for (int r = 0; r < g->ndom; ++r) {
+        lulesh_domain *d = g->doms[r];
+        DomDev &D = d->dev;
+        const HostDomain &h = d->host;
+        D.nx = h.nx; D.numElem = h.numElem; D.numNode = h.numNode; D.allElem = h.allElem;
+        D.col = h.col; D.row = h.row; D.plane = h.plane; D.tp = h.tp;
+        for (int f = 0; f < 6; ++f) D.ghost[f] = h.ghost[f];
+        D.ctl = g->dctl + r;
+        D.grp = g->gctl;
+    }
+    // initial nodalMass exchange (lulesh.cc:3147-3161)
+    const int fm = LULESH_NODAL_MASS;
+    rc = group_exchange(g.get(), LULESH_MSG_COMM_SBN, 1, &fm, g->stream);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    // set-up arrays no longer needed on the host
+    for (auto *d : g->doms) {
+        d->host.x = std::vector<double>();
+        d->host.y = std::vector<double>();
+        d->host.z = std::vector<double>();
+        d->host.volo = std::vector<double>();
+        d->host.nodalMass = std::vector<double>();
+    }
+    *out = g.release();
+    return LULESH_OK;
+}
+
+int group_reset_ctl(lulesh_group *g)
+{
+    std::vector<char> buf(sizeof(GroupCtl) + (size_t)g->ndom * sizeof(DomCtl));
+    GroupCtl *G = (GroupCtl *)buf.data();
+    DomCtl *D = (DomCtl *)(buf.data() + sizeof(GroupCtl));
+    memset(buf.data(), 0, buf.size());
+    G->stoptime = g->stoptime;
+    G->qstop = g->qstop;
+    G->dctl = g->dctl;
+    G->max_cycles = g->max_cycles;
+    G->ndom = g->ndom;
+    double big = 1.0e+20;
+    unsigned long long bb;
+    memcpy(&bb, &big, 8);
+    for (int r = 0; r < g->ndom; ++r) {
+        D[r].time = 0.0;
+        D[r].deltatime = g->doms.empty() ? 0.0 : g->doms[r]->host.deltatime0;
+        D[r].dtcourant_bits = bb;
+        D[r].dthydro_bits = bb;
+        D[r].cycle = 0;
+    }
+    HIPCHK(hipMemcpy(g->ctl_mem, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    return LULESH_OK;
+}
+
+int group_set_param(lulesh_group *g)
+{
+    // stoptime / qstop / max_cycles live in the device GroupCtl
+    GroupCtl G;
+    HIPCHK(hipMemcpy(&G, g->gctl, sizeof(G), hipMemcpyDeviceToHost));
+    G.stoptime = g->stoptime;
+    G.qstop = g->qstop;
+    G.max_cycles = g->max_cycles;
+    HIPCHK(hipMemcpy(g->gctl, &G, sizeof(G), hipMemcpyHostToDevice));
+    return LULESH_OK;
+}
+
+void group_free(lulesh_group *g)
+{
+    if (!g) return;
+    if (g->engine) engine_destroy(g->engine);
+    g->plans.clear();
+    for (auto *d : g->doms) {
+        if (d->recv) g->backend->teardown(d, d->recv, g->backend_ctx);
+        if (d->arena) (void)hipFree(d->arena);
+        delete d;
+    }
+    if (g->ctl_mem) (void)hipFree(g->ctl_mem);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+int group_read_ctl(lulesh_group *g, GroupCtl *G, DomCtl *D0, hipStream_t s)
+{
+    std::vector<char> buf(sizeof(GroupCtl) + sizeof(DomCtl));
+    HIPCHK(hipMemcpyAsync(buf.data(), g->ctl_mem, buf.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    memcpy(G, buf.data(), sizeof(GroupCtl));
+    memcpy(D0, buf.data() + sizeof(GroupCtl), sizeof(DomCtl));
+    return LULESH_OK;
+}
+
+}  // namespace lulesh
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+static hipStream_t stream_of(lulesh_group *g, lulesh_stream_t s)
+{
+    return s ? (hipStream_t)s : g->stream;
+}
+
+int lulesh_init(int device)
+{
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return LULESH_EINVAL;
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        fprintf(stderr, "lulesh_amd: device %d is %s, built for gfx950\n", device, prop.gcnArchName);
+        return LULESH_EINVAL;
+    }
+    return LULESH_OK;
+}
+
+const char *lulesh_status_string(int status)
+{
+    switch (status) {
+    case LULESH_OK: return "ok";
+    case LULESH_VOLUME_ERROR: return "VolumeError";
+    case LULESH_QSTOP_ERROR: return "QStopError";
+    case LULESH_EINVAL: return "invalid argument";
+    case LULESH_EHIP: return "HIP runtime error";
+    case LULESH_ENOMEM: return "out of device memory";
+    case LULESH_EDIRECTION: return "no message expected from that direction";
+    case LULESH_EBUSY: return "server queue full";
+    default: return "unknown";
+    }
+}
+
+const char *lulesh_version(void) { return "lulesh_amd 0.1 (gfx950)"; }
+
+int lulesh_group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **out)
+{
+    lulesh_group *g = nullptr;
+    const int rc = group_create(nx, tp, nr, balance, cost, &g);
+    if (rc) { group_free(g); return rc; }
+    *out = g;
+    return LULESH_OK;
+}
+
+int lulesh_group_destroy(lulesh_group *g)
+{
+    group_free(g);
+    return LULESH_OK;
+}
+
+int lulesh_group_num_domains(const lulesh_group *g) { return g ? g->ndom : LULESH_EINVAL; }
+
+int lulesh_group_domain(lulesh_group *g, int rank, lulesh_domain **out)
+{
+    if (!g || rank < 0 || rank >= g->ndom || !out) return LULESH_EINVAL;
+    *out = g->doms[rank];
+    return LULESH_OK;
+}
+
+int lulesh_group_set_stoptime(lulesh_group *g, double t)
+{
+    if (!g) return LULESH_EINVAL;
+    g->stoptime = t;
+    return group_set_param(g);
+}
+
+int lulesh_group_set_qstop(lulesh_group *g, double q)
+{
+    if (!g) return LULESH_EINVAL;
+    g->qstop = q;
+    return group_set_param(g);
+}
+
+int lulesh_group_set_max_cycles(lulesh_group *g, int m)
+{
+    if (!g) return LULESH_EINVAL;
+    g->max_cycles = m;
+    return group_set_param(g);
+}
+
+int lulesh_group_enqueue(lulesh_group *g, int ncycles, lulesh_stream_t stream)
+{
+    if (!g || ncycles < 0) return LULESH_EINVAL;
+    if (!g->engine) {
+        std::vector<lulesh_group *> one{g};
+        const int rc = engine_create(one, &g->engine);
+        if (rc) return rc;
+    }
+    return engine_enqueue(g->engine, ncycles, stream_of(g, stream));
+}
+
+int lulesh_group_sync(lulesh_group *g, int *cycle, double *time, double *dt, lulesh_stream_t stream)
+{
+    if (!g) return LULESH_EINVAL;
+    GroupCtl G;
+    DomCtl D0;
+    const int rc = group_read_ctl(g, &G, &D0, stream_of(g, stream));
+    if (rc) return rc;
+    if (cycle) *cycle = D0.cycle;
+    if (time) *time = D0.time;
+    if (dt) *dt = D0.deltatime;
+    return G.status;
+}
+
+int lulesh_group_run(lulesh_group *g, int cycles_per_sync, lulesh_stream_t stream)
+{
+    if (!g) return LULESH_EINVAL;
+    if (cycles_per_sync <= 0) cycles_per_sync = 32;
+    hipStream_t s = stream_of(g, stream);
+    for (;;) {
+        int rc = lulesh_group_enqueue(g, cycles_per_sync, s);
+        if (rc) return rc;
+        GroupCtl G;
+        DomCtl D0;
+        rc = group_read_ctl(g, &G, &D0, s);
+        if (rc) return rc;
+        if (G.status) return G.status;
+        if (G.done) return LULESH_OK;
+    }
+}
+
+int lulesh_group_profile(lulesh_group *g, int ncycles, double ms[7])
+{
+    if (!g || ncycles < 0 || !ms) return LULESH_EINVAL;
+    if (!g->engine) {
+        std::vector<lulesh_group *> one{g};
+        const int rc = engine_create(one, &g->engine);
+        if (rc) return rc;
+    }
+    return engine_profile(g->engine, ncycles, g->stream, ms);
+}
+
+int lulesh_group_verify(lulesh_group *g, int rank, double out[4])
+{
+    if (!g || rank < 0 || rank >= g->ndom || !out) return LULESH_EINVAL;
+    lulesh_domain *d = g->doms[rank];
+    const int nx = g->nx;
+    std::vector<double> e((size_t)nx * nx);
+    HIPCHK(hipStreamSynchronize(g->stream));
+    HIPCHK(hipMemcpy(e.data(), d->dev.e, e.size() * 8, hipMemcpyDeviceToHost));
+    // VerifyAndWriteFinalOutput, lulesh-util.cc:192-210
+    double MaxAbsDiff = 0.0, TotalAbsDiff = 0.0, MaxRelDiff = 0.0;
+    for (int j = 0; j < nx; ++j)
+        for (int k = j + 1; k < nx; ++k) {
+            const double AbsDiff = fabs(e[(size_t)j * nx + k] - e[(size_t)k * nx + j]);
+            TotalAbsDiff += AbsDiff;
+            if (MaxAbsDiff < AbsDiff) MaxAbsDiff = AbsDiff;
+            const double RelDiff = AbsDiff / e[(size_t)k * nx + j];
+            if (MaxRelDiff < RelDiff) MaxRelDiff = RelDiff;
+        }
+    out[0] = e[0];
+    out[1] = MaxAbsDiff;
+    out[2] = TotalAbsDiff;
+    out[3] = MaxRelDiff;
+    return LULESH_OK;
+}
+
+int lulesh_domain_rank(const lulesh_domain *d) { return d ? d->rank : LULESH_EINVAL; }
+
+int lulesh_domain_sizes(const lulesh_domain *d, int *nx, int *ne, int *nn, int *all)
+{
+    if (!d) return LULESH_EINVAL;
+    if (nx) *nx = d->host.nx;
+    if (ne) *ne = d->host.numElem;
+    if (nn) *nn = d->host.numNode;
+    if (all) *all = d->host.allElem;
+    return LULESH_OK;
+}
+
+int lulesh_domain_get_field(lulesh_domain *d, int f, double *host, size_t n)
+{
+    if (!d || !host) return LULESH_EINVAL;
+    size_t len;
+    double *p = field_dev_ptr(d, f, &len);
+    if (!p || n < len) return LULESH_EINVAL;
+    HIPCHK(hipStreamSynchronize(d->g->stream));
+    HIPCHK(hipMemcpy(host, p, len * 8, hipMemcpyDeviceToHost));
+    return (int)len;
+}
+
+int lulesh_domain_set_field(lulesh_domain *d, int f, const double *host, size_t n)
+{
+    if (!d || !host) return LULESH_EINVAL;
+    size_t len;
+    double *p = field_dev_ptr(d, f, &len);
+    if (!p || n < len) return LULESH_EINVAL;
+    HIPCHK(hipStreamSynchronize(d->g->stream));
+    HIPCHK(hipMemcpy(p, host, len * 8, hipMemcpyHostToDevice));
+    return (int)len;
+}
+
+int lulesh_domain_field_ptr(lulesh_domain *d, int f, double **dev)
+{
+    if (!d || !dev) return LULESH_EINVAL;
+    size_t len;
+    *dev = field_dev_ptr(d, f, &len);
+    return *dev ? LULESH_OK : LULESH_EINVAL;
+}
+
+int lulesh_domain_get_elem_rep(lulesh_domain *d, int *host, size_t n)
+{
+    if (!d || !host || n < d->host.elemRep.size()) return LULESH_EINVAL;
+    memcpy(host, d->host.elemRep.data(), d->host.elemRep.size() * sizeof(int));
+    return (int)d->host.elemRep.size();
+}
+
+int lulesh_domain_get_region(lulesh_domain *d, int *host, size_t n)
+{
+    if (!d || !host || n < d->host.region.size()) return LULESH_EINVAL;
+    memcpy(host, d->host.region.data(), d->host.region.size() * sizeof(int));
+    return (int)d->host.region.size();
+}
+
+// ---------------------------------------------------------------------------
+// the reference's comm interface
+// ---------------------------------------------------------------------------
+static int check_dims(const lulesh_domain *d, int msgType, int dx, int dy, int dz)
+{
+    const int n = d->host.nx + (msgType == LULESH_MSG_MONOQ ? 0 : 1);
+    if (dx != n || dy != n || dz != n) return LULESH_EINVAL;
+    if (msgType != LULESH_MSG_COMM_SBN && msgType != LULESH_MSG_SYNC_POS_VEL &&
+        msgType != LULESH_MSG_MONOQ)
+        return LULESH_EINVAL;
+    return LULESH_OK;
+}
+
+// CommRecv (lulesh-comm.cc:59-353).  device_local is one-sided: senders
+// write into this subdomain's buffer directly, so posting a receive only
+// validates the request (COMM_RECV_SKIP of the one-sided backends,
+// cuda comm/comm_backend.h:15).
+int CommRecv(lulesh_domain *d, int msgType, int xferFields, int dx, int dy, int dz, int doRecv,
+             int planeOnly)
+{
+    if (!d || xferFields < 1 || xferFields > 6) return LULESH_EINVAL;
+    const int rc = check_dims(d, msgType, dx, dy, dz);
+    if (rc) return rc;
+    if ((msgType == LULESH_MSG_MONOQ) != (planeOnly != 0)) return LULESH_EINVAL;
+    (void)doRecv;
+    return LULESH_OK;
+}
+
+// CommSend (lulesh-comm.cc:357-844 / CommSendGpu, lulesh-comms-gpu.cu:348)
+int CommSend(lulesh_domain *d, int msgType, int xferFields, const int *fields, int dx, int dy, int dz,
+             int doSend, int planeOnly, lulesh_stream_t stream)
+{
+    if (!d || !fields || xferFields < 1 || xferFields > 6) return LULESH_EINVAL;
+    int rc = check_dims(d, msgType, dx, dy, dz);
+    if (rc) return rc;
+    lulesh_group *g = d->g;
+    if (g->ndom == 1) return LULESH_OK;   // numRanks == 1: nothing to send
+    DevPlan *p = nullptr;
+    rc = group_plan(g, msgType, xferFields, fields, doSend != 0, planeOnly != 0, &p);
+    if (rc) return rc;
+    hipStream_t s = stream_of(g, stream);
+    rc = plan_pack(g, p, d->rank, s);
+    if (rc) return rc;
+    return g->backend->send_finish(d, s, g->backend_ctx);
+}
+
+// CommSBN (lulesh-comm.cc:848-1257 / CommSBNGpu, lulesh-comms-gpu.cu:805)
+int CommSBN(lulesh_domain *d, int xferFields, const int *fields, lulesh_stream_t stream)
+{
+    if (!d || !fields || xferFields < 1 || xferFields > 6) return LULESH_EINVAL;
+    lulesh_group *g = d->g;
+    if (g->ndom == 1) return LULESH_OK;
+    DevPlan *p = nullptr;
+    int rc = group_plan(g, LULESH_MSG_COMM_SBN, xferFields, fields, true, false, &p);
+    if (rc) return rc;
+    return plan_unpack(g, p, d->rank, stream_of(g, stream));
+}
+
+// CommSyncPosVel (lulesh-comm.cc:1261-1680 / CommSyncPosVelGpu,
+// lulesh-comms-gpu.cu:1231)
+int CommSyncPosVel(lulesh_domain *d, lulesh_stream_t stream)
+{
+    if (!d) return LULESH_EINVAL;
+    lulesh_group *g = d->g;
+    if (g->ndom == 1) return LULESH_OK;
+    DevPlan *p = nullptr;
+    int rc = group_plan(g, LULESH_MSG_SYNC_POS_VEL, 6, kPosVelFields, false, false, &p);
+    if (rc) return rc;
+    return plan_unpack(g, p, d->rank, stream_of(g, stream));
+}
+
+// CommMonoQ (lulesh-comm.cc:1684-1837 / CommMonoQGpu, lulesh-comms-gpu.cu:1665)
+int CommMonoQ(lulesh_domain *d, lulesh_stream_t stream)
+{
+    if (!d) return LULESH_EINVAL;
+    lulesh_group *g = d->g;
+    if (g->ndom == 1) return LULESH_OK;
+    DevPlan *p = nullptr;
+    int rc = group_plan(g, LULESH_MSG_MONOQ, 3, kMonoQFields, true, true, &p);
+    if (rc) return rc;
+    return plan_unpack(g, p, d->rank, stream_of(g, stream));
+}
+
+int lulesh_comm_buf_size(const lulesh_domain *d) { return d ? d->host.comBufSize : LULESH_EINVAL; }
+
+int lulesh_comm_recv_offset(const lulesh_domain *d, int dc, int dr, int dp, int xfer, int doRecv,
+                            int planeOnly)
+{
+    if (!d) return LULESH_EINVAL;
+    return recv_offset(d->host, dc, dr, dp, xfer, doRecv != 0, planeOnly != 0);
+}
+
+int lulesh_comm_get_recv_buffer(lulesh_domain *d, double *host, size_t n)
+{
+    if (!d || !host || n < (size_t)d->host.comBufSize) return LULESH_EINVAL;
+    lulesh_group *g = d->g;
+    const double *base = g->backend->recv_base(d, g->backend_ctx);
+    HIPCHK(hipStreamSynchronize(g->stream));
+    HIPCHK(hipMemcpy(host, base, (size_t)d->host.comBufSize * 8, hipMemcpyDeviceToHost));
+    return d->host.comBufSize;
+}
+
+const lulesh_comm_backend_ops *lulesh_device_local_backend(void) { return &kDeviceLocal; }
+
+int lulesh_group_set_comm_backend(lulesh_group *g, const lulesh_comm_backend_ops *ops, void *ctx)
+{
+    if (!g || !ops || !ops->alloc_recv || !ops->teardown || !ops->pack_dest || !ops->send_finish ||
+        !ops->recv_base)
+        return LULESH_EINVAL;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    // receive buffers belong to the backend: hand them over
+    for (auto *d : g->doms) {
+        double *old = d->recv;
+        double *fresh = nullptr;
+        int rc = ops->alloc_recv(d, (size_t)d->host.comBufSize, &fresh, ctx);
+        if (rc) return rc;
+        if (old) g->backend->teardown(d, old, g->backend_ctx);
+        d->recv = fresh;
+    }
+    g->backend = ops;
+    g->backend_ctx = ctx;
+    g->plans.clear();
+    if (g->engine) { engine_destroy(g->engine); g->engine = nullptr; }
+    return LULESH_OK;
+}
+
+int lulesh_group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, lulesh_stream_t stream)
+{
+    if (!g || !fields) return LULESH_EINVAL;
+    return group_exchange(g, msgType, nf, fields, stream_of(g, stream));
+}
+
+// element-wise probes of the device arithmetic (parity tests)
+int lulesh_probe_cbrt(const double *host_in, double *host_out, int n)
+{
+    if (n < 0) return LULESH_EINVAL;
+    if (n == 0) return LULESH_OK;
+    double *din = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc((void **)&din, (size_t)n * 8));
+    HIPCHK(hipMalloc((void **)&dout, (size_t)n * 8));
+    HIPCHK(hipMemcpy(din, host_in, (size_t)n * 8, hipMemcpyHostToDevice));
+    HIPCHK(launch_probe_cbrt(din, dout, n, 0));
+    HIPCHK(hipMemcpy(host_out, dout, (size_t)n * 8, hipMemcpyDeviceToHost));
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return LULESH_OK;
+}
+
+int lulesh_probe_sqrt_div(const double *a, const double *b, double *out2, int n)
+{
+    if (n < 0) return LULESH_EINVAL;
+    if (n == 0) return LULESH_OK;
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc((void **)&da, (size_t)n * 8));
+    HIPCHK(hipMalloc((void **)&db, (size_t)n * 8));
+    HIPCHK(hipMalloc((void **)&dout, (size_t)n * 16));
+    HIPCHK(hipMemcpy(da, a, (size_t)n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(db, b, (size_t)n * 8, hipMemcpyHostToDevice));
+    HIPCHK(launch_probe_sqrt_div(da, db, dout, n, 0));
+    HIPCHK(hipMemcpy(out2, dout, (size_t)n * 16, hipMemcpyDeviceToHost));
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dout);
+    return LULESH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// lulesh_dev.h -- device-visible data layout of the MI355X LULESH step.
+//
+// Layout in HBM (per subdomain, structure of arrays, FP64):
+//   node fields   x y z xd yd zd fx fy fz nodalMass        [(nx+1)^3]
+//   element       e p q v volo delv vdov arealg ss elemMass vnew
+//                 delx_xi delx_eta delx_zeta                [nx^3]
+//   gradients     delv_xi delv_eta delv_zeta   [nx^3 + 6 nx^2 ghost planes]
+//   corner forces corner[48][nx^3]  (stress x,y,z then hourglass x,y,z,
+//                 8 corners each; element index fastest -> coalesced)
+//   elemRep       int32 [nx^3]
+// The mesh is the reference's structured Sedov block (BuildMesh,
+// LULESH/omp_4.0/lulesh-init.cc:189-238), so element->node connectivity,
+// face neighbours and boundary masks are computed from (i,j,k) instead of
+// being read from nodelist / lxim.. / elemBC arrays.
+#pragma once
+#include <stdint.h>
+
+namespace lulesh {
+
+constexpr int kCornerComps = 48;     // 6 components x 8 corners
+
+// per-subdomain descriptor, read by every kernel through the tile table
+struct DomDev {
+    double *x, *y, *z, *xd, *yd, *zd, *fx, *fy, *fz, *nodalMass;
+    double *e, *p, *q, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass, *vnew;
+    double *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
+    double *corner;
+    const int *elemRep;
+    int nx, numElem, numNode, allElem;
+    int col, row, plane, tp;
+    // ghost element offsets of the velocity-gradient arrays for the six
+    // faces (planeMin, planeMax, rowMin, rowMax, colMin, colMax); -1 when
+    // the face is a physical boundary (SetupBoundaryConditions,
+    // lulesh-init.cc:533-631)
+    int ghost[6];
+    struct DomCtl *ctl;      // this subdomain's time state
+    struct GroupCtl *grp;    // its group's loop state
+};
+
+// per-subdomain time state (every rank of the reference keeps its own)
+struct DomCtl {
+    double time, deltatime;
+    unsigned long long dtcourant_bits, dthydro_bits;   // atomicMin slots
+    int cycle;
+    int pad;
+};
+
+// one allocation per group: GroupCtl followed by its ndom DomCtl
+struct GroupCtl {
+    double stoptime, qstop;
+    DomCtl *dctl;     // the group's subdomain time states (contiguous)
+    int max_cycles;   // < 0: none
+    int done;         // stop condition reached
+    int status;       // 0, -1 VolumeError, -2 QStopError
+    int ndom;
+};
+
+struct Tile {
+    int dom;
+    int start;
+};
+
+// physics constants of Domain::Domain (LULESH/omp_4.0/lulesh-init.cc:20-38)
+struct Phys {
+    static constexpr double e_cut = 1.0e-7, p_cut = 1.0e-7, q_cut = 1.0e-7;
+    static constexpr double v_cut = 1.0e-10, u_cut = 1.0e-7;
+    static constexpr double hgcoef = 3.0;
+    static constexpr double monoq_max_slope = 1.0, monoq_limiter_mult = 2.0;
+    static constexpr double qlc_monoq = 0.5, qqc_monoq = 2.0 / 3.0, qqc = 2.0;
+    static constexpr double eosvmax = 1.0e+9, eosvmin = 1.0e-9;
+    static constexpr double pmin = 0., emin = -1.0e+15;
+    static constexpr double dvovmax = 0.1, refdens = 1.0;
+    static constexpr double dtfixed = -1.0e-6, dtmax = 1.0e-2;
+    static constexpr double deltatimemultlb = 1.1, deltatimemultub = 1.2;
+};
+
+// halo-exchange tables (device_local backend)
+struct PackEnt {          // one element of one message
+    double *dst;          // receiver buffer address of field 0
+    int src_dom;
+    int src_idx;
+    int cnt;              // message length per field (field stride)
+    int pad;
+};
+
+struct UnpackTgt {        // one receiving index and its messages
+    int dom;
+    int idx;              // node index, or ghost element index
+    int first;            // into the UnpackSrc list
+    int count;
+};
+
+struct UnpackSrc {
+    const double *src;    // receive-buffer address of field 0
+    int cnt;              // field stride
+    int pad;
+};
+
+}  // namespace lulesh

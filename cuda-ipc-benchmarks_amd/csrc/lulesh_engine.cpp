@@ -1,0 +1,593 @@
+// lulesh_engine.cpp -- the stepping engine, the batched ensemble and the
+// continuous-batching server.
+//
+// An Engine steps any set of groups together.  One cycle is
+//   k_time_increment  (all groups)
+//   k_elem_force      (all subdomains)
+//   k_node_gather     (fused with the node update when no group has halos)
+//   [SBN exchange, k_node_update, PosVel exchange]   multi-subdomain groups
+//   k_kinematics
+//   [MonoQ exchange]
+//   k_qeos
+// and `n` cycles are captured once into a HIP graph and replayed, so the
+// host launches one graph per n cycles and never waits inside a cycle: dt,
+// time, the stop test and error codes all live in device control blocks.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <thread>
+#include <vector>
+
+#include "lulesh_amd.h"
+#include "lulesh_engine.h"
+#include "lulesh_kernels.h"
+
+#define HIPCHK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            fprintf(stderr, "lulesh_amd: %s:%d: %s: %s\n", __FILE__, __LINE__, #call,    \
+                    hipGetErrorString(e_));                                              \
+            return LULESH_EHIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+namespace lulesh {
+
+static const int kSbn[3] = {LULESH_FX, LULESH_FY, LULESH_FZ};
+static const int kPosVel[6] = {LULESH_X, LULESH_Y, LULESH_Z, LULESH_XD, LULESH_YD, LULESH_ZD};
+static const int kMonoQ[3] = {LULESH_DELV_XI, LULESH_DELV_ETA, LULESH_DELV_ZETA};
+
+static void engine_drop_graph(Engine *e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    e->graph_cycles = 0;
+}
+
+static int build_tiles(Engine *e)
+{
+    std::vector<Tile> et, nt;
+    for (int di = 0, gi = 0; gi < (int)e->groups.size(); ++gi)
+        for (auto *d : e->groups[gi]->doms) {
+            for (int s = 0; s < d->host.numElem; s += e->tile_e) et.push_back(Tile{di, s});
+            for (int s = 0; s < d->host.numNode; s += e->tile_n) nt.push_back(Tile{di, s});
+            ++di;
+        }
+    if (e->d_etiles) (void)hipFree(e->d_etiles);
+    if (e->d_ntiles) (void)hipFree(e->d_ntiles);
+    e->d_etiles = e->d_ntiles = nullptr;
+    e->n_etiles = (int)et.size();
+    e->n_ntiles = (int)nt.size();
+    HIPCHK(hipMalloc((void **)&e->d_etiles, std::max<size_t>(1, et.size()) * sizeof(Tile)));
+    HIPCHK(hipMalloc((void **)&e->d_ntiles, std::max<size_t>(1, nt.size()) * sizeof(Tile)));
+    if (!et.empty()) HIPCHK(hipMemcpy(e->d_etiles, et.data(), et.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    if (!nt.empty()) HIPCHK(hipMemcpy(e->d_ntiles, nt.data(), nt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    return LULESH_OK;
+}
+
+int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
+{
+    Engine *e = new Engine();
+    e->groups = groups;
+    const char *ng = getenv("LULESH_NO_GRAPH");
+    e->use_graph = !(ng && ng[0] == '1');
+    std::vector<DomDev> doms;
+    std::vector<GroupCtl *> gptr;
+    for (auto *g : groups) {
+        if (g->ndom > 1) e->fused = false;
+        for (auto *d : g->doms) doms.push_back(d->dev);
+        gptr.push_back(g->gctl);
+        if (g->ndom > 1) {
+            // build the exchange plans now: no allocation may happen while a
+            // graph is being captured
+            DevPlan *p;
+            int rc = group_plan(g, LULESH_MSG_COMM_SBN, 3, kSbn, true, false, &p);
+            if (!rc) rc = group_plan(g, LULESH_MSG_SYNC_POS_VEL, 6, kPosVel, false, false, &p);
+            if (!rc) rc = group_plan(g, LULESH_MSG_MONOQ, 3, kMonoQ, true, true, &p);
+            if (rc) { engine_destroy(e); return rc; }
+        }
+    }
+    e->ndoms = (int)doms.size();
+    if (hipMalloc((void **)&e->d_doms, std::max<size_t>(1, doms.size()) * sizeof(DomDev)) != hipSuccess ||
+        hipMalloc((void **)&e->d_groups, std::max<size_t>(1, gptr.size()) * sizeof(GroupCtl *)) != hipSuccess) {
+        engine_destroy(e);
+        return LULESH_ENOMEM;
+    }
+    if (!doms.empty())
+        (void)hipMemcpy(e->d_doms, doms.data(), doms.size() * sizeof(DomDev), hipMemcpyHostToDevice);
+    if (!gptr.empty())
+        (void)hipMemcpy(e->d_groups, gptr.data(), gptr.size() * sizeof(GroupCtl *), hipMemcpyHostToDevice);
+    const int rc = build_tiles(e);
+    if (rc) { engine_destroy(e); return rc; }
+    *out = e;
+    return LULESH_OK;
+}
+
+void engine_destroy(Engine *e)
+{
+    if (!e) return;
+    engine_drop_graph(e);
+    if (e->d_doms) (void)hipFree(e->d_doms);
+    if (e->d_groups) (void)hipFree(e->d_groups);
+    if (e->d_etiles) (void)hipFree(e->d_etiles);
+    if (e->d_ntiles) (void)hipFree(e->d_ntiles);
+    delete e;
+}
+
+int engine_set_tiles(Engine *e, int tile_e, int tile_n)
+{
+    if (tile_e > 0) {
+        if (tile_e > 256 || tile_e % 64) return LULESH_EINVAL;
+        e->tile_e = tile_e;
+    }
+    if (tile_n > 0) {
+        if (tile_n > 256 || tile_n % 64) return LULESH_EINVAL;
+        e->tile_n = tile_n;
+    }
+    engine_drop_graph(e);
+    return build_tiles(e);
+}
+
+// Optional per-kernel-class timing of engine_enqueue_cycle: an event pair
+// around every step on the engine's stream (only outside graph capture).
+struct Prof {
+    hipEvent_t a = nullptr, b = nullptr;
+    double ms[7] = {0, 0, 0, 0, 0, 0, 0};
+};
+
+static int prof_begin(Prof *p, hipStream_t s)
+{
+    if (p) HIPCHK(hipEventRecord(p->a, s));
+    return LULESH_OK;
+}
+
+static int prof_end(Prof *p, int k, hipStream_t s)
+{
+    if (!p) return LULESH_OK;
+    HIPCHK(hipEventRecord(p->b, s));
+    HIPCHK(hipEventSynchronize(p->b));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, p->a, p->b));
+    p->ms[k] += t;
+    return LULESH_OK;
+}
+
+#define STEP(k, body)                                    \
+    do {                                                 \
+        int rc_ = prof_begin(prof, s);                   \
+        if (rc_) return rc_;                             \
+        body;                                            \
+        rc_ = prof_end(prof, k, s);                      \
+        if (rc_) return rc_;                             \
+    } while (0)
+
+static int exchange_all(Engine *e, int msg, int nf, const int *f, hipStream_t s)
+{
+    for (auto *g : e->groups) {
+        const int rc = group_exchange(g, msg, nf, f, s, true);
+        if (rc) return rc;
+    }
+    return LULESH_OK;
+}
+
+// kernel classes: 0 time_increment, 1 elem_force, 2 node_gather,
+// 3 node_update, 4 kinematics, 5 qeos, 6 halo exchanges
+static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
+{
+    const int ng = (int)e->groups.size();
+    int rc = LULESH_OK;
+    STEP(0, HIPCHK(launch_time_increment(e->d_groups, ng, s)));
+    STEP(1, HIPCHK(launch_elem_force(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
+    STEP(2, HIPCHK(launch_node_gather(e->fused, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+    if (!e->fused) {
+        STEP(6, rc = exchange_all(e, LULESH_MSG_COMM_SBN, 3, kSbn, s));
+        if (rc) return rc;
+        STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+        STEP(6, rc = exchange_all(e, LULESH_MSG_SYNC_POS_VEL, 6, kPosVel, s));
+        if (rc) return rc;
+    }
+    STEP(4, HIPCHK(launch_kinematics(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
+    if (!e->fused) {
+        STEP(6, rc = exchange_all(e, LULESH_MSG_MONOQ, 3, kMonoQ, s));
+        if (rc) return rc;
+    }
+    STEP(5, HIPCHK(launch_qeos(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
+    return LULESH_OK;
+}
+
+int engine_enqueue_cycle(Engine *e, hipStream_t s) { return enqueue_cycle(e, s, nullptr); }
+
+int engine_profile(Engine *e, int ncycles, hipStream_t s, double ms[7])
+{
+    Prof p;
+    HIPCHK(hipEventCreate(&p.a));
+    HIPCHK(hipEventCreate(&p.b));
+    int rc = LULESH_OK;
+    for (int c = 0; c < ncycles && rc == LULESH_OK; ++c) rc = enqueue_cycle(e, s, &p);
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+    for (int k = 0; k < 7; ++k) ms[k] = p.ms[k];
+    return rc;
+}
+
+int engine_enqueue(Engine *e, int ncycles, hipStream_t s)
+{
+    if (ncycles <= 0) return LULESH_OK;
+    if (!e->use_graph) {
+        for (int c = 0; c < ncycles; ++c) {
+            const int rc = engine_enqueue_cycle(e, s);
+            if (rc) return rc;
+        }
+        return LULESH_OK;
+    }
+    if (!e->exec || e->graph_cycles != ncycles || e->graph_stream != s) {
+        engine_drop_graph(e);
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int rc = LULESH_OK;
+        for (int c = 0; c < ncycles && rc == LULESH_OK; ++c) rc = engine_enqueue_cycle(e, s);
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIPCHK(ce);
+        e->graph = graph;
+        HIPCHK(hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0));
+        e->graph_cycles = ncycles;
+        e->graph_stream = s;
+    }
+    HIPCHK(hipGraphLaunch(e->exec, s));
+    return LULESH_OK;
+}
+
+}  // namespace lulesh
+
+using namespace lulesh;
+
+// ===========================================================================
+// batched ensemble
+// ===========================================================================
+struct lulesh_batch {
+    std::vector<lulesh_group *> groups;
+    Engine *engine = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+extern "C" {
+
+int lulesh_batch_create(int nprob, const int *nx, const int *nr, const int *balance, const int *cost,
+                        lulesh_batch **out)
+{
+    if (nprob < 1 || !nx || !nr || !balance || !cost || !out) return LULESH_EINVAL;
+    lulesh_batch *b = new lulesh_batch();
+    for (int i = 0; i < nprob; ++i) {
+        lulesh_group *g = nullptr;
+        const int rc = group_create(nx[i], 1, nr[i], balance[i], cost[i], &g);
+        if (rc) {
+            group_free(g);
+            lulesh_batch_destroy(b);
+            return rc;
+        }
+        b->groups.push_back(g);
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        lulesh_batch_destroy(b);
+        return LULESH_EHIP;
+    }
+    const int rc = engine_create(b->groups, &b->engine);
+    if (rc) { lulesh_batch_destroy(b); return rc; }
+    *out = b;
+    return LULESH_OK;
+}
+
+int lulesh_batch_destroy(lulesh_batch *b)
+{
+    if (!b) return LULESH_OK;
+    if (b->engine) engine_destroy(b->engine);
+    for (auto *g : b->groups) group_free(g);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return LULESH_OK;
+}
+
+int lulesh_batch_set_max_cycles(lulesh_batch *b, int m)
+{
+    if (!b) return LULESH_EINVAL;
+    for (auto *g : b->groups) {
+        g->max_cycles = m;
+        const int rc = group_set_param(g);
+        if (rc) return rc;
+    }
+    return LULESH_OK;
+}
+
+int lulesh_batch_set_tiles(lulesh_batch *b, int tile_e, int tile_n)
+{
+    if (!b) return LULESH_EINVAL;
+    return engine_set_tiles(b->engine, tile_e, tile_n);
+}
+
+int lulesh_batch_group(lulesh_batch *b, int i, lulesh_group **out)
+{
+    if (!b || i < 0 || i >= (int)b->groups.size() || !out) return LULESH_EINVAL;
+    *out = b->groups[i];
+    return LULESH_OK;
+}
+
+static int batch_poll(lulesh_batch *b, bool *all_done, int *first_err)
+{
+    // every group's GroupCtl sits at the head of its own allocation
+    *all_done = true;
+    *first_err = 0;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (auto *g : b->groups) {
+        GroupCtl G;
+        HIPCHK(hipMemcpy(&G, g->gctl, sizeof(G), hipMemcpyDeviceToHost));
+        if (G.status && !*first_err) *first_err = G.status;
+        if (!G.done && !G.status) *all_done = false;
+    }
+    return LULESH_OK;
+}
+
+int lulesh_batch_run(lulesh_batch *b, int cycles_per_sync, lulesh_stream_t stream)
+{
+    if (!b) return LULESH_EINVAL;
+    (void)stream;
+    if (cycles_per_sync <= 0) cycles_per_sync = 32;
+    for (;;) {
+        int rc = engine_enqueue(b->engine, cycles_per_sync, b->stream);
+        if (rc) return rc;
+        bool done;
+        int err;
+        rc = batch_poll(b, &done, &err);
+        if (rc) return rc;
+        if (done) return err;
+    }
+}
+
+int lulesh_batch_results(lulesh_batch *b, int *cycles, int *status)
+{
+    if (!b) return LULESH_EINVAL;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < b->groups.size(); ++i) {
+        GroupCtl G;
+        DomCtl D0;
+        const int rc = group_read_ctl(b->groups[i], &G, &D0, b->stream);
+        if (rc) return rc;
+        if (cycles) cycles[i] = D0.cycle;
+        if (status) status[i] = G.status;
+    }
+    return LULESH_OK;
+}
+
+// Enqueues `ncycles` cycles of the whole ensemble without waiting (used by
+// the tile autotuner and the benchmark's timed region).
+int lulesh_batch_enqueue(lulesh_batch *b, int ncycles)
+{
+    if (!b) return LULESH_EINVAL;
+    return engine_enqueue(b->engine, ncycles, b->stream);
+}
+
+int lulesh_batch_profile(lulesh_batch *b, int ncycles, double ms[7])
+{
+    if (!b || ncycles < 0 || !ms) return LULESH_EINVAL;
+    return engine_profile(b->engine, ncycles, b->stream, ms);
+}
+
+int lulesh_batch_synchronize(lulesh_batch *b)
+{
+    if (!b) return LULESH_EINVAL;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return LULESH_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// continuous-batching server
+// ===========================================================================
+namespace {
+
+struct Request {
+    int id;
+    int nx, nr, balance, cost;
+    double arrival;
+    size_t bytes;
+};
+
+struct Result {
+    int status = 1;   // 1 = not finished
+    int cycles = 0;
+    double energy = 0, tad = 0;
+    double t_admit = -1, t_done = -1;
+};
+
+// device footprint of one admitted single-domain problem: its field arena
+// plus control block and tile-table slack
+size_t request_bytes(int nx) { return domain_device_bytes(nx) + 4096; }
+
+}  // namespace
+
+struct lulesh_server {
+    size_t budget = 0, in_use = 0;
+    int max_slots = 0;
+    std::deque<Request> queue;          // arrival order
+    std::vector<std::pair<Request, lulesh_group *>> active;
+    std::vector<Result> results;
+    Engine *engine = nullptr;
+    bool dirty = true;
+    hipStream_t stream = nullptr;
+    std::chrono::steady_clock::time_point t0;
+    int done = 0;
+    double now() const
+    {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+extern "C" {
+
+int lulesh_server_create(size_t budget, int max_slots, lulesh_server **out)
+{
+    if (!out || max_slots < 1) return LULESH_EINVAL;
+    lulesh_server *s = new lulesh_server();
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) { delete s; return LULESH_EHIP; }
+        budget = fr - fr / 16;   // leave headroom for the runtime
+    }
+    s->budget = budget;
+    s->max_slots = max_slots;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return LULESH_EHIP;
+    }
+    s->t0 = std::chrono::steady_clock::now();
+    *out = s;
+    return LULESH_OK;
+}
+
+int lulesh_server_destroy(lulesh_server *s)
+{
+    if (!s) return LULESH_OK;
+    if (s->engine) engine_destroy(s->engine);
+    for (auto &a : s->active) group_free(a.second);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return LULESH_OK;
+}
+
+double lulesh_server_now(lulesh_server *s) { return s ? s->now() : 0.0; }
+
+int lulesh_server_submit(lulesh_server *s, int nx, int nr, int balance, int cost, double arrival)
+{
+    if (!s || nx < 1 || nr < 1) return LULESH_EINVAL;
+    Request r;
+    r.id = (int)s->results.size();
+    r.nx = nx; r.nr = nr; r.balance = balance; r.cost = cost;
+    r.arrival = arrival;
+    r.bytes = request_bytes(nx);
+    if (r.bytes > s->budget) return LULESH_ENOMEM;
+    s->queue.push_back(r);
+    s->results.push_back(Result());
+    return r.id;
+}
+
+// admit arrived requests in arrival order while a slot and HBM are free
+static int server_admit(lulesh_server *s)
+{
+    const double t = s->now();
+    while (!s->queue.empty() && (int)s->active.size() < s->max_slots) {
+        const Request &r = s->queue.front();
+        if (r.arrival > t) break;
+        if (s->in_use + r.bytes > s->budget) break;
+        lulesh_group *g = nullptr;
+        const int rc = group_create(r.nx, 1, r.nr, r.balance, r.cost, &g);
+        if (rc) {
+            group_free(g);
+            return rc;
+        }
+        s->results[r.id].t_admit = s->now();
+        s->in_use += r.bytes;
+        s->active.push_back({r, g});
+        s->queue.pop_front();
+        s->dirty = true;
+    }
+    return LULESH_OK;
+}
+
+int lulesh_server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream)
+{
+    if (!s) return LULESH_EINVAL;
+    (void)stream;
+    if (cycles <= 0) cycles = 16;
+    int rc = server_admit(s);
+    if (rc) return rc;
+    if (s->active.empty()) {
+        if (!s->queue.empty()) {
+            const double wait = s->queue.front().arrival - s->now();
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+        }
+        return 0;
+    }
+    if (s->dirty) {
+        if (s->engine) engine_destroy(s->engine);
+        s->engine = nullptr;
+        std::vector<lulesh_group *> gs;
+        for (auto &a : s->active) gs.push_back(a.second);
+        rc = engine_create(gs, &s->engine);
+        if (rc) return rc;
+        s->dirty = false;
+    }
+    rc = engine_enqueue(s->engine, cycles, s->stream);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s->stream));
+    int completed = 0;
+    for (size_t i = 0; i < s->active.size();) {
+        lulesh_group *g = s->active[i].second;
+        GroupCtl G;
+        DomCtl D0;
+        rc = group_read_ctl(g, &G, &D0, s->stream);
+        if (rc) return rc;
+        if (G.done || G.status) {
+            Result &res = s->results[s->active[i].first.id];
+            res.status = G.status;
+            res.cycles = D0.cycle;
+            double v[4] = {0, 0, 0, 0};
+            lulesh_group_verify(g, 0, v);
+            res.energy = v[0];
+            res.tad = v[2];
+            res.t_done = s->now();
+            s->in_use -= s->active[i].first.bytes;
+            group_free(g);
+            s->active.erase(s->active.begin() + i);
+            s->dirty = true;
+            ++completed;
+            ++s->done;
+        } else {
+            ++i;
+        }
+    }
+    return completed;
+}
+
+int lulesh_server_result(lulesh_server *s, int id, int *status, int *cycles, double *energy, double *tad)
+{
+    if (!s || id < 0 || id >= (int)s->results.size()) return LULESH_EINVAL;
+    const Result &r = s->results[id];
+    if (status) *status = r.status;
+    if (cycles) *cycles = r.cycles;
+    if (energy) *energy = r.energy;
+    if (tad) *tad = r.tad;
+    return LULESH_OK;
+}
+
+int lulesh_server_times(lulesh_server *s, int id, double *t_admit, double *t_done)
+{
+    if (!s || id < 0 || id >= (int)s->results.size()) return LULESH_EINVAL;
+    if (t_admit) *t_admit = s->results[id].t_admit;
+    if (t_done) *t_done = s->results[id].t_done;
+    return LULESH_OK;
+}
+
+int lulesh_server_stats(lulesh_server *s, int *queued, int *active, int *done, size_t *hbm)
+{
+    if (!s) return LULESH_EINVAL;
+    if (queued) *queued = (int)s->queue.size();
+    if (active) *active = (int)s->active.size();
+    if (done) *done = s->done;
+    if (hbm) *hbm = s->in_use;
+    return LULESH_OK;
+}
+
+}  // extern "C"

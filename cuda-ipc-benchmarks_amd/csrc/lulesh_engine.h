@@ -1,0 +1,95 @@
+// lulesh_engine.h -- host-side objects behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "lulesh_amd.h"
+#include "lulesh_dev.h"
+#include "lulesh_setup.h"
+
+namespace lulesh {
+struct Engine;
+
+// a halo-exchange step of one group, resident on the device
+struct DevPlan {
+    int msgType = 0, nfields = 0, mode = 0;
+    PackEnt *pack = nullptr;
+    UnpackTgt *tgt = nullptr;
+    UnpackSrc *src = nullptr;
+    double **fptr = nullptr;
+    int npack = 0, ntgt = 0;
+    std::vector<int> pack_start, tgt_start;
+    ~DevPlan();
+};
+}  // namespace lulesh
+
+struct lulesh_domain {
+    lulesh_group *g = nullptr;
+    int rank = 0;
+    lulesh::HostDomain host;
+    lulesh::DomDev dev{};
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    double *recv = nullptr;
+};
+
+struct lulesh_group {
+    int nx = 0, tp = 1, ndom = 1;
+    std::vector<lulesh_domain *> doms;
+    void *ctl_mem = nullptr;
+    lulesh::GroupCtl *gctl = nullptr;
+    lulesh::DomCtl *dctl = nullptr;
+    double stoptime = 1.0e-2, qstop = 1.0e+12;
+    int max_cycles = -1;
+    const lulesh_comm_backend_ops *backend = nullptr;
+    void *backend_ctx = nullptr;
+    std::map<std::string, std::unique_ptr<lulesh::DevPlan>> plans;
+    lulesh::Engine *engine = nullptr;   // standalone stepping
+    hipStream_t stream = nullptr;
+};
+
+namespace lulesh {
+
+// Steps a set of groups together: one tile table per kernel class covers
+// every subdomain of every member group, and a cycle of the whole set is
+// captured once into a HIP graph and replayed.
+struct Engine {
+    std::vector<lulesh_group *> groups;
+    bool fused = true;          // every member has one subdomain
+    DomDev *d_doms = nullptr;
+    int ndoms = 0;
+    GroupCtl **d_groups = nullptr;
+    Tile *d_etiles = nullptr, *d_ntiles = nullptr;
+    int n_etiles = 0, n_ntiles = 0;
+    int tile_e = 256, tile_n = 256;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_cycles = 0;
+    hipStream_t graph_stream = nullptr;
+    bool use_graph = true;
+};
+
+int engine_create(const std::vector<lulesh_group *> &groups, Engine **out);
+void engine_destroy(Engine *e);
+int engine_set_tiles(Engine *e, int tile_e, int tile_n);
+int engine_enqueue(Engine *e, int ncycles, hipStream_t s);
+int engine_enqueue_cycle(Engine *e, hipStream_t s);
+int engine_profile(Engine *e, int ncycles, hipStream_t s, double ms[7]);
+
+int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **out);
+void group_free(lulesh_group *g);
+int group_reset_ctl(lulesh_group *g);
+int group_set_param(lulesh_group *g);
+int group_read_ctl(lulesh_group *g, GroupCtl *G, DomCtl *D0, hipStream_t s);
+// gated: skip when the group has stopped (the exchanges inside the time loop)
+int group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, hipStream_t s,
+                   bool gated = false);
+int group_plan(lulesh_group *g, int msgType, int nf, const int *fields, bool doFlag, bool planeOnly,
+               DevPlan **out);
+double *field_dev_ptr(lulesh_domain *d, int f, size_t *len);
+
+}  // namespace lulesh

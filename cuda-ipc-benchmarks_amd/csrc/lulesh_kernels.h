@@ -1,0 +1,23 @@
+// lulesh_kernels.h -- host-side launchers of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "lulesh_dev.h"
+
+namespace lulesh {
+
+hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, hipStream_t s);
+hipError_t launch_elem_force(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+hipError_t launch_kinematics(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+// gate: skip the exchange once the group has stopped (NULL: always run)
+hipError_t launch_pack(const PackEnt *e, int n, const double *const *fptr, int nfields,
+                       const GroupCtl *gate, hipStream_t s);
+hipError_t launch_unpack(const UnpackTgt *t, int n, const UnpackSrc *src, double *const *fptr,
+                         int nfields, int mode, const GroupCtl *gate, hipStream_t s);
+hipError_t launch_probe_cbrt(const double *in, double *out, int n, hipStream_t s);
+hipError_t launch_probe_sqrt_div(const double *a, const double *b, double *out, int n, hipStream_t s);
+
+}  // namespace lulesh

@@ -1,0 +1,763 @@
+// lulesh_kernels.hip -- HIP kernels of the LULESH Lagrange leapfrog step
+// for gfx950 (MI355X).
+//
+// One cycle = TimeIncrement + LagrangeLeapFrog
+// (LULESH/omp_4.0/lulesh.cc:3224-3225, 3040-3084), as the launches
+//   k_time_increment      per group: dt, time, cycle on the device
+//   k_elem_force          per element: stress + FB hourglass corner forces
+//   k_node_gather<fuse>   per node: ordered corner gather; for a single
+//                         subdomain also the acceleration/BC/velocity/position
+//                         update (no halo step in between)
+//   k_node_update         per node (multi-subdomain, after the SBN exchange)
+//   k_kinematics          per element: volume, arealg, vdov, MonoQ gradients
+//   k_qeos                per element: MonoQ region, qstop, EOS x rep,
+//                         Courant/hydro dt minima
+// Every kernel walks a tile table (subdomain, first item) so one launch
+// covers every subdomain of every group on the GPU.  Integer atomics are
+// used only for the dt minima (MIN is exact) and the error code; all force
+// sums run in the reference's element order, so results are bit-for-bit
+// reproducible.
+#include <hip/hip_runtime.h>
+
+#include "lulesh_dev.h"
+#include "lulesh_kernels.h"
+#include "lulesh_math.h"
+
+namespace lulesh {
+
+// Consecutive workgroups are dealt round-robin to the 8 XCDs; give each XCD
+// a contiguous run of tiles so neighbouring elements share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, slot = b >> 3;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+__device__ __forceinline__ bool group_stopped(const GroupCtl &G)
+{
+    return (G.done | G.status) != 0;
+}
+
+__device__ __forceinline__ void raise_status(GroupCtl *G, int code)
+{
+    atomicCAS(&G->status, 0, code);
+}
+
+__device__ __forceinline__ unsigned long long dbits(double v)
+{
+    return (unsigned long long)__double_as_longlong(v);
+}
+
+// ---------------------------------------------------------------------------
+// TimeIncrement (lulesh.cc:207-263) and the loop test of main
+// (lulesh.cc:3222), one thread per group.  gnewdt is formed per subdomain
+// and reduced with MIN across the group (the MPI_Allreduce); every
+// subdomain then applies the ratio clamps to its own deltatime.
+// ---------------------------------------------------------------------------
+__global__ void k_time_increment(GroupCtl *const *groups, int ngroups)
+{
+    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= ngroups) return;
+    GroupCtl &G = *groups[gi];
+    if (group_stopped(G)) return;
+    DomCtl *D = G.dctl;
+    if (!(D[0].time < G.stoptime) || (G.max_cycles >= 0 && D[0].cycle >= G.max_cycles)) {
+        G.done = 1;
+        return;
+    }
+    double newdt = 1.0e+20;
+    for (int r = 0; r < G.ndom; ++r) {
+        double gnewdt = 1.0e+20;
+        const double dc = __longlong_as_double((long long)D[r].dtcourant_bits);
+        const double dh = __longlong_as_double((long long)D[r].dthydro_bits);
+        if (dc < gnewdt) gnewdt = dc / 2.0;
+        if (dh < gnewdt) gnewdt = dh * 2.0 / 3.0;
+        if (r == 0 || gnewdt < newdt) newdt = gnewdt;
+    }
+    for (int r = 0; r < G.ndom; ++r) {
+        DomCtl &d = D[r];
+        double targetdt = G.stoptime - d.time;
+        if ((Phys::dtfixed <= 0.0) && (d.cycle != 0)) {
+            const double olddt = d.deltatime;
+            double dt = newdt;
+            const double ratio = dt / olddt;
+            if (ratio >= 1.0) {
+                if (ratio < Phys::deltatimemultlb) dt = olddt;
+                else if (ratio > Phys::deltatimemultub) dt = olddt * Phys::deltatimemultub;
+            }
+            if (dt > Phys::dtmax) dt = Phys::dtmax;
+            d.deltatime = dt;
+        }
+        if ((targetdt > d.deltatime) && (targetdt < (4.0 * d.deltatime / 3.0)))
+            targetdt = 2.0 * d.deltatime / 3.0;
+        if (targetdt < d.deltatime) d.deltatime = targetdt;
+        d.time += d.deltatime;
+        ++d.cycle;
+        // CalcTimeConstraintsForElems restarts from 1e20 (lulesh.cc:3019-3020)
+        d.dtcourant_bits = dbits(1.0e+20);
+        d.dthydro_bits = dbits(1.0e+20);
+    }
+}
+
+// node ids of element (i,j,k) in the structured block (BuildMesh,
+// lulesh-init.cc:214-237): corners 0-3 on plane k, 4-7 on plane k+1
+__device__ __forceinline__ void elem_nodes(int nx, int el, int n[8], int &i, int &j, int &k)
+{
+    const int s1 = nx + 1, s2 = s1 * s1;
+    k = el / (nx * nx);
+    const int rem = el - k * nx * nx;
+    j = rem / nx;
+    i = rem - j * nx;
+    const int n0 = k * s2 + j * s1 + i;
+    n[0] = n0;
+    n[1] = n0 + 1;
+    n[2] = n0 + s1 + 1;
+    n[3] = n0 + s1;
+    n[4] = n0 + s2;
+    n[5] = n0 + s2 + 1;
+    n[6] = n0 + s2 + s1 + 1;
+    n[7] = n0 + s2 + s1;
+}
+
+// ---------------------------------------------------------------------------
+// IntegrateStressForElems + CalcHourglassControlForElems +
+// CalcFBHourglassForceForElems (lulesh.cc:546-712, 837-1297): per element,
+// the 8 stress corner forces -(sigma B) from the face normals and the 8
+// Flanagan-Belytschko hourglass corner forces.  Both sets are stored
+// separately because the reference sums them separately at each node.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Tile *tiles, int ntiles)
+{
+    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const DomDev &D = doms[t.dom];
+    GroupCtl *G = D.grp;
+    if (group_stopped(*G)) return;
+    const int el = t.start + threadIdx.x;
+    if (el >= D.numElem) return;
+    int n[8], i, j, k;
+    elem_nodes(D.nx, el, n, i, j, k);
+    const size_t E = (size_t)D.numElem;
+
+    double x[8], y[8], z[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) { x[c] = D.x[n[c]]; y[c] = D.y[n[c]]; z[c] = D.z[n[c]]; }
+
+    // stress part
+    {
+        const Jac J = elem_jacobian(x, y, z);
+        if (J.volume <= 0.0) raise_status(G, -1);      // lulesh.cc:1340-1349
+        double bx[8], by[8], bz[8];
+        node_normals(x, y, z, bx, by, bz);
+        const double sig = -D.p[el] - D.q[el];         // InitStressTermsForElems
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            D.corner[(0 * 8 + c) * E + el] = -(sig * bx[c]);
+            D.corner[(1 * 8 + c) * E + el] = -(sig * by[c]);
+            D.corner[(2 * 8 + c) * E + el] = -(sig * bz[c]);
+        }
+    }
+
+    // hourglass part
+    const double vv = D.v[el];
+    if (vv <= 0.0) raise_status(G, -1);                // lulesh.cc:1263-1265
+    const double determ = D.volo[el] * vv;
+    double dvdx[8], dvdy[8], dvdz[8];
+    volume_derivs(x, y, z, dvdx, dvdy, dvdz);
+    const double volinv = 1.0 / determ;
+    double hg[8][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        double hx = x[0] * hg_gamma(m, 0), hy = y[0] * hg_gamma(m, 0), hz = z[0] * hg_gamma(m, 0);
+#pragma unroll
+        for (int c = 1; c < 8; ++c) {
+            hx = hx + x[c] * hg_gamma(m, c);
+            hy = hy + y[c] * hg_gamma(m, c);
+            hz = hz + z[c] * hg_gamma(m, c);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            hg[c][m] = hg_gamma(m, c) - volinv * (dvdx[c] * hx + dvdy[c] * hy + dvdz[c] * hz);
+    }
+    const double volume13 = cbrt_cr(determ);
+    const double coefficient = -Phys::hgcoef * 0.01 * D.ss[el] * D.elemMass[el] / volume13;
+    const double *vel[3] = {D.xd, D.yd, D.zd};
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+        double vl[8], h[4];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) vl[c] = vel[comp][n[c]];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            double acc = hg[0][m] * vl[0];
+#pragma unroll
+            for (int c = 1; c < 8; ++c) acc = acc + hg[c][m] * vl[c];
+            h[m] = acc;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            D.corner[((3 + comp) * 8 + c) * E + el] =
+                coefficient * (hg[c][0] * h[0] + hg[c][1] * h[1] + hg[c][2] * h[2] + hg[c][3] * h[3]);
+    }
+}
+
+// CalcAccelerationForNodes + ApplyAccelerationBoundaryConditionsForNodes +
+// CalcVelocityForNodes + CalcPositionForNodes (lulesh.cc:1415-1550) for
+// node (i,j,k) of subdomain D
+__device__ __forceinline__ void node_advance(const DomDev &D, int nd, int i, int j, int k,
+                                             double fx, double fy, double fz, double dt)
+{
+    const double one_over_nMass = 1. / D.nodalMass[nd];
+    double ax = fx * one_over_nMass, ay = fy * one_over_nMass, az = fz * one_over_nMass;
+    // symmetry planes exist on the subdomains at the mesh origin
+    // (SetupSymmetryPlanes, lulesh-init.cc:487-508)
+    if (D.col == 0 && i == 0) ax = 0.0;
+    if (D.row == 0 && j == 0) ay = 0.0;
+    if (D.plane == 0 && k == 0) az = 0.0;
+    double vx = D.xd[nd] + ax * dt;
+    if (fabs(vx) < Phys::u_cut) vx = 0.0;
+    double vy = D.yd[nd] + ay * dt;
+    if (fabs(vy) < Phys::u_cut) vy = 0.0;
+    double vz = D.zd[nd] + az * dt;
+    if (fabs(vz) < Phys::u_cut) vz = 0.0;
+    D.xd[nd] = vx;
+    D.yd[nd] = vy;
+    D.zd[nd] = vz;
+    D.x[nd] = D.x[nd] + vx * dt;
+    D.y[nd] = D.y[nd] + vy * dt;
+    D.z[nd] = D.z[nd] + vz * dt;
+}
+
+// ---------------------------------------------------------------------------
+// Node gather (lulesh.cc:685-703 and 1134-1152): the node's corners in
+// ascending element order -- elements (i-1,j-1,k-1) ... (i,j,k) -- summed
+// from 0.0, stress first, then hourglass, f = stress + hourglass.
+// ---------------------------------------------------------------------------
+template <bool FUSE_UPDATE>
+__global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const Tile *tiles, int ntiles)
+{
+    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const DomDev &D = doms[t.dom];
+    if (group_stopped(*D.grp)) return;
+    const int nd = t.start + threadIdx.x;
+    if (nd >= D.numNode) return;
+    const int nx = D.nx, s1 = nx + 1;
+    const int k = nd / (s1 * s1);
+    const int rem = nd - k * s1 * s1;
+    const int j = rem / s1;
+    const int i = rem - j * s1;
+    const size_t E = (size_t)D.numElem;
+
+    // (di, dj, dk, corner): element (i+di, j+dj, k+dk) holds this node as
+    // the listed corner; listed in ascending element index
+    constexpr int L[8][4] = {{-1, -1, -1, 6}, {0, -1, -1, 7}, {-1, 0, -1, 5}, {0, 0, -1, 4},
+                             {-1, -1, 0, 2}, {0, -1, 0, 3}, {-1, 0, 0, 1}, {0, 0, 0, 0}};
+    int eidx[8];
+    int ccorner[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int a = i + L[q][0], b = j + L[q][1], c = k + L[q][2];
+        const bool ok = a >= 0 && a < nx && b >= 0 && b < nx && c >= 0 && c < nx;
+        eidx[q] = ok ? (c * nx + b) * nx + a : -1;
+        ccorner[q] = L[q][3];
+    }
+    double sx = 0.0, sy = 0.0, sz = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if (eidx[q] < 0) continue;
+        const size_t e = (size_t)eidx[q];
+        const int c = ccorner[q];
+        sx += D.corner[(0 * 8 + c) * E + e];
+        sy += D.corner[(1 * 8 + c) * E + e];
+        sz += D.corner[(2 * 8 + c) * E + e];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if (eidx[q] < 0) continue;
+        const size_t e = (size_t)eidx[q];
+        const int c = ccorner[q];
+        hx += D.corner[(3 * 8 + c) * E + e];
+        hy += D.corner[(4 * 8 + c) * E + e];
+        hz += D.corner[(5 * 8 + c) * E + e];
+    }
+    const double fx = sx + hx, fy = sy + hy, fz = sz + hz;
+    if (FUSE_UPDATE) {
+        node_advance(D, nd, i, j, k, fx, fy, fz, D.ctl->deltatime);
+    } else {
+        D.fx[nd] = fx;
+        D.fy[nd] = fy;
+        D.fz[nd] = fz;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const Tile *tiles, int ntiles)
+{
+    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const DomDev &D = doms[t.dom];
+    if (group_stopped(*D.grp)) return;
+    const int nd = t.start + threadIdx.x;
+    if (nd >= D.numNode) return;
+    const int s1 = D.nx + 1;
+    const int k = nd / (s1 * s1);
+    const int rem = nd - k * s1 * s1;
+    const int j = rem / s1;
+    const int i = rem - j * s1;
+    node_advance(D, nd, i, j, k, D.fx[nd], D.fy[nd], D.fz[nd], D.ctl->deltatime);
+}
+
+// ---------------------------------------------------------------------------
+// CalcKinematicsForElems + CalcLagrangeElements + CalcMonotonicQGradients-
+// ForElems (lulesh.cc:1861-2054, 2059-2244): one gather of the element's
+// positions and velocities feeds vnew, delv, arealg, vdov and the six
+// MonoQ gradients.  The deviatoric strains of the reference are not
+// stored: nothing reads them.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Tile *tiles, int ntiles)
+{
+    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const DomDev &D = doms[t.dom];
+    GroupCtl *G = D.grp;
+    if (group_stopped(*G)) return;
+    const int el = t.start + threadIdx.x;
+    if (el >= D.numElem) return;
+    int n[8], i, j, k;
+    elem_nodes(D.nx, el, n, i, j, k);
+    const double dt = D.ctl->deltatime;
+
+    double x[8], y[8], z[8], xv[8], yv[8], zv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        x[c] = D.x[n[c]]; y[c] = D.y[n[c]]; z[c] = D.z[n[c]];
+        xv[c] = D.xd[n[c]]; yv[c] = D.yd[n[c]]; zv[c] = D.zd[n[c]];
+    }
+    const double volo = D.volo[el];
+    const double volume = elem_volume(x, y, z);
+    const double vnew = volume / volo;
+    D.vnew[el] = vnew;
+    D.delv[el] = vnew - D.v[el];
+    D.arealg[el] = char_length(x, y, z, volume);
+    if (vnew <= 0.0) raise_status(G, -1);              // lulesh.cc:2037-2049
+
+    {
+        const double dt2 = 0.5 * dt;
+        double xh[8], yh[8], zh[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            xh[c] = x[c] - dt2 * xv[c];
+            yh[c] = y[c] - dt2 * yv[c];
+            zh[c] = z[c] - dt2 * zv[c];
+        }
+        const Jac J = elem_jacobian(xh, yh, zh);
+        double b[3][4];
+        shape_b4(J, b);
+        const double inv_detJ = 1.0 / J.volume;
+        // CalcElemVelocityGradient diagonal, lulesh.cc:1809-1822
+        const double dxx = inv_detJ * (b[0][0] * (xv[0] - xv[6]) + b[0][1] * (xv[1] - xv[7]) +
+                                       b[0][2] * (xv[2] - xv[4]) + b[0][3] * (xv[3] - xv[5]));
+        const double dyy = inv_detJ * (b[1][0] * (yv[0] - yv[6]) + b[1][1] * (yv[1] - yv[7]) +
+                                       b[1][2] * (yv[2] - yv[4]) + b[1][3] * (yv[3] - yv[5]));
+        const double dzz = inv_detJ * (b[2][0] * (zv[0] - zv[6]) + b[2][1] * (zv[1] - zv[7]) +
+                                       b[2][2] * (zv[2] - zv[4]) + b[2][3] * (zv[3] - zv[5]));
+        D.vdov[el] = dxx + dyy + dzz;
+    }
+
+    // MonoQ gradients at the end-of-step positions
+    const double ptiny = 1.e-36;
+    const double vol = volo * vnew;
+    const double norm = 1.0 / (vol + ptiny);
+    const double dxj = -0.25 * ((x[0] + x[1] + x[5] + x[4]) - (x[3] + x[2] + x[6] + x[7]));
+    const double dyj = -0.25 * ((y[0] + y[1] + y[5] + y[4]) - (y[3] + y[2] + y[6] + y[7]));
+    const double dzj = -0.25 * ((z[0] + z[1] + z[5] + z[4]) - (z[3] + z[2] + z[6] + z[7]));
+    const double dxi = 0.25 * ((x[1] + x[2] + x[6] + x[5]) - (x[0] + x[3] + x[7] + x[4]));
+    const double dyi = 0.25 * ((y[1] + y[2] + y[6] + y[5]) - (y[0] + y[3] + y[7] + y[4]));
+    const double dzi = 0.25 * ((z[1] + z[2] + z[6] + z[5]) - (z[0] + z[3] + z[7] + z[4]));
+    const double dxk = 0.25 * ((x[4] + x[5] + x[6] + x[7]) - (x[0] + x[1] + x[2] + x[3]));
+    const double dyk = 0.25 * ((y[4] + y[5] + y[6] + y[7]) - (y[0] + y[1] + y[2] + y[3]));
+    const double dzk = 0.25 * ((z[4] + z[5] + z[6] + z[7]) - (z[0] + z[1] + z[2] + z[3]));
+    double ax, ay, az, dxv, dyv, dzv;
+
+    ax = dyi * dzj - dzi * dyj; ay = dzi * dxj - dxi * dzj; az = dxi * dyj - dyi * dxj;
+    D.delx_zeta[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    ax *= norm; ay *= norm; az *= norm;
+    dxv = 0.25 * ((xv[4] + xv[5] + xv[6] + xv[7]) - (xv[0] + xv[1] + xv[2] + xv[3]));
+    dyv = 0.25 * ((yv[4] + yv[5] + yv[6] + yv[7]) - (yv[0] + yv[1] + yv[2] + yv[3]));
+    dzv = 0.25 * ((zv[4] + zv[5] + zv[6] + zv[7]) - (zv[0] + zv[1] + zv[2] + zv[3]));
+    D.delv_zeta[el] = ax * dxv + ay * dyv + az * dzv;
+
+    ax = dyj * dzk - dzj * dyk; ay = dzj * dxk - dxj * dzk; az = dxj * dyk - dyj * dxk;
+    D.delx_xi[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    ax *= norm; ay *= norm; az *= norm;
+    dxv = 0.25 * ((xv[1] + xv[2] + xv[6] + xv[5]) - (xv[0] + xv[3] + xv[7] + xv[4]));
+    dyv = 0.25 * ((yv[1] + yv[2] + yv[6] + yv[5]) - (yv[0] + yv[3] + yv[7] + yv[4]));
+    dzv = 0.25 * ((zv[1] + zv[2] + zv[6] + zv[5]) - (zv[0] + zv[3] + zv[7] + zv[4]));
+    D.delv_xi[el] = ax * dxv + ay * dyv + az * dzv;
+
+    ax = dyk * dzi - dzk * dyi; ay = dzk * dxi - dxk * dzi; az = dxk * dyi - dyk * dxi;
+    D.delx_eta[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    ax *= norm; ay *= norm; az *= norm;
+    dxv = -0.25 * ((xv[0] + xv[1] + xv[5] + xv[4]) - (xv[3] + xv[2] + xv[6] + xv[7]));
+    dyv = -0.25 * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
+    dzv = -0.25 * ((zv[0] + zv[1] + zv[5] + zv[4]) - (zv[3] + zv[2] + zv[6] + zv[7]));
+    D.delv_eta[el] = ax * dxv + ay * dyv + az * dzv;
+}
+
+// one direction of CalcMonotonicQRegionForElems (lulesh.cc:2307-2340):
+// the limiter phi from the element's own gradient and its two neighbours'
+__device__ __forceinline__ double monoq_phi(double self, double dm, double dp)
+{
+    const double ptiny = 1.e-36;
+    const double norm = 1. / (self + ptiny);
+    double delvm = dm * norm, delvp = dp * norm;
+    double phi = .5 * (delvm + delvp);
+    delvm *= Phys::monoq_limiter_mult;
+    delvp *= Phys::monoq_limiter_mult;
+    if (delvm < phi) phi = delvm;
+    if (delvp < phi) phi = delvp;
+    if (phi < 0.) phi = 0.;
+    if (phi > Phys::monoq_max_slope) phi = Phys::monoq_max_slope;
+    return phi;
+}
+
+// gradient of the neighbour across one face: interior neighbour, own value
+// on a symmetry plane, 0 on a free surface, ghost plane on a subdomain face
+// (the elemBC / lxim.. logic of SetupBoundaryConditions,
+// lulesh-init.cc:576-630)
+__device__ __forceinline__ double face_nbr(const double *g, int el, bool at_edge, int nbr,
+                                           int ghost_base, int ghost_off, bool symm_side)
+{
+    if (!at_edge) return g[nbr];
+    if (ghost_base >= 0) return g[ghost_base + ghost_off];
+    return symm_side ? g[el] : 0.0;
+}
+
+__device__ __forceinline__ double ss_of(double pbvc, double e_new, double vv, double bvc,
+                                        double p_new)
+{
+    double ssc = (pbvc * e_new + vv * vv * bvc * p_new) / Phys::refdens;
+    if (ssc <= .1111111e-36) ssc = .3333333e-18;
+    else ssc = sqrt(ssc);
+    return ssc;
+}
+
+struct EosOut {
+    double p, e, q, bvc, pbvc;
+};
+
+// one rep of EvalEOSForElems (lulesh.cc:2677-2815); every rep recomputes
+// the same values from the same inputs
+__device__ __forceinline__ EosOut eos_rep(double vnewc, double e_old, double delvc, double p_old,
+                                          double q_old, double qq_old, double ql_old)
+{
+    const double c1s = 2.0 / 3.0, sixth = 1.0 / 6.0;
+    double compression = 1. / vnewc - 1.;
+    const double vchalf = vnewc - delvc * .5;
+    double compHalfStep = 1. / vchalf - 1.;
+    if (vnewc <= Phys::eosvmin) compHalfStep = compression;
+    if (vnewc >= Phys::eosvmax) { p_old = 0.; compression = 0.; compHalfStep = 0.; }
+    const double work = 0.;
+    double e_new = e_old - 0.5 * delvc * (p_old + q_old) + 0.5 * work;
+    if (e_new < Phys::emin) e_new = Phys::emin;
+    double bvc = c1s * (compHalfStep + 1.);
+    double pbvc = c1s;
+    double pHalfStep = bvc * e_new;
+    if (fabs(pHalfStep) < Phys::p_cut) pHalfStep = 0.0;
+    if (vnewc >= Phys::eosvmax) pHalfStep = 0.0;
+    if (pHalfStep < Phys::pmin) pHalfStep = Phys::pmin;
+    const double vhalf = 1. / (1. + compHalfStep);
+    double q_new;
+    if (delvc > 0.) {
+        q_new = 0.;
+    } else {
+        double ssc = (pbvc * e_new + vhalf * vhalf * bvc * pHalfStep) / Phys::refdens;
+        if (ssc <= .1111111e-36) ssc = .3333333e-18;
+        else ssc = sqrt(ssc);
+        q_new = (ssc * ql_old + qq_old);
+    }
+    e_new = e_new + 0.5 * delvc * (3.0 * (p_old + q_old) - 4.0 * (pHalfStep + q_new));
+    e_new += 0.5 * work;
+    if (fabs(e_new) < Phys::e_cut) e_new = 0.;
+    if (e_new < Phys::emin) e_new = Phys::emin;
+    bvc = c1s * (compression + 1.);
+    pbvc = c1s;
+    double p_new = bvc * e_new;
+    if (fabs(p_new) < Phys::p_cut) p_new = 0.0;
+    if (vnewc >= Phys::eosvmax) p_new = 0.0;
+    if (p_new < Phys::pmin) p_new = Phys::pmin;
+    double q_tilde;
+    if (delvc > 0.) q_tilde = 0.;
+    else q_tilde = ss_of(pbvc, e_new, vnewc, bvc, p_new) * ql_old + qq_old;
+    e_new = e_new - (7.0 * (p_old + q_old) - 8.0 * (pHalfStep + q_new) + (p_new + q_tilde)) * delvc * sixth;
+    if (fabs(e_new) < Phys::e_cut) e_new = 0.;
+    if (e_new < Phys::emin) e_new = Phys::emin;
+    bvc = c1s * (compression + 1.);
+    pbvc = c1s;
+    p_new = bvc * e_new;
+    if (fabs(p_new) < Phys::p_cut) p_new = 0.0;
+    if (vnewc >= Phys::eosvmax) p_new = 0.0;
+    if (p_new < Phys::pmin) p_new = Phys::pmin;
+    if (delvc <= 0.) {
+        q_new = ss_of(pbvc, e_new, vnewc, bvc, p_new) * ql_old + qq_old;
+        if (fabs(q_new) < Phys::q_cut) q_new = 0.;
+    }
+    EosOut o;
+    o.p = p_new; o.e = e_new; o.q = q_new; o.bvc = bvc; o.pbvc = pbvc;
+    return o;
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// CalcMonotonicQRegionForElems + the qstop scan + EvalEOSForElems x rep +
+// CalcCourantConstraintForElems + CalcHydroConstraintForElems
+// (lulesh.cc:2249-2546, 2586-2836, 2874-3035), fused per element.  The
+// rep loop is executed rep times as in the reference; its inputs are
+// passed through an empty asm barrier each round so the compiler cannot
+// collapse the identical rounds into one.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *tiles, int ntiles)
+{
+    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const DomDev &D = doms[t.dom];
+    GroupCtl *G = D.grp;
+    if (group_stopped(*G)) return;
+    const int el = t.start + threadIdx.x;
+    const bool live = el < D.numElem;
+    unsigned long long cbits = dbits(1.0e+20), hbits = dbits(1.0e+20);
+    if (live) {
+        const int nx = D.nx;
+        const int k = el / (nx * nx);
+        const int rem = el - k * nx * nx;
+        const int j = rem / nx;
+        const int i = rem - j * nx;
+
+        // --- MonoQ region ---
+        const double gxi = D.delv_xi[el], geta = D.delv_eta[el], gzeta = D.delv_zeta[el];
+        const double xim = face_nbr(D.delv_xi, el, i == 0, el - 1, D.ghost[4], k * nx + j, D.col == 0);
+        const double xip = face_nbr(D.delv_xi, el, i == nx - 1, el + 1, D.ghost[5], k * nx + j, false);
+        const double etam = face_nbr(D.delv_eta, el, j == 0, el - nx, D.ghost[2], k * nx + i, D.row == 0);
+        const double etap = face_nbr(D.delv_eta, el, j == nx - 1, el + nx, D.ghost[3], k * nx + i, false);
+        const double zetam = face_nbr(D.delv_zeta, el, k == 0, el - nx * nx, D.ghost[0], j * nx + i,
+                                      D.plane == 0);
+        const double zetap = face_nbr(D.delv_zeta, el, k == nx - 1, el + nx * nx, D.ghost[1], j * nx + i,
+                                      false);
+        const double phixi = monoq_phi(gxi, xim, xip);
+        const double phieta = monoq_phi(geta, etam, etap);
+        const double phizeta = monoq_phi(gzeta, zetam, zetap);
+        const double vdov = D.vdov[el];
+        const double vnew_raw = D.vnew[el];
+        double qlin, qquad;
+        if (vdov > 0.) {
+            qlin = 0.; qquad = 0.;
+        } else {
+            double dxi = gxi * D.delx_xi[el];
+            double deta = geta * D.delx_eta[el];
+            double dzeta = gzeta * D.delx_zeta[el];
+            if (dxi > 0.) dxi = 0.;
+            if (deta > 0.) deta = 0.;
+            if (dzeta > 0.) dzeta = 0.;
+            const double rho = D.elemMass[el] / (D.volo[el] * vnew_raw);
+            qlin = -Phys::qlc_monoq * rho *
+                   (dxi * (1. - phixi) + deta * (1. - phieta) + dzeta * (1. - phizeta));
+            qquad = Phys::qqc_monoq * rho *
+                    (dxi * dxi * (1. - phixi * phixi) + deta * deta * (1. - phieta * phieta) +
+                     dzeta * dzeta * (1. - phizeta * phizeta));
+        }
+
+        // --- qstop on the q of the previous cycle (lulesh.cc:2531-2546) ---
+        double q_old = D.q[el];
+        if (q_old > G->qstop) raise_status(G, -2);
+
+        // --- EOS ---
+        double vnewc = vnew_raw;
+        if (vnewc < Phys::eosvmin) vnewc = Phys::eosvmin;
+        if (vnewc > Phys::eosvmax) vnewc = Phys::eosvmax;
+        double e_old = D.e[el], delvc = D.delv[el], p_old = D.p[el];
+        const int rep = D.elemRep[el];
+        EosOut o;
+        o.p = o.e = o.q = o.bvc = o.pbvc = 0.0;
+        for (int r = 0; r < rep; ++r) {
+            asm volatile("" : "+v"(vnewc), "+v"(e_old), "+v"(delvc), "+v"(p_old));
+            asm volatile("" : "+v"(q_old), "+v"(qquad), "+v"(qlin));
+            o = eos_rep(vnewc, e_old, delvc, p_old, q_old, qquad, qlin);
+        }
+        D.p[el] = o.p;
+        D.e[el] = o.e;
+        D.q[el] = o.q;
+        const double ss = ss_of(o.pbvc, o.e, vnewc, o.bvc, o.p);
+        D.ss[el] = ss;
+        double vfin = vnewc;
+        if (fabs(vfin - 1.0) < Phys::v_cut) vfin = 1.0;
+        D.v[el] = vfin;
+
+        // --- time constraints ---
+        if (vdov != 0.) {
+            const double arealg = D.arealg[el];
+            const double qqc2 = 64.0 * Phys::qqc * Phys::qqc;
+            double dtf = ss * ss;
+            if (vdov < 0.) dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;
+            dtf = sqrt(dtf);
+            dtf = arealg / dtf;
+            const unsigned long long b = dbits(dtf);
+            cbits = b < cbits ? b : cbits;
+            const double dtdvov = Phys::dvovmax / (fabs(vdov) + 1.e-20);
+            const unsigned long long h = dbits(dtdvov);
+            hbits = h < hbits ? h : hbits;
+        }
+    }
+    // positive doubles order like their bit patterns: MIN via integer MIN
+    cbits = wave_min_u64(cbits);
+    hbits = wave_min_u64(hbits);
+    if ((threadIdx.x & 63) == 0) {
+        DomCtl &C = *D.ctl;
+        if (cbits < dbits(1.0e+20)) atomicMin(&C.dtcourant_bits, cbits);
+        if (hbits < dbits(1.0e+20)) atomicMin(&C.dthydro_bits, hbits);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device_local halo exchange
+// ---------------------------------------------------------------------------
+
+// pack (CommSend, lulesh-comm.cc:357-844): gather boundary values of the
+// sender into the receiver's buffer, fields back to back
+__global__ __launch_bounds__(256) void k_pack(const PackEnt *ents, int n, const double *const *fptr,
+                                              int nfields, const GroupCtl *gate)
+{
+    if (gate && group_stopped(*gate)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const PackEnt e = ents[t];
+    for (int f = 0; f < nfields; ++f)
+        e.dst[(size_t)f * e.cnt] = fptr[e.src_dom * nfields + f][e.src_idx];
+}
+
+// unpack: mode 0 = add in slot order (CommSBN, lulesh-comm.cc:848-1257),
+// 1 = overwrite, last slot wins (CommSyncPosVel, :1261-1680),
+// 2 = ghost copy (CommMonoQ, :1684-1837)
+__global__ __launch_bounds__(256) void k_unpack(const UnpackTgt *tg, int n, const UnpackSrc *src,
+                                                double *const *fptr, int nfields, int mode,
+                                                const GroupCtl *gate)
+{
+    if (gate && group_stopped(*gate)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const UnpackTgt u = tg[t];
+    for (int f = 0; f < nfields; ++f) {
+        double *fp = fptr[u.dom * nfields + f];
+        if (mode == 0) {
+            double v = fp[u.idx];
+            for (int m = 0; m < u.count; ++m) {
+                const UnpackSrc s = src[u.first + m];
+                v += s.src[(size_t)f * s.cnt];
+            }
+            fp[u.idx] = v;
+        } else {
+            const UnpackSrc s = src[u.first + u.count - 1];
+            fp[u.idx] = s.src[(size_t)f * s.cnt];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline int grid_of(int n, int b) { return (n + b - 1) / b; }
+
+hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_time_increment, dim3(grid_of(ngroups, 64)), dim3(64), 0, s, groups, ngroups);
+    return hipGetLastError();
+}
+
+hipError_t launch_elem_force(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+{
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_elem_force, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+{
+    if (nt == 0) return hipSuccess;
+    if (fuse) hipLaunchKernelGGL(k_node_gather<true>, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    else hipLaunchKernelGGL(k_node_gather<false>, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+{
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_node_update, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_kinematics(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+{
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kinematics, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+{
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_qeos, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const PackEnt *e, int n, const double *const *fptr, int nfields,
+                       const GroupCtl *gate, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack, dim3(grid_of(n, 256)), dim3(256), 0, s, e, n, fptr, nfields, gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack(const UnpackTgt *t, int n, const UnpackSrc *src, double *const *fptr,
+                         int nfields, int mode, const GroupCtl *gate, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack, dim3(grid_of(n, 256)), dim3(256), 0, s, t, n, src, fptr, nfields, mode,
+                       gate);
+    return hipGetLastError();
+}
+
+// element-wise probes of the device math, used by the parity tests
+__global__ void k_probe_cbrt(const double *in, double *out, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = cbrt_cr(in[t]);
+}
+
+__global__ void k_probe_sqrt_div(const double *a, const double *b, double *out, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        out[2 * t] = sqrt(a[t]);
+        out[2 * t + 1] = a[t] / b[t];
+    }
+}
+
+hipError_t launch_probe_cbrt(const double *in, double *out, int n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_probe_cbrt, dim3(grid_of(n, 256)), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_sqrt_div(const double *a, const double *b, double *out, int n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_probe_sqrt_div, dim3(grid_of(n, 256)), dim3(256), 0, s, a, b, out, n);
+    return hipGetLastError();
+}
+
+}  // namespace lulesh

@@ -1,0 +1,257 @@
+// lulesh_math.h -- per-element arithmetic of the Lagrange step (device).
+//
+// Each function evaluates the reference's expressions in the reference's
+// association order (LULESH/omp_4.0/lulesh.cc), and the library is built
+// with -ffp-contract=off, so the results are the same IEEE-754 doubles the
+// CPU reference produces.  Citations are path:line into the reference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lulesh {
+
+// ---------------------------------------------------------------------------
+// Correctly rounded cube root.
+//
+// |x| = M 2^E with M a 53-bit integer; t = floor((E-104)/3), k = E - 3t in
+// {104,105,106}, so cbrt(|x|) = cbrt(M 2^k) 2^t with cbrt(M 2^k) in
+// [2^52, 2^53].  A float seed refined by two Newton steps in double gives
+// the integer Y within a few units; the exact midpoint test
+// (2Y-1)^3 < 8 M 2^k < (2Y+1)^3 in 192-bit integer arithmetic then fixes
+// Y.  Ties cannot occur (odd cube vs. even right-hand side), so Y 2^t is
+// the correctly rounded cube root.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int cmp_cube192(uint64_t A, uint64_t n2, uint64_t n1)
+{
+    // A^3 as three 64-bit limbs (A < 2^55), compared with (n2, n1, 0)
+    uint64_t a2l = A * A, a2h = __umul64hi(A, A);
+    uint64_t pll = a2l * A, plh = __umul64hi(a2l, A);
+    uint64_t phl = a2h * A, phh = __umul64hi(a2h, A);
+    uint64_t w0 = pll;
+    uint64_t w1 = plh + phl;
+    uint64_t w2 = phh + (w1 < plh ? 1ull : 0ull);
+    if (n2 != w2) return n2 < w2 ? -1 : 1;
+    if (n1 != w1) return n1 < w1 ? -1 : 1;
+    return w0 ? -1 : 0;
+}
+
+__device__ __forceinline__ double cbrt_cr(double x)
+{
+    if (x == 0.0 || !isfinite(x)) return x;
+    const bool neg = x < 0.0;
+    const double ax = neg ? -x : x;
+    int e;
+    const double m = frexp(ax, &e);                       // [0.5, 1)
+    const uint64_t M = (uint64_t)ldexp(m, 53);            // exact
+    const int E = e - 53;
+    const int t = (E >= 104) ? (E - 104) / 3 : -((104 - E + 2) / 3);
+    const int k = E - 3 * t;                              // 104..106
+    // seed: cbrt(M 2^k) = 2^52 cbrt(g), g = m 2^(k-103) in [1, 8)
+    const double g = ldexp(m, k - 103);
+    double y = (double)cbrtf((float)g);
+    y = y - (y * y * y - g) / (3.0 * y * y);
+    y = y - (y * y * y - g) / (3.0 * y * y);
+    uint64_t Y = (uint64_t)__double2ull_rn(ldexp(y, 52));
+    const int s = k + 3;                                  // 107..109
+    const uint64_t n1 = M << (s - 64);
+    const uint64_t n2 = M >> (128 - s);
+    for (int it = 0; it < 8; ++it) {
+        if (cmp_cube192(2 * Y + 1, n2, n1) > 0) { ++Y; continue; }
+        if (cmp_cube192(2 * Y - 1, n2, n1) < 0) { --Y; continue; }
+        break;
+    }
+    const double r = ldexp((double)Y, t);
+    return neg ? -r : r;
+}
+
+// ---------------------------------------------------------------------------
+// element geometry
+// ---------------------------------------------------------------------------
+
+// CalcElemVolume, lulesh.cc:1627-1709
+__device__ __forceinline__ double elem_volume(const double x[8], const double y[8],
+                                              const double z[8])
+{
+    const double twelveth = 1.0 / 12.0;
+    const double dx61 = x[6] - x[1], dy61 = y[6] - y[1], dz61 = z[6] - z[1];
+    const double dx70 = x[7] - x[0], dy70 = y[7] - y[0], dz70 = z[7] - z[0];
+    const double dx63 = x[6] - x[3], dy63 = y[6] - y[3], dz63 = z[6] - z[3];
+    const double dx20 = x[2] - x[0], dy20 = y[2] - y[0], dz20 = z[2] - z[0];
+    const double dx50 = x[5] - x[0], dy50 = y[5] - y[0], dz50 = z[5] - z[0];
+    const double dx64 = x[6] - x[4], dy64 = y[6] - y[4], dz64 = z[6] - z[4];
+    const double dx31 = x[3] - x[1], dy31 = y[3] - y[1], dz31 = z[3] - z[1];
+    const double dx72 = x[7] - x[2], dy72 = y[7] - y[2], dz72 = z[7] - z[2];
+    const double dx43 = x[4] - x[3], dy43 = y[4] - y[3], dz43 = z[4] - z[3];
+    const double dx57 = x[5] - x[7], dy57 = y[5] - y[7], dz57 = z[5] - z[7];
+    const double dx14 = x[1] - x[4], dy14 = y[1] - y[4], dz14 = z[1] - z[4];
+    const double dx25 = x[2] - x[5], dy25 = y[2] - y[5], dz25 = z[2] - z[5];
+    // determinant of rows (a b c; d e f; g h i): a(ei-fh) + d(ch-bi) + g(bf-ce)
+    auto det3 = [](double a, double b, double c, double d, double e, double f,
+                   double g, double h, double i) {
+        return a * (e * i - f * h) + d * (c * h - b * i) + g * (b * f - c * e);
+    };
+    double vol = det3(dx31 + dx72, dx63, dx20, dy31 + dy72, dy63, dy20, dz31 + dz72, dz63, dz20) +
+                 det3(dx43 + dx57, dx64, dx70, dy43 + dy57, dy64, dy70, dz43 + dz57, dz64, dz70) +
+                 det3(dx14 + dx25, dx61, dx50, dy14 + dy25, dy61, dy50, dz14 + dz25, dz61, dz50);
+    return vol * twelveth;
+}
+
+// Jacobian terms of CalcElemShapeFunctionDerivatives, lulesh.cc:342-428
+struct Jac {
+    double cjxxi, cjxet, cjxze, cjyxi, cjyet, cjyze, cjzxi, cjzet, cjzze;
+    double volume;
+};
+
+__device__ __forceinline__ Jac elem_jacobian(const double x[8], const double y[8], const double z[8])
+{
+    const double fjxxi = .125 * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
+    const double fjxet = .125 * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
+    const double fjxze = .125 * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
+    const double fjyxi = .125 * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
+    const double fjyet = .125 * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
+    const double fjyze = .125 * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
+    const double fjzxi = .125 * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
+    const double fjzet = .125 * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
+    const double fjzze = .125 * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
+    Jac J;
+    J.cjxxi = (fjyet * fjzze) - (fjzet * fjyze);
+    J.cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
+    J.cjxze = (fjyxi * fjzet) - (fjzxi * fjyet);
+    J.cjyxi = -(fjxet * fjzze) + (fjzet * fjxze);
+    J.cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
+    J.cjyze = -(fjxxi * fjzet) + (fjzxi * fjxet);
+    J.cjzxi = (fjxet * fjyze) - (fjyet * fjxze);
+    J.cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
+    J.cjzze = (fjxxi * fjyet) - (fjyxi * fjxet);
+    J.volume = 8. * (fjxet * J.cjxet + fjyet * J.cjyet + fjzet * J.cjzet);
+    return J;
+}
+
+// first four shape-function derivative columns (the rest are their
+// negatives, lulesh.cc:399-424)
+__device__ __forceinline__ void shape_b4(const Jac &J, double b[3][4])
+{
+    b[0][0] = -J.cjxxi - J.cjxet - J.cjxze;
+    b[0][1] = J.cjxxi - J.cjxet - J.cjxze;
+    b[0][2] = J.cjxxi + J.cjxet - J.cjxze;
+    b[0][3] = -J.cjxxi + J.cjxet - J.cjxze;
+    b[1][0] = -J.cjyxi - J.cjyet - J.cjyze;
+    b[1][1] = J.cjyxi - J.cjyet - J.cjyze;
+    b[1][2] = J.cjyxi + J.cjyet - J.cjyze;
+    b[1][3] = -J.cjyxi + J.cjyet - J.cjyze;
+    b[2][0] = -J.cjzxi - J.cjzet - J.cjzze;
+    b[2][1] = J.cjzxi - J.cjzet - J.cjzze;
+    b[2][2] = J.cjzxi + J.cjzet - J.cjzze;
+    b[2][3] = -J.cjzxi + J.cjzet - J.cjzze;
+}
+
+// SumElemFaceNormal for one face (a b c d), lulesh.cc:433-466
+__device__ __forceinline__ void face_area(const double x[8], const double y[8], const double z[8],
+                                          int a, int b, int c, int d,
+                                          double &ax, double &ay, double &az)
+{
+    const double bx0 = 0.5 * (x[d] + x[c] - x[b] - x[a]);
+    const double by0 = 0.5 * (y[d] + y[c] - y[b] - y[a]);
+    const double bz0 = 0.5 * (z[d] + z[c] - z[b] - z[a]);
+    const double bx1 = 0.5 * (x[c] + x[b] - x[d] - x[a]);
+    const double by1 = 0.5 * (y[c] + y[b] - y[d] - y[a]);
+    const double bz1 = 0.5 * (z[c] + z[b] - z[d] - z[a]);
+    ax = 0.25 * (by0 * bz1 - bz0 * by1);
+    ay = 0.25 * (bz0 * bx1 - bx0 * bz1);
+    az = 0.25 * (bx0 * by1 - by0 * bx1);
+}
+
+// CalcElemNodeNormals, lulesh.cc:471-525: the six faces in reference order,
+// each adding its area vector to its four nodes (accumulated from 0.0)
+__device__ __forceinline__ void node_normals(const double x[8], const double y[8], const double z[8],
+                                             double nx_[8], double ny_[8], double nz_[8])
+{
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { nx_[i] = 0.0; ny_[i] = 0.0; nz_[i] = 0.0; }
+    constexpr int F[6][4] = {{0, 1, 2, 3}, {0, 4, 5, 1}, {1, 5, 6, 2},
+                             {2, 6, 7, 3}, {3, 7, 4, 0}, {4, 7, 6, 5}};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        double ax, ay, az;
+        face_area(x, y, z, F[f][0], F[f][1], F[f][2], F[f][3], ax, ay, az);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            nx_[F[f][k]] += ax;
+            ny_[F[f][k]] += ay;
+            nz_[F[f][k]] += az;
+        }
+    }
+}
+
+// VoluDer + CalcElemVolumeDerivative, lulesh.cc:717-789; row j holds the
+// node arguments the reference passes for corner j
+__device__ __forceinline__ void volume_derivs(const double x[8], const double y[8], const double z[8],
+                                              double dvdx[8], double dvdy[8], double dvdz[8])
+{
+    constexpr int A[8][6] = {{1, 2, 3, 4, 5, 7}, {2, 3, 0, 5, 6, 4}, {3, 0, 1, 6, 7, 5},
+                             {0, 1, 2, 7, 4, 6}, {7, 6, 5, 0, 3, 1}, {4, 7, 6, 1, 0, 2},
+                             {5, 4, 7, 2, 1, 3}, {6, 5, 4, 3, 2, 0}};
+    const double twelfth = 1.0 / 12.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const double x0 = x[A[j][0]], x1 = x[A[j][1]], x2 = x[A[j][2]];
+        const double x3 = x[A[j][3]], x4 = x[A[j][4]], x5 = x[A[j][5]];
+        const double y0 = y[A[j][0]], y1 = y[A[j][1]], y2 = y[A[j][2]];
+        const double y3 = y[A[j][3]], y4 = y[A[j][4]], y5 = y[A[j][5]];
+        const double z0 = z[A[j][0]], z1 = z[A[j][1]], z2 = z[A[j][2]];
+        const double z3 = z[A[j][3]], z4 = z[A[j][4]], z5 = z[A[j][5]];
+        const double dx = (y1 + y2) * (z0 + z1) - (y0 + y1) * (z1 + z2) +
+                          (y0 + y4) * (z3 + z4) - (y3 + y4) * (z0 + z4) -
+                          (y2 + y5) * (z3 + z5) + (y3 + y5) * (z2 + z5);
+        const double dy = -(x1 + x2) * (z0 + z1) + (x0 + x1) * (z1 + z2) -
+                          (x0 + x4) * (z3 + z4) + (x3 + x4) * (z0 + z4) +
+                          (x2 + x5) * (z3 + z5) - (x3 + x5) * (z2 + z5);
+        const double dz = -(y1 + y2) * (x0 + x1) + (y0 + y1) * (x1 + x2) -
+                          (y0 + y4) * (x3 + x4) + (y3 + y4) * (x0 + x4) +
+                          (y2 + y5) * (x3 + x5) - (y3 + y5) * (x2 + x5);
+        dvdx[j] = dx * twelfth;
+        dvdy[j] = dy * twelfth;
+        dvdz[j] = dz * twelfth;
+    }
+}
+
+// AreaFace, lulesh.cc:1725-1744
+__device__ __forceinline__ double area_face(const double x[8], const double y[8], const double z[8],
+                                            int a, int b, int c, int d)
+{
+    const double fx = (x[c] - x[a]) - (x[d] - x[b]);
+    const double fy = (y[c] - y[a]) - (y[d] - y[b]);
+    const double fz = (z[c] - z[a]) - (z[d] - z[b]);
+    const double gx = (x[c] - x[a]) + (x[d] - x[b]);
+    const double gy = (y[c] - y[a]) + (y[d] - y[b]);
+    const double gz = (z[c] - z[a]) + (z[d] - z[b]);
+    return (fx * fx + fy * fy + fz * fz) * (gx * gx + gy * gy + gz * gz) -
+           (fx * gx + fy * gy + fz * gz) * (fx * gx + fy * gy + fz * gz);
+}
+
+// CalcElemCharacteristicLength, lulesh.cc:1751-1791
+__device__ __forceinline__ double char_length(const double x[8], const double y[8], const double z[8],
+                                              double volume)
+{
+    constexpr int F[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                             {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+    double cl = 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const double a = area_face(x, y, z, F[f][0], F[f][1], F[f][2], F[f][3]);
+        cl = a > cl ? a : cl;
+    }
+    return 4.0 * volume / sqrt(cl);
+}
+
+// hourglass base vectors, lulesh.cc:909-940 (gamma[mode][corner])
+__device__ __forceinline__ double hg_gamma(int m, int j)
+{
+    constexpr signed char G[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1},
+                                     {1, -1, -1, 1, -1, 1, 1, -1},
+                                     {1, -1, 1, -1, 1, -1, 1, -1},
+                                     {-1, 1, -1, 1, 1, -1, 1, -1}};
+    return (double)G[m][j];
+}
+
+}  // namespace lulesh

@@ -1,0 +1,201 @@
+// lulesh_setup.cpp -- host-side Sedov set-up of one subdomain.
+//
+// Reproduces Domain::Domain and its helpers (LULESH/omp_4.0/lulesh-init.cc)
+// so the initial state the GPU starts from is the reference's, bit for bit:
+// the same coordinate formula, the same element volumes in the same order,
+// the same nodal-mass accumulation, glibc rand() for the region layout.
+#include "lulesh_setup.h"
+
+#include <math.h>
+#include <stdlib.h>
+
+namespace lulesh {
+
+// CalcElemVolume (LULESH/omp_4.0/lulesh.cc:1627-1709), host copy for set-up
+static double host_elem_volume(const double x[8], const double y[8], const double z[8])
+{
+    const double twelveth = 1.0 / 12.0;
+    const double dx61 = x[6] - x[1], dy61 = y[6] - y[1], dz61 = z[6] - z[1];
+    const double dx70 = x[7] - x[0], dy70 = y[7] - y[0], dz70 = z[7] - z[0];
+    const double dx63 = x[6] - x[3], dy63 = y[6] - y[3], dz63 = z[6] - z[3];
+    const double dx20 = x[2] - x[0], dy20 = y[2] - y[0], dz20 = z[2] - z[0];
+    const double dx50 = x[5] - x[0], dy50 = y[5] - y[0], dz50 = z[5] - z[0];
+    const double dx64 = x[6] - x[4], dy64 = y[6] - y[4], dz64 = z[6] - z[4];
+    const double dx31 = x[3] - x[1], dy31 = y[3] - y[1], dz31 = z[3] - z[1];
+    const double dx72 = x[7] - x[2], dy72 = y[7] - y[2], dz72 = z[7] - z[2];
+    const double dx43 = x[4] - x[3], dy43 = y[4] - y[3], dz43 = z[4] - z[3];
+    const double dx57 = x[5] - x[7], dy57 = y[5] - y[7], dz57 = z[5] - z[7];
+    const double dx14 = x[1] - x[4], dy14 = y[1] - y[4], dz14 = z[1] - z[4];
+    const double dx25 = x[2] - x[5], dy25 = y[2] - y[5], dz25 = z[2] - z[5];
+    auto det3 = [](double a, double b, double c, double d, double e, double f,
+                   double g, double h, double i) {
+        return a * (e * i - f * h) + d * (c * h - b * i) + g * (b * f - c * e);
+    };
+    double vol = det3(dx31 + dx72, dx63, dx20, dy31 + dy72, dy63, dy20, dz31 + dz72, dz63, dz20) +
+                 det3(dx43 + dx57, dx64, dx70, dy43 + dy57, dy64, dy70, dz43 + dz57, dz64, dz70) +
+                 det3(dx14 + dx25, dx61, dx50, dy14 + dy25, dy61, dy50, dz14 + dz25, dz61, dz50);
+    return vol * twelveth;
+}
+
+// Domain::CreateRegionIndexSets (lulesh-init.cc:377-484) and the elemRep
+// assignment of main (lulesh.cc:3171-3192).  Results never depend on the
+// regions -- every EOS rep recomputes the same values -- only the cost does.
+static void create_regions(HostDomain &h, int nr, int balance, int cost)
+{
+    const int ne = h.numElem;
+    srand(h.rank);
+    h.region.assign(ne, 0);
+    std::vector<int> regNum(ne, 0);
+    int nextIndex = 0;
+    if (nr == 1) {
+        while (nextIndex < ne) regNum[nextIndex++] = 1;
+    } else {
+        std::vector<int> binEnd(nr);
+        int costDenominator = 0;
+        for (int i = 0; i < nr; ++i) {
+            costDenominator += pow((i + 1), balance);
+            binEnd[i] = costDenominator;
+        }
+        int lastReg = -1;
+        while (nextIndex < ne) {
+            int regionVar = rand() % costDenominator;
+            int i = 0;
+            while (regionVar >= binEnd[i]) i++;
+            int regionNum = ((i + h.rank) % nr) + 1;
+            while (regionNum == lastReg) {
+                regionVar = rand() % costDenominator;
+                i = 0;
+                while (regionVar >= binEnd[i]) i++;
+                regionNum = ((i + h.rank) % nr) + 1;
+            }
+            const int binSize = rand() % 1000;
+            int elements;
+            if (binSize < 773) elements = rand() % 15 + 1;
+            else if (binSize < 937) elements = rand() % 16 + 16;
+            else if (binSize < 970) elements = rand() % 32 + 32;
+            else if (binSize < 974) elements = rand() % 64 + 64;
+            else if (binSize < 978) elements = rand() % 128 + 128;
+            else if (binSize < 981) elements = rand() % 256 + 256;
+            else elements = rand() % 1537 + 512;
+            const int runto = elements + nextIndex;
+            while (nextIndex < runto && nextIndex < ne) regNum[nextIndex++] = regionNum;
+            lastReg = regionNum;
+        }
+    }
+    h.elemRep.assign(ne, 1);
+    for (int i = 0; i < ne; ++i) {
+        const int r = regNum[i] - 1;
+        int rep;
+        if (r < nr / 2) rep = 1;
+        else if (r < (nr - (nr + 15) / 20)) rep = 1 + cost;
+        else rep = 10 * (1 + cost);
+        h.region[i] = r;
+        h.elemRep[i] = rep;
+    }
+}
+
+void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost)
+{
+    h.nx = nx;
+    h.tp = tp;
+    h.rank = rank;
+    // InitMeshDecomp, lulesh-init.cc:635-712 (one domain per rank)
+    h.col = rank % tp;
+    h.row = (rank / tp) % tp;
+    h.plane = rank / (tp * tp);
+    const int en = nx + 1;
+    h.numElem = nx * nx * nx;
+    h.numNode = en * en * en;
+    h.allElem = h.numElem + 6 * nx * nx;
+
+    // SetupCommBuffers, lulesh-init.cc:317-372
+    h.rowMin = h.row != 0;
+    h.rowMax = h.row != tp - 1;
+    h.colMin = h.col != 0;
+    h.colMax = h.col != tp - 1;
+    h.planeMin = h.plane != 0;
+    h.planeMax = h.plane != tp - 1;
+    const int pad = 16;   // 128-byte coherence pad in doubles
+    h.maxPlaneSize = ((en * en) + pad - 1) & ~(pad - 1);
+    h.maxEdgeSize = (en + pad - 1) & ~(pad - 1);
+    const int faces = h.rowMin + h.rowMax + h.colMin + h.colMax + h.planeMin + h.planeMax;
+    const int edges = (h.rowMin & h.colMin) + (h.rowMin & h.planeMin) + (h.colMin & h.planeMin) +
+                      (h.rowMax & h.colMax) + (h.rowMax & h.planeMax) + (h.colMax & h.planeMax) +
+                      (h.rowMax & h.colMin) + (h.rowMin & h.planeMax) + (h.colMin & h.planeMax) +
+                      (h.rowMin & h.colMax) + (h.rowMax & h.planeMin) + (h.colMax & h.planeMin);
+    int corners = 0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c)
+                corners += (a ? h.rowMax : h.rowMin) & (b ? h.colMax : h.colMin) &
+                           (c ? h.planeMax : h.planeMin);
+    h.comBufSize = faces * h.maxPlaneSize * 6 + edges * h.maxEdgeSize * 6 + corners * pad;
+
+    // ghost offsets, SetupBoundaryConditions, lulesh-init.cc:545-574
+    int pidx = h.numElem;
+    const bool present[6] = {h.planeMin, h.planeMax, h.rowMin, h.rowMax, h.colMin, h.colMax};
+    for (int f = 0; f < 6; ++f) {
+        if (present[f]) { h.ghost[f] = pidx; pidx += nx * nx; }
+        else h.ghost[f] = -1;
+    }
+
+    // BuildMesh, lulesh-init.cc:189-212
+    h.x.resize(h.numNode);
+    h.y.resize(h.numNode);
+    h.z.resize(h.numNode);
+    const int meshEdgeElems = tp * nx;
+    int nidx = 0;
+    double tz = 1.125 * (double)(h.plane * nx) / (double)meshEdgeElems;
+    for (int plane = 0; plane < en; ++plane) {
+        double ty = 1.125 * (double)(h.row * nx) / (double)meshEdgeElems;
+        for (int row = 0; row < en; ++row) {
+            double tx = 1.125 * (double)(h.col * nx) / (double)meshEdgeElems;
+            for (int col = 0; col < en; ++col) {
+                h.x[nidx] = tx;
+                h.y[nidx] = ty;
+                h.z[nidx] = tz;
+                ++nidx;
+                tx = 1.125 * (double)(h.col * nx + col + 1) / (double)meshEdgeElems;
+            }
+            ty = 1.125 * (double)(h.row * nx + row + 1) / (double)meshEdgeElems;
+        }
+        tz = 1.125 * (double)(h.plane * nx + plane + 1) / (double)meshEdgeElems;
+    }
+
+    create_regions(h, nr, balance, cost);
+
+    // initial volumes and masses, lulesh-init.cc:151-168 (element order)
+    h.volo.resize(h.numElem);
+    h.nodalMass.assign(h.numNode, 0.0);
+    const int s2 = en * en;
+    for (int k = 0, el = 0; k < nx; ++k)
+        for (int j = 0; j < nx; ++j)
+            for (int i = 0; i < nx; ++i, ++el) {
+                const int n0 = k * s2 + j * en + i;
+                const int nl[8] = {n0, n0 + 1, n0 + en + 1, n0 + en,
+                                   n0 + s2, n0 + s2 + 1, n0 + s2 + en + 1, n0 + s2 + en};
+                double xl[8], yl[8], zl[8];
+                for (int c = 0; c < 8; ++c) { xl[c] = h.x[nl[c]]; yl[c] = h.y[nl[c]]; zl[c] = h.z[nl[c]]; }
+                const double volume = host_elem_volume(xl, yl, zl);
+                h.volo[el] = volume;
+                for (int c = 0; c < 8; ++c) h.nodalMass[nl[c]] += volume / 8.0;
+            }
+
+    // initial energy and dt0, lulesh-init.cc:170-180
+    const double ebase = 3.948746e+7;
+    const double scale = (nx * tp) / 45.0;
+    const double einit = ebase * scale * scale * scale;
+    h.e0 = (h.row + h.col + h.plane == 0) ? einit : 0.0;
+    h.deltatime0 = (.5 * cbrt(h.volo[0])) / sqrt(2.0 * einit);
+}
+
+size_t domain_device_bytes(int nx)
+{
+    const size_t ne = (size_t)nx * nx * nx, nn = (size_t)(nx + 1) * (nx + 1) * (nx + 1);
+    const size_t all = ne + 6 * (size_t)nx * nx;
+    // 10 node fields, 11 element fields, 3 ghosted gradients, 3 plain
+    // gradients, 48 corner components, elemRep; 256-byte alignment slack
+    return 8 * (10 * nn + 11 * ne + 3 * all + 3 * ne + 48 * ne) + 4 * ne + 64 * 256;
+}
+
+}  // namespace lulesh

@@ -1,0 +1,24 @@
+// lulesh_setup.h -- host-side Sedov set-up of one subdomain.
+#pragma once
+#include <stddef.h>
+
+#include <vector>
+
+namespace lulesh {
+
+struct HostDomain {
+    int nx = 0, tp = 1, rank = 0, col = 0, row = 0, plane = 0;
+    int numElem = 0, numNode = 0, allElem = 0;
+    bool rowMin = false, rowMax = false, colMin = false, colMax = false;
+    bool planeMin = false, planeMax = false;
+    int maxPlaneSize = 0, maxEdgeSize = 0, comBufSize = 0;
+    int ghost[6] = {-1, -1, -1, -1, -1, -1};
+    std::vector<double> x, y, z, volo, nodalMass;
+    std::vector<int> elemRep, region;
+    double e0 = 0.0, deltatime0 = 0.0;
+};
+
+void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost);
+size_t domain_device_bytes(int nx);
+
+}  // namespace lulesh

@@ -1,0 +1,106 @@
+// lulesh_amd_run -- command-line driver with the reference's options and
+// final output (LULESH/omp_4.0/lulesh.cc:3089-3275, lulesh-util.cc:60-224),
+// running the Lagrange step on the GPU through the C ABI.
+//
+//   -s <size>   elements per edge of each subdomain   (default 30)
+//   -i <its>    cycle cap                              (default none)
+//   -r <nr>     regions                                (default 11)
+//   -b <bal>    region balance                         (default 1)
+//   -c <cost>   region cost                            (default 1)
+//   -t <tp>     subdomains per edge; tp^3 subdomains share the GPU (default 1)
+//   -p          print progress every 100 cycles
+//   -q          quiet
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+
+#include "lulesh_amd.h"
+
+static int parse_int(const char *s, int *out)
+{
+    char *end = nullptr;
+    const long v = strtol(s, &end, 10);
+    if (end == s || (*end != '\0' && *end != ' ')) return 0;
+    *out = (int)v;
+    return 1;
+}
+
+int main(int argc, char **argv)
+{
+    int its = -1, nx = 30, nr = 11, bal = 1, cost = 1, tp = 1, prog = 0, quiet = 0;
+    for (int i = 1; i < argc; ++i) {
+        int *dst = nullptr;
+        if (!strcmp(argv[i], "-s")) dst = &nx;
+        else if (!strcmp(argv[i], "-i")) dst = &its;
+        else if (!strcmp(argv[i], "-r")) dst = &nr;
+        else if (!strcmp(argv[i], "-b")) dst = &bal;
+        else if (!strcmp(argv[i], "-c")) dst = &cost;
+        else if (!strcmp(argv[i], "-t")) dst = &tp;
+        else if (!strcmp(argv[i], "-p")) { prog = 1; continue; }
+        else if (!strcmp(argv[i], "-q")) { quiet = 1; continue; }
+        else {
+            fprintf(stderr, "unknown option %s\n", argv[i]);
+            return 1;
+        }
+        if (i + 1 >= argc || !parse_int(argv[i + 1], dst)) {
+            fprintf(stderr, "integer value required after %s\n", argv[i]);
+            return 1;
+        }
+        ++i;
+    }
+    int rc = lulesh_init(0);
+    if (rc) { fprintf(stderr, "lulesh_init: %s\n", lulesh_status_string(rc)); return 1; }
+    lulesh_group *g = nullptr;
+    rc = lulesh_group_create(nx, tp, nr, bal, cost, &g);
+    if (rc) { fprintf(stderr, "create: %s\n", lulesh_status_string(rc)); return 1; }
+    if (its >= 0) lulesh_group_set_max_cycles(g, its);
+    const int ndom = tp * tp * tp;
+    if (!quiet) {
+        printf("Running problem size %d^3 per domain until completion\n", nx);
+        printf("Num subdomains: %d (on one GPU)\n", ndom);
+        printf("Total number of elements: %lld\n\n", (long long)ndom * nx * nx * nx);
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    int cycle = 0;
+    double time = 0, dt = 0;
+    for (;;) {
+        rc = lulesh_group_enqueue(g, prog ? 100 : 64, nullptr);
+        if (rc) break;
+        const int prev = cycle;
+        rc = lulesh_group_sync(g, &cycle, &time, &dt, nullptr);
+        if (rc) break;
+        if (prog && !quiet) printf("cycle = %d, time = %e, dt=%e\n", cycle, time, dt);
+        if (cycle == prev) break;   // stop condition reached on the device
+    }
+    const double elapsed =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (rc) {
+        fprintf(stderr, "run stopped: %s\n", lulesh_status_string(rc));
+        lulesh_group_destroy(g);
+        return rc == LULESH_VOLUME_ERROR || rc == LULESH_QSTOP_ERROR ? -rc : 1;
+    }
+    double v[4];
+    lulesh_group_verify(g, 0, v);
+    if (!quiet) {
+        // VerifyAndWriteFinalOutput, lulesh-util.cc:177-224
+        const double grind1 = ((elapsed * 1e6) / cycle) / ((double)nx * nx * nx);
+        const double grind2 = grind1 / ndom;
+        printf("Run completed:  \n");
+        printf("   Problem size        =  %i \n", nx);
+        printf("   MPI tasks           =  %i \n", ndom);
+        printf("   Iteration count     =  %i \n", cycle);
+        printf("   Final Origin Energy = %12.6e \n", v[0]);
+        printf("   Testing Plane 0 of Energy Array on rank 0:\n");
+        printf("        MaxAbsDiff   = %12.6e\n", v[1]);
+        printf("        TotalAbsDiff = %12.6e\n", v[2]);
+        printf("        MaxRelDiff   = %12.6e\n\n", v[3]);
+        printf("\nElapsed time         = %10.2f (s)\n", elapsed);
+        printf("Grind time (us/z/c)  = %10.8g (per dom)  (%10.8g overall)\n", grind1, grind2);
+        printf("FOM                  = %10.8g (z/s)\n\n", 1000.0 / grind2);
+        printf("FullPrecision OriginEnergy = %.17e\n", v[0]);
+    }
+    lulesh_group_destroy(g);
+    return 0;
+}

@@ -1,0 +1,277 @@
+/*
+ * lulesh_amd.h -- C ABI of the MI355X LULESH Lagrange step and its halo
+ * exchange.
+ *
+ * The library advances LULESH 2.0 Sedov problems (FP64) on one AMD Instinct
+ * MI355X (gfx950) with hand-written HIP kernels.  A *group* is one
+ * decomposed problem: tp^3 subdomains of nx^3 elements, all resident in the
+ * same GPU's HBM; tp = 1 is the plain single-domain problem.  Subdomains
+ * exchange ghost planes through the reference's comm-backend interface
+ * (CommRecv / CommSend / CommSBN / CommSyncPosVel / CommMonoQ), served by
+ * an on-device "device_local" backend: pack kernels write straight into the
+ * receiving subdomain's receive buffer at the reference's wire offsets, and
+ * unpack kernels apply the messages in the reference's slot order, so the
+ * result is bit-reproducible.
+ *
+ * All entry points return an int status (enum lulesh_status); nothing in
+ * the library calls exit().  Plain pointers and sizes only.
+ */
+#ifndef LULESH_AMD_H
+#define LULESH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Opaque HIP stream (hipStream_t); NULL = the library's own stream. */
+typedef void *lulesh_stream_t;
+
+typedef struct lulesh_group  lulesh_group;   /* one decomposed problem        */
+typedef struct lulesh_domain lulesh_domain;  /* one subdomain of a group      */
+typedef struct lulesh_batch  lulesh_batch;   /* ensemble of independent groups */
+typedef struct lulesh_server lulesh_server;  /* continuous-batching server     */
+
+/* Status codes.  VOLUME / QSTOP keep the reference's exit values
+ * (LULESH/omp_4.0/lulesh.h:41, enum { VolumeError = -1, QStopError = -2 }). */
+enum lulesh_status {
+    LULESH_OK           = 0,
+    LULESH_VOLUME_ERROR = -1,   /* negative element volume            */
+    LULESH_QSTOP_ERROR  = -2,   /* q above qstop                      */
+    LULESH_EINVAL       = -3,   /* bad argument                       */
+    LULESH_EHIP         = -4,   /* HIP runtime error                  */
+    LULESH_ENOMEM       = -5,   /* allocation / HBM budget            */
+    LULESH_EDIRECTION   = -6,   /* no message expected from a direction
+                                   (shmRecvOffset abort, cuda
+                                   comm/comm_backend.h:134-137)      */
+    LULESH_EBUSY        = -7    /* server queue full                  */
+};
+
+/* Field selector; replaces the reference's Domain_member pointer-to-member
+ * (LULESH/cuda/src/lulesh.h:300).  Node fields are < 16. */
+enum lulesh_field {
+    LULESH_X = 0, LULESH_Y = 1, LULESH_Z = 2,
+    LULESH_XD = 3, LULESH_YD = 4, LULESH_ZD = 5,
+    LULESH_FX = 9, LULESH_FY = 10, LULESH_FZ = 11,
+    LULESH_NODAL_MASS = 12,
+    LULESH_E = 16, LULESH_P = 17, LULESH_Q = 18,
+    LULESH_V = 21, LULESH_VOLO = 22, LULESH_DELV = 23, LULESH_VDOV = 24,
+    LULESH_AREALG = 25, LULESH_SS = 26, LULESH_ELEM_MASS = 27,
+    LULESH_DELV_XI = 28, LULESH_DELV_ETA = 29, LULESH_DELV_ZETA = 30,
+    LULESH_DELX_XI = 31, LULESH_DELX_ETA = 32, LULESH_DELX_ZETA = 33,
+    LULESH_VNEW = 34
+};
+
+/* Message tags, LULESH/omp_4.0/lulesh.h:92-94. */
+#define LULESH_MSG_COMM_SBN     1024
+#define LULESH_MSG_SYNC_POS_VEL 2048
+#define LULESH_MSG_MONOQ        3072
+
+/* ------------------------------------------------------------------ */
+/* library                                                             */
+/* ------------------------------------------------------------------ */
+
+/* Selects the device (0 on a one-GPU box) and checks it is gfx950. */
+int lulesh_init(int device);
+const char *lulesh_status_string(int status);
+const char *lulesh_version(void);
+
+/* ------------------------------------------------------------------ */
+/* problems                                                            */
+/* ------------------------------------------------------------------ */
+
+/* Sedov set-up of tp^3 subdomains with nx^3 elements each, nr regions,
+ * balance and cost as the reference's -s/-r/-b/-c options
+ * (LULESH/omp_4.0/lulesh-init.cc:16-184, lulesh-util.cc:60-173).  Replaces
+ * Domain::Domain + InitMeshDecomp + the initial nodalMass exchange
+ * (lulesh.cc:3140-3161). */
+int lulesh_group_create(int nx, int tp, int nr, int balance, int cost,
+                        lulesh_group **out);
+int lulesh_group_destroy(lulesh_group *g);
+int lulesh_group_num_domains(const lulesh_group *g);
+int lulesh_group_domain(lulesh_group *g, int rank, lulesh_domain **out);
+
+/* Overrides for tests and drivers. */
+int lulesh_group_set_stoptime(lulesh_group *g, double stoptime);
+int lulesh_group_set_qstop(lulesh_group *g, double qstop);
+/* Cap on the cycle count, the reference's -i option (default: none). */
+int lulesh_group_set_max_cycles(lulesh_group *g, int max_cycles);
+
+/* Runs the time loop of main (lulesh.cc:3222-3235) on the device until
+ * time >= stoptime, the cycle cap, or an error.  cycles_per_sync cycles are
+ * enqueued (as one captured HIP graph) between host checks of the device
+ * control block; 0 picks a default.  Returns the status. */
+int lulesh_group_run(lulesh_group *g, int cycles_per_sync, lulesh_stream_t stream);
+
+/* Enqueues exactly `ncycles` leapfrog cycles (each one a TimeIncrement +
+ * LagrangeLeapFrog, lulesh.cc:3224-3225) without waiting.  Cycles after the
+ * stop condition are no-ops on the device. */
+int lulesh_group_enqueue(lulesh_group *g, int ncycles, lulesh_stream_t stream);
+/* Waits for the stream and reports the status, cycle count and time of
+ * rank 0. */
+int lulesh_group_sync(lulesh_group *g, int *cycle, double *time,
+                      double *deltatime, lulesh_stream_t stream);
+
+/* Runs `ncycles` cycles launch by launch on the group's stream with a HIP
+ * event pair around each kernel class and returns the summed milliseconds:
+ * ms[0] time increment, [1] element forces, [2] node gather (+ update when
+ * fused), [3] node update, [4] kinematics + gradients, [5] MonoQ + EOS +
+ * dt, [6] halo exchanges.  These are real cycles of the run. */
+int lulesh_group_profile(lulesh_group *g, int ncycles, double ms[7]);
+
+/* Final-output figures of VerifyAndWriteFinalOutput
+ * (LULESH/omp_4.0/lulesh-util.cc:177-224) for subdomain `rank`:
+ * out[0] origin energy e(0), out[1] MaxAbsDiff, out[2] TotalAbsDiff,
+ * out[3] MaxRelDiff of the plane-0 energy symmetry. */
+int lulesh_group_verify(lulesh_group *g, int rank, double out[4]);
+
+/* ------------------------------------------------------------------ */
+/* subdomains                                                          */
+/* ------------------------------------------------------------------ */
+
+int lulesh_domain_rank(const lulesh_domain *d);
+/* elements per edge, element count, node count, element count incl. the
+ * six ghost planes of the velocity gradients */
+int lulesh_domain_sizes(const lulesh_domain *d, int *nx, int *num_elem,
+                        int *num_node, int *all_elem);
+/* Host copies of a field (n = number of doubles the buffer holds). */
+int lulesh_domain_get_field(lulesh_domain *d, int field, double *host, size_t n);
+int lulesh_domain_set_field(lulesh_domain *d, int field, const double *host, size_t n);
+/* Device pointer of a field (stays valid for the life of the group). */
+int lulesh_domain_field_ptr(lulesh_domain *d, int field, double **dev_ptr);
+/* Region bookkeeping of CreateRegionIndexSets (lulesh-init.cc:377-484) and
+ * the per-element repetition count of main (lulesh.cc:3171-3192). */
+int lulesh_domain_get_elem_rep(lulesh_domain *d, int *host, size_t n);
+int lulesh_domain_get_region(lulesh_domain *d, int *host, size_t n);
+
+/* ------------------------------------------------------------------ */
+/* halo exchange: the reference's comm interface                       */
+/* ------------------------------------------------------------------ */
+
+/* Same names, argument meaning and call order as lulesh-comm.cc
+ * (LULESH/omp_4.0/lulesh-comm.cc:59, 357, 848, 1261, 1684) and the GPU
+ * forms of LULESH/cuda/src/lulesh.h:321-340.  `fields` holds lulesh_field
+ * values in place of Domain_member pointers.  Within one exchange step the
+ * caller issues CommSend for every subdomain of the group before any
+ * unpack (CommSBN / CommSyncPosVel / CommMonoQ), which is the ordering MPI
+ * gives the reference.  All work is enqueued on `stream`. */
+int CommRecv(lulesh_domain *d, int msgType, int xferFields,
+             int dx, int dy, int dz, int doRecv, int planeOnly);
+int CommSend(lulesh_domain *d, int msgType, int xferFields, const int *fields,
+             int dx, int dy, int dz, int doSend, int planeOnly,
+             lulesh_stream_t stream);
+int CommSBN(lulesh_domain *d, int xferFields, const int *fields,
+            lulesh_stream_t stream);
+int CommSyncPosVel(lulesh_domain *d, lulesh_stream_t stream);
+int CommMonoQ(lulesh_domain *d, lulesh_stream_t stream);
+
+/* Receive-buffer wire layout: comBufSize of SetupCommBuffers
+ * (lulesh-init.cc:317-372) and the receiver offset of a message from
+ * direction (dcol,drow,dplane) (shmRecvOffset, cuda
+ * comm/comm_backend.h:75-138); -1 = no message expected. */
+int lulesh_comm_buf_size(const lulesh_domain *d);
+int lulesh_comm_recv_offset(const lulesh_domain *d, int dcol, int drow, int dplane,
+                            int xferFields, int doRecv, int planeOnly);
+/* Host copy of a subdomain's receive buffer (comBufSize doubles). */
+int lulesh_comm_get_recv_buffer(lulesh_domain *d, double *host, size_t n);
+
+/* Backend hook table: the run-time form of the compile-time backend macros
+ * of LULESH/cuda/src/comm/comm_backend.h:7-22,156-173 (commAllocRecv,
+ * commTeardown, COMM_RECV_SKIP, COMM_PACK_DEST, COMM_SEND_FINISH,
+ * COMM_RECV_BASE).  The library ships one backend, "device_local", for
+ * subdomains that share one GPU's HBM. */
+typedef struct lulesh_comm_backend_ops {
+    const char *name;
+    /* receive buffer of `comBufSize` doubles for subdomain d */
+    int (*alloc_recv)(lulesh_domain *d, size_t comBufSize, double **recv, void *ctx);
+    int (*teardown)(lulesh_domain *d, double *recv, void *ctx);
+    /* where the pack kernels of sender `from` write the message to
+     * receiver `to` that lands at `offset` doubles into its buffer */
+    double *(*pack_dest)(lulesh_domain *from, lulesh_domain *to, size_t offset, void *ctx);
+    /* completion of all sends of `from` (stream-ordered for device_local) */
+    int (*send_finish)(lulesh_domain *from, lulesh_stream_t stream, void *ctx);
+    /* buffer the unpack kernels of d read */
+    double *(*recv_base)(lulesh_domain *d, void *ctx);
+} lulesh_comm_backend_ops;
+
+const lulesh_comm_backend_ops *lulesh_device_local_backend(void);
+/* Installs a backend on a group before its first exchange (the set-up
+ * exchange of lulesh_group_create uses device_local). */
+int lulesh_group_set_comm_backend(lulesh_group *g, const lulesh_comm_backend_ops *ops,
+                                  void *ctx);
+/* One whole exchange step on every subdomain of the group: CommSend on
+ * all, then the matching unpack on all, through the installed backend. */
+int lulesh_group_exchange(lulesh_group *g, int msgType, int xferFields,
+                          const int *fields, lulesh_stream_t stream);
+
+/* ------------------------------------------------------------------ */
+/* batched ensemble                                                    */
+/* ------------------------------------------------------------------ */
+
+/* Many independent single-domain Sedov problems advanced together: every
+ * kernel of a cycle covers all problems through one tile table, each
+ * problem keeps its own dt / time / cycle, and one cycle of the whole
+ * ensemble is replayed as one captured HIP graph. */
+int lulesh_batch_create(int nprob, const int *nx, const int *nr,
+                        const int *balance, const int *cost, lulesh_batch **out);
+int lulesh_batch_destroy(lulesh_batch *b);
+int lulesh_batch_set_max_cycles(lulesh_batch *b, int max_cycles);
+/* Runs every problem to its stop condition; returns the first error. */
+int lulesh_batch_run(lulesh_batch *b, int cycles_per_sync, lulesh_stream_t stream);
+int lulesh_batch_group(lulesh_batch *b, int i, lulesh_group **out);
+/* per-problem cycles and status after a run */
+int lulesh_batch_results(lulesh_batch *b, int *cycles, int *status);
+/* Element/block tile shapes of the batched kernels (the autotuner's
+ * output): tile_elems = elements per workgroup for element kernels,
+ * tile_nodes for node kernels (64, 128, 192 or 256); 0 keeps the current
+ * value. */
+int lulesh_batch_set_tiles(lulesh_batch *b, int tile_elems, int tile_nodes);
+/* Enqueue `ncycles` cycles of the whole ensemble without waiting / wait. */
+int lulesh_batch_enqueue(lulesh_batch *b, int ncycles);
+int lulesh_batch_synchronize(lulesh_batch *b);
+/* lulesh_group_profile for the whole ensemble */
+int lulesh_batch_profile(lulesh_batch *b, int ncycles, double ms[7]);
+
+/* ------------------------------------------------------------------ */
+/* serving                                                             */
+/* ------------------------------------------------------------------ */
+
+/* Continuous-batching server: requests are admitted into slots while the
+ * free-HBM budget allows, their state stays resident, all admitted
+ * problems are stepped together, and finished ones free their slot. */
+int lulesh_server_create(size_t hbm_budget_bytes, int max_slots, lulesh_server **out);
+int lulesh_server_destroy(lulesh_server *s);
+/* Queues a request; returns its id (>= 0) or a status < 0. */
+int lulesh_server_submit(lulesh_server *s, int nx, int nr, int balance, int cost,
+                         double arrival_time);
+/* Admits what fits, advances all admitted problems by `cycles` cycles and
+ * retires finished ones; returns the number of requests completed in this
+ * call. */
+int lulesh_server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream);
+/* Result of a completed request: status, cycle count, origin energy, and
+ * the TotalAbsDiff symmetry figure. */
+int lulesh_server_result(lulesh_server *s, int id, int *status, int *cycles,
+                         double *origin_energy, double *total_abs_diff);
+int lulesh_server_stats(lulesh_server *s, int *queued, int *active, int *done,
+                        size_t *hbm_in_use);
+/* Server clock (seconds since lulesh_server_create); arrival times of
+ * lulesh_server_submit are on this clock and a request is admitted only
+ * once it has arrived.  lulesh_server_times gives admission and completion
+ * times of a request on the same clock. */
+double lulesh_server_now(lulesh_server *s);
+int lulesh_server_times(lulesh_server *s, int id, double *t_admit, double *t_done);
+
+/* ------------------------------------------------------------------ */
+/* arithmetic probes (parity tests of the device math)                 */
+/* ------------------------------------------------------------------ */
+/* out[i] = correctly rounded cube root of in[i], computed on the GPU */
+int lulesh_probe_cbrt(const double *in, double *out, int n);
+/* out2[2i] = sqrt(a[i]), out2[2i+1] = a[i] / b[i], computed on the GPU */
+int lulesh_probe_sqrt_div(const double *a, const double *b, double *out2, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LULESH_AMD_H */

@@ -1,0 +1,106 @@
+"""Single-domain parity of the HIP path against the CPU oracle.
+
+The oracle in LO_CBRT_EXACT mode uses the correctly rounded cube root the
+kernels compute; everything else is the reference's arithmetic in the
+reference's order, so the GPU state must match it bit for bit.  Against the
+oracle in libm mode (= the compiled reference) the bar is the origin-energy
+tolerance of 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+NODE = ["x", "y", "z", "xd", "yd", "zd"]
+ELEM = ["e", "p", "q", "v", "ss", "volo", "delv", "vdov", "arealg", "vnew",
+        "delx_xi", "delx_eta", "delx_zeta"]
+
+
+def test_probe_cbrt_matches_exact(amd, oracle_mod):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([
+        rng.uniform(1e-9, 1e-2, 20000), rng.uniform(0.5, 8.0, 20000),
+        np.exp(rng.uniform(-700, 700, 20000)), -rng.uniform(1e-3, 1e3, 2000),
+        np.array([1.0, 8.0, 27.0, 0.125, 2.0, 7.999999999999999, 1e-300, 5e-324, 1.7e308]),
+    ])
+    got = amd.probe_cbrt(x)
+    want = np.array([oracle_mod.cbrt_exact(v) for v in x])
+    assert_bitwise(got, want, "cbrt")
+
+
+def test_probe_sqrt_div_ieee(amd):
+    rng = np.random.default_rng(2)
+    a = np.exp(rng.uniform(-300, 300, 50000))
+    b = np.exp(rng.uniform(-300, 300, 50000))
+    s, q = amd.probe_sqrt_div(a, b)
+    assert_bitwise(s, np.sqrt(a), "sqrt")
+    assert_bitwise(q, a / b, "div")
+
+
+def _compare_state(g, o, what):
+    for f in NODE:
+        assert_bitwise(g.field(f), o.field(f), f"{what} {f}")
+    for f in ELEM:
+        assert_bitwise(g.field(f), o.field(f), f"{what} {f}")
+
+
+@pytest.mark.parametrize("nx,cycles", [(4, 1), (4, -1), (8, 5), (8, -1), (16, 40)])
+def test_single_domain_bitwise(amd, oracle_mod, nx, cycles):
+    o = oracle_mod.Oracle(nx, cbrt_mode=oracle_mod.CBRT_EXACT)
+    assert o.run(cycles) == 0
+    g = amd.Group(nx)
+    if cycles >= 0:
+        g.set_max_cycles(cycles)
+    assert g.run() == 0
+    rc, cyc, t, dt = g.sync()
+    assert cyc == o.cycle
+    assert t == o.time and dt == o.deltatime
+    _compare_state(g, o, f"s={nx} cycles={cycles}")
+    g.close()
+    o.close()
+
+
+def test_s16_full_run_bitwise(amd, oracle_mod):
+    o = oracle_mod.Oracle(16, cbrt_mode=oracle_mod.CBRT_EXACT)
+    o.run()
+    g = amd.Group(16)
+    assert g.run() == 0
+    _compare_state(g, o, "s=16 full")
+    vg, vo = g.verify(), o.verify()
+    for k in vg:
+        assert vg[k] == vo[k], k
+
+
+def test_rep_and_region_invariance(amd):
+    # results never depend on -r/-b/-c: the EOS repetitions recompute the
+    # same values (LULESH/omp_4.0/lulesh.cc:2677-2815)
+    base = amd.Group(8, nr=11, balance=1, cost=1)
+    base.run()
+    for nr, b, c in [(1, 1, 1), (5, 2, 2), (16, 4, 1)]:
+        g = amd.Group(8, nr=nr, balance=b, cost=c)
+        g.run()
+        assert_bitwise(g.field("e"), base.field("e"), f"r={nr} b={b} c={c}")
+        assert_bitwise(g.field("x"), base.field("x"), f"r={nr} b={b} c={c}")
+
+
+def test_deterministic(amd):
+    a = amd.Group(12)
+    a.run()
+    b = amd.Group(12)
+    b.run()
+    for f in ["e", "p", "q", "x", "xd"]:
+        assert_bitwise(a.field(f), b.field(f), f)
+
+
+def test_s30_energy_vs_reference(amd, oracle_mod):
+    o = oracle_mod.Oracle(30, cbrt_mode=oracle_mod.CBRT_LIBM)
+    o.run()
+    g = amd.Group(30)
+    assert g.run() == 0
+    vg, vo = g.verify(), o.verify()
+    rel = abs(vg["origin_energy"] - vo["origin_energy"]) / abs(vo["origin_energy"])
+    assert rel <= 1e-10, rel
+    assert abs(vg["total_abs_diff"] - vo["total_abs_diff"]) <= 1e-8
+    assert g.sync()[1] == o.cycle
