@@ -38,11 +38,16 @@ HBM_PEAK_GBS = 8000.0
 
 # algorithmic (compulsory) bytes per launch of each kernel class, per
 # element E and node N of the launch; see DESIGN.md
-KERNEL_NAMES = ["time_increment", "elem_force", "node_gather_update", "node_update",
+KERNEL_NAMES = ["time_increment", "force_gather", "node_gather_update", "node_update",
                 "kinematics", "qeos", "halo_exchange"]
 
 
+TWOPASS = os.environ.get("LULESH_FORCE_PATH") == "twopass"
+
+
 def kernel_bytes(k, E, N):
+    if k == 1 and not TWOPASS:   # fused force+gather: x..zd, 6 elem doubles in, f out
+        return 48 * N + 48 * E + 24 * N
     if k == 1:   # x..zd of 8 nodes (once per node), 6 elem doubles in, 48 corner doubles out
         return 48 * N + 48 * E + 384 * E
     if k == 2:   # 48 corner doubles in, nodalMass + x..zd in, x..zd out
@@ -166,7 +171,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=100)
     ap.add_argument("--chunk", type=int, default=50, help="cycles per captured graph launch")
-    ap.add_argument("--mode", default="plain", choices=["plain"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "copy"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -174,6 +179,13 @@ def main():
               "(subdomains of a decomposition share one GPU's HBM instead)", file=sys.stderr)
         raise SystemExit(2)
     lulesh_amd.init(0)
+    if args.mode == "copy":
+        # streaming-copy yardstick: 2^26 doubles (512 MiB) read + written
+        n = 1 << 26
+        ms = lulesh_amd.probe_copy_ms(n, max(1, args.steps))
+        print(json.dumps({"mode": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n,
+                          "ms": ms, "GBs": 16 * n / (ms * 1e-3) / 1e9}), flush=True)
+        return
     line = bench_plain(args)
     print(json.dumps(line), flush=True)
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -37,6 +38,14 @@ namespace lulesh {
 // ---------------------------------------------------------------------------
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// The per-element corner-force array (48 doubles per element) is needed
+// only by the two-pass force path, kept for cross-checking the fused one.
+bool corner_path_enabled()
+{
+    const char *fp = getenv("LULESH_FORCE_PATH");
+    return fp && !strcmp(fp, "twopass");
+}
+
 int domain_alloc(lulesh_domain *d)
 {
     const HostDomain &h = d->host;
@@ -50,7 +59,7 @@ int domain_alloc(lulesh_domain *d)
         {&D.vdov, ne}, {&D.arealg, ne}, {&D.ss, ne}, {&D.elemMass, ne}, {&D.vnew, ne},
         {&D.delv_xi, all}, {&D.delv_eta, all}, {&D.delv_zeta, all},
         {&D.delx_xi, ne}, {&D.delx_eta, ne}, {&D.delx_zeta, ne},
-        {&D.corner, (size_t)kCornerComps * ne}};
+        {&D.corner, corner_path_enabled() ? (size_t)kCornerComps * ne : 0}};
     size_t bytes = 0;
     for (auto &s : slots) bytes += align256(s.n * sizeof(double));
     bytes += align256(ne * sizeof(int));
@@ -759,6 +768,30 @@ int lulesh_probe_cbrt(const double *host_in, double *host_out, int n)
     (void)hipFree(din);
     (void)hipFree(dout);
     return LULESH_OK;
+}
+
+double lulesh_probe_copy(size_t n, int reps)
+{
+    if (n < 2 || reps < 1) return -1.0;
+    double *a = nullptr, *b = nullptr;
+    if (hipMalloc((void **)&a, n * 8) != hipSuccess) return -1.0;
+    if (hipMalloc((void **)&b, n * 8) != hipSuccess) { (void)hipFree(a); return -1.0; }
+    (void)hipMemset(a, 0, n * 8);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)launch_probe_copy(a, b, n, 0);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r) (void)launch_probe_copy(a, b, n, 0);
+    (void)hipEventRecord(e1, 0);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return ms / reps;
 }
 
 int lulesh_probe_sqrt_div(const double *a, const double *b, double *out2, int n)

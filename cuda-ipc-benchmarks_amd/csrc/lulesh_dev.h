@@ -60,6 +60,13 @@ struct Tile {
     int start;
 };
 
+// column tile of the fused force kernel: 15x15 nodes from (i0, j0), node
+// planes [k0, k0 + kchunk)
+struct ColTile {
+    int dom;
+    int i0, j0, k0;
+};
+
 // physics constants of Domain::Domain (LULESH/omp_4.0/lulesh-init.cc:20-38)
 struct Phys {
     static constexpr double e_cut = 1.0e-7, p_cut = 1.0e-7, q_cut = 1.0e-7;

@@ -53,9 +53,42 @@ static void engine_drop_graph(Engine *e)
     e->graph_cycles = 0;
 }
 
+// node planes per column tile: enough workgroups to fill 256 CUs twice over
+// a few times, but never fewer than 4 planes (the bottom element plane of a
+// chunk is recomputed by its owner and the chunk below)
+static int auto_kchunk(const Engine *e)
+{
+    long cols = 0, planes = 0;
+    for (auto *g : e->groups)
+        for (auto *d : g->doms) {
+            const int c = (d->host.nx + 1 + 14) / 15;
+            cols += (long)c * c;
+            planes = std::max<long>(planes, d->host.nx + 1);
+        }
+    const long target = 2048;
+    long k = (cols * planes + target - 1) / target;
+    return (int)std::max<long>(4, std::min<long>(k, 32));
+}
+
 static int build_tiles(Engine *e)
 {
     std::vector<Tile> et, nt;
+    std::vector<ColTile> ct;
+    const int kc = e->kchunk > 0 ? e->kchunk : auto_kchunk(e);
+    for (int di = 0, gi = 0; gi < (int)e->groups.size(); ++gi)
+        for (auto *d : e->groups[gi]->doms) {
+            const int s1 = d->host.nx + 1;
+            for (int k0 = 0; k0 < s1; k0 += kc)
+                for (int j0 = 0; j0 < s1; j0 += 15)
+                    for (int i0 = 0; i0 < s1; i0 += 15) ct.push_back(ColTile{di, i0, j0, k0});
+            ++di;
+        }
+    e->kchunk_used = kc;
+    if (e->d_ctiles) (void)hipFree(e->d_ctiles);
+    e->d_ctiles = nullptr;
+    e->n_ctiles = (int)ct.size();
+    HIPCHK(hipMalloc((void **)&e->d_ctiles, std::max<size_t>(1, ct.size()) * sizeof(ColTile)));
+    if (!ct.empty()) HIPCHK(hipMemcpy(e->d_ctiles, ct.data(), ct.size() * sizeof(ColTile), hipMemcpyHostToDevice));
     for (int di = 0, gi = 0; gi < (int)e->groups.size(); ++gi)
         for (auto *d : e->groups[gi]->doms) {
             for (int s = 0; s < d->host.numElem; s += e->tile_e) et.push_back(Tile{di, s});
@@ -80,6 +113,10 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
     e->groups = groups;
     const char *ng = getenv("LULESH_NO_GRAPH");
     e->use_graph = !(ng && ng[0] == '1');
+    const char *fp = getenv("LULESH_FORCE_PATH");
+    e->march = !(fp && !strcmp(fp, "twopass"));
+    const char *kc = getenv("LULESH_KCHUNK");
+    if (kc) e->kchunk = atoi(kc);
     std::vector<DomDev> doms;
     std::vector<GroupCtl *> gptr;
     for (auto *g : groups) {
@@ -120,6 +157,7 @@ void engine_destroy(Engine *e)
     if (e->d_groups) (void)hipFree(e->d_groups);
     if (e->d_etiles) (void)hipFree(e->d_etiles);
     if (e->d_ntiles) (void)hipFree(e->d_ntiles);
+    if (e->d_ctiles) (void)hipFree(e->d_ctiles);
     delete e;
 }
 
@@ -186,12 +224,23 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
     const int ng = (int)e->groups.size();
     int rc = LULESH_OK;
     STEP(0, HIPCHK(launch_time_increment(e->d_groups, ng, s)));
-    STEP(1, HIPCHK(launch_elem_force(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
-    STEP(2, HIPCHK(launch_node_gather(e->fused, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
-    if (!e->fused) {
-        STEP(6, rc = exchange_all(e, LULESH_MSG_COMM_SBN, 3, kSbn, s));
-        if (rc) return rc;
+    if (e->march) {
+        STEP(1, HIPCHK(launch_force_march(e->d_doms, e->d_ctiles, e->n_ctiles, e->kchunk_used, s)));
+        if (!e->fused) {
+            STEP(6, rc = exchange_all(e, LULESH_MSG_COMM_SBN, 3, kSbn, s));
+            if (rc) return rc;
+        }
         STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+    } else {
+        STEP(1, HIPCHK(launch_elem_force(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
+        STEP(2, HIPCHK(launch_node_gather(e->fused, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+        if (!e->fused) {
+            STEP(6, rc = exchange_all(e, LULESH_MSG_COMM_SBN, 3, kSbn, s));
+            if (rc) return rc;
+            STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+        }
+    }
+    if (!e->fused) {
         STEP(6, rc = exchange_all(e, LULESH_MSG_SYNC_POS_VEL, 6, kPosVel, s));
         if (rc) return rc;
     }
@@ -205,6 +254,14 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
 }
 
 int engine_enqueue_cycle(Engine *e, hipStream_t s) { return enqueue_cycle(e, s, nullptr); }
+
+int engine_set_kchunk(Engine *e, int kchunk)
+{
+    if (kchunk < 0) return LULESH_EINVAL;
+    e->kchunk = kchunk;
+    engine_drop_graph(e);
+    return build_tiles(e);
+}
 
 int engine_profile(Engine *e, int ncycles, hipStream_t s, double ms[7])
 {
@@ -414,7 +471,7 @@ struct Result {
 
 // device footprint of one admitted single-domain problem: its field arena
 // plus control block and tile-table slack
-size_t request_bytes(int nx) { return domain_device_bytes(nx) + 4096; }
+size_t request_bytes(int nx) { return domain_device_bytes(nx, corner_path_enabled()) + 4096; }
 
 }  // namespace
 

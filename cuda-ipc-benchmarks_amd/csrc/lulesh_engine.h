@@ -66,6 +66,13 @@ struct Engine {
     Tile *d_etiles = nullptr, *d_ntiles = nullptr;
     int n_etiles = 0, n_ntiles = 0;
     int tile_e = 256, tile_n = 256;
+    // fused force + gather (z-marching column tiles) or the two-pass
+    // corner-array path
+    bool march = true;
+    int kchunk = 0;            // node planes per column tile (0 = auto)
+    int kchunk_used = 8;
+    ColTile *d_ctiles = nullptr;
+    int n_ctiles = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_cycles = 0;
@@ -76,6 +83,7 @@ struct Engine {
 int engine_create(const std::vector<lulesh_group *> &groups, Engine **out);
 void engine_destroy(Engine *e);
 int engine_set_tiles(Engine *e, int tile_e, int tile_n);
+int engine_set_kchunk(Engine *e, int kchunk);
 int engine_enqueue(Engine *e, int ncycles, hipStream_t s);
 int engine_enqueue_cycle(Engine *e, hipStream_t s);
 int engine_profile(Engine *e, int ncycles, hipStream_t s, double ms[7]);
@@ -91,5 +99,6 @@ int group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, hipS
 int group_plan(lulesh_group *g, int msgType, int nf, const int *fields, bool doFlag, bool planeOnly,
                DevPlan **out);
 double *field_dev_ptr(lulesh_domain *d, int f, size_t *len);
+bool corner_path_enabled();
 
 }  // namespace lulesh

@@ -9,6 +9,8 @@ namespace lulesh {
 hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, hipStream_t s);
 hipError_t launch_elem_force(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+// fused force + ordered gather (16x16-thread column tiles, kchunk node planes)
+hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, hipStream_t s);
 hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_kinematics(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
@@ -17,6 +19,7 @@ hipError_t launch_pack(const PackEnt *e, int n, const double *const *fptr, int n
                        const GroupCtl *gate, hipStream_t s);
 hipError_t launch_unpack(const UnpackTgt *t, int n, const UnpackSrc *src, double *const *fptr,
                          int nfields, int mode, const GroupCtl *gate, hipStream_t s);
+hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s);
 hipError_t launch_probe_cbrt(const double *in, double *out, int n, hipStream_t s);
 hipError_t launch_probe_sqrt_div(const double *a, const double *b, double *out, int n, hipStream_t s);
 

@@ -201,6 +201,233 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused force + ordered node gather, z-marching (the production path).
+//
+// A workgroup owns a column of 15 x 15 nodes and a chunk of node planes
+// [k0, k1).  Its 16 x 16 threads are the elements of the column's halo-
+// extended element tile; the block walks element planes c = k0-1 .. k1-1.
+// At each plane every thread computes its element's 8 stress corner forces
+// and 8 hourglass corner forces (the same arithmetic as k_elem_force) into
+// LDS, and each node thread adds its four neighbouring elements' corners:
+// corners 2,3,1,0 of plane c complete node plane c, corners 6,7,5,4 start
+// node plane c+1.  That is exactly the reference's per-node order (elements
+// ascending: plane c-1 elements (i-1,j-1),(i,j-1),(i-1,j),(i,j), then the
+// same four of plane c), summed from 0.0, stress and hourglass separately,
+// f = stress + hourglass -- so the result is the reference's to the bit,
+// without the 48-doubles-per-element corner array ever touching HBM.
+// Elements outside the mesh contribute +0.0, which leaves a sum that
+// started from +0.0 unchanged.
+// ---------------------------------------------------------------------------
+constexpr int kMT = 16;        // element tile edge = threads per row
+constexpr int kMN = kMT - 1;   // node tile edge
+
+__device__ __forceinline__ void load_elem_xyz(const DomDev &D, const int n[8], double x[8], double y[8],
+                                              double z[8])
+{
+#pragma unroll
+    for (int c = 0; c < 8; ++c) { x[c] = D.x[n[c]]; y[c] = D.y[n[c]]; z[c] = D.z[n[c]]; }
+}
+
+// stress corner forces -(sigma B) from the face normals
+// (IntegrateStressForElems, lulesh.cc:546-675); returns the Jacobian volume
+__device__ __forceinline__ double elem_stress(const double x[8], const double y[8], const double z[8],
+                                              double sig, double fx[8], double fy[8], double fz[8])
+{
+    const Jac J = elem_jacobian(x, y, z);
+    double bx[8], by[8], bz[8];
+    node_normals(x, y, z, bx, by, bz);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        fx[c] = -(sig * bx[c]);
+        fy[c] = -(sig * by[c]);
+        fz[c] = -(sig * bz[c]);
+    }
+    return J.volume;
+}
+
+// FB hourglass corner forces (CalcFBHourglassForceForElems,
+// lulesh.cc:906-1056) of one velocity component at a time
+struct HgModes {
+    double g[8][4];
+    double coefficient;
+};
+
+__device__ __forceinline__ void hg_modes(const double x[8], const double y[8], const double z[8],
+                                         double determ, double ss, double mass, HgModes &H)
+{
+    double dvdx[8], dvdy[8], dvdz[8];
+    volume_derivs(x, y, z, dvdx, dvdy, dvdz);
+    const double volinv = 1.0 / determ;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        double hx = x[0] * hg_gamma(m, 0), hy = y[0] * hg_gamma(m, 0), hz = z[0] * hg_gamma(m, 0);
+#pragma unroll
+        for (int c = 1; c < 8; ++c) {
+            hx = hx + x[c] * hg_gamma(m, c);
+            hy = hy + y[c] * hg_gamma(m, c);
+            hz = hz + z[c] * hg_gamma(m, c);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            H.g[c][m] = hg_gamma(m, c) - volinv * (dvdx[c] * hx + dvdy[c] * hy + dvdz[c] * hz);
+    }
+    const double volume13 = cbrt_cr(determ);
+    H.coefficient = -Phys::hgcoef * 0.01 * ss * mass / volume13;
+}
+
+__device__ __forceinline__ void hg_force(const HgModes &H, const double vl[8], double out[8])
+{
+    double h[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        double acc = H.g[0][m] * vl[0];
+#pragma unroll
+        for (int c = 1; c < 8; ++c) acc = acc + H.g[c][m] * vl[c];
+        h[m] = acc;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        out[c] = H.coefficient * (H.g[c][0] * h[0] + H.g[c][1] * h[1] + H.g[c][2] * h[2] + H.g[c][3] * h[3]);
+}
+
+__global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, const ColTile *tiles,
+                                                        int ntiles, int kchunk)
+{
+    __shared__ double lds[24][kMT * kMT];
+    const ColTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const DomDev &D = doms[t.dom];
+    GroupCtl *G = D.grp;
+    if (group_stopped(*G)) return;
+    const int nx = D.nx, s1 = nx + 1, s2 = s1 * s1;
+    const int tid = threadIdx.x;
+    const int ta = tid & (kMT - 1), tb = tid / kMT;
+    const int a = t.i0 - 1 + ta, b = t.j0 - 1 + tb;
+    const bool col_ok = a >= 0 && a < nx && b >= 0 && b < nx;
+    const int ni = t.i0 + ta, nj = t.j0 + tb;
+    const bool node_ok = ta < kMN && tb < kMN && ni <= nx && nj <= nx;
+    const int k0 = t.k0;
+    const int k1 = min(t.k0 + kchunk, nx + 1);
+    // lower / upper corner slots seen from the node thread:
+    // own element, +x, +y, +x+y neighbours
+    const int t1 = tid + 1, t2 = tid + kMT, t3 = tid + kMT + 1;
+    double ps[3] = {0.0, 0.0, 0.0}, ph[3] = {0.0, 0.0, 0.0};   // node plane c+1 partials
+    double fs[3] = {0.0, 0.0, 0.0};                              // node plane c stress sums
+
+    for (int c = k0 - 1; c < k1; ++c) {
+        const bool el_ok = col_ok && c >= 0 && c < nx;
+        const int el = (c * nx + b) * nx + a;
+        int n[8];
+        double x[8], y[8], z[8];
+        if (el_ok) {
+            const int n0 = c * s2 + b * s1 + a;
+            n[0] = n0; n[1] = n0 + 1; n[2] = n0 + s1 + 1; n[3] = n0 + s1;
+            n[4] = n0 + s2; n[5] = n0 + s2 + 1; n[6] = n0 + s2 + s1 + 1; n[7] = n0 + s2 + s1;
+            load_elem_xyz(D, n, x, y, z);
+        }
+        // ---- stress corners ----
+        {
+            double fx[8], fy[8], fz[8];
+            if (el_ok) {
+                const double sig = -D.p[el] - D.q[el];
+                const double vol = elem_stress(x, y, z, sig, fx, fy, fz);
+                if (vol <= 0.0) raise_status(G, -1);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) fx[q] = fy[q] = fz[q] = 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                lds[0 * 8 + q][tid] = fx[q];
+                lds[1 * 8 + q][tid] = fy[q];
+                lds[2 * 8 + q][tid] = fz[q];
+            }
+        }
+        __syncthreads();
+        if (node_ok) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (c >= k0) {
+                    double v = ps[d];
+                    v += lds[d * 8 + 2][tid];
+                    v += lds[d * 8 + 3][t1];
+                    v += lds[d * 8 + 1][t2];
+                    v += lds[d * 8 + 0][t3];
+                    fs[d] = v;
+                }
+                if (c + 1 < k1) {
+                    double v = 0.0;
+                    v += lds[d * 8 + 6][tid];
+                    v += lds[d * 8 + 7][t1];
+                    v += lds[d * 8 + 5][t2];
+                    v += lds[d * 8 + 4][t3];
+                    ps[d] = v;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- hourglass corners ----
+        {
+            double hx[8], hy[8], hz[8];
+            if (el_ok) {
+                const double vv = D.v[el];
+                if (vv <= 0.0) raise_status(G, -1);
+                HgModes H;
+                hg_modes(x, y, z, D.volo[el] * vv, D.ss[el], D.elemMass[el], H);
+                double vl[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) vl[q] = D.xd[n[q]];
+                hg_force(H, vl, hx);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) vl[q] = D.yd[n[q]];
+                hg_force(H, vl, hy);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) vl[q] = D.zd[n[q]];
+                hg_force(H, vl, hz);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) hx[q] = hy[q] = hz[q] = 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                lds[0 * 8 + q][tid] = hx[q];
+                lds[1 * 8 + q][tid] = hy[q];
+                lds[2 * 8 + q][tid] = hz[q];
+            }
+        }
+        __syncthreads();
+        if (node_ok) {
+            double f[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (c >= k0) {
+                    double v = ph[d];
+                    v += lds[d * 8 + 2][tid];
+                    v += lds[d * 8 + 3][t1];
+                    v += lds[d * 8 + 1][t2];
+                    v += lds[d * 8 + 0][t3];
+                    f[d] = fs[d] + v;
+                }
+                if (c + 1 < k1) {
+                    double v = 0.0;
+                    v += lds[d * 8 + 6][tid];
+                    v += lds[d * 8 + 7][t1];
+                    v += lds[d * 8 + 5][t2];
+                    v += lds[d * 8 + 4][t3];
+                    ph[d] = v;
+                }
+            }
+            if (c >= k0) {
+                const int nd = c * s2 + nj * s1 + ni;
+                D.fx[nd] = f[0];
+                D.fy[nd] = f[1];
+                D.fz[nd] = f[2];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // CalcAccelerationForNodes + ApplyAccelerationBoundaryConditionsForNodes +
 // CalcVelocityForNodes + CalcPositionForNodes (lulesh.cc:1415-1550) for
 // node (i,j,k) of subdomain D
@@ -514,23 +741,45 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
     return v;
 }
 
+__device__ __forceinline__ int wave_incl_scan(int v, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // CalcMonotonicQRegionForElems + the qstop scan + EvalEOSForElems x rep +
 // CalcCourantConstraintForElems + CalcHydroConstraintForElems
-// (lulesh.cc:2249-2546, 2586-2836, 2874-3035), fused per element.  The
-// rep loop is executed rep times as in the reference; its inputs are
-// passed through an empty asm barrier each round so the compiler cannot
-// collapse the identical rounds into one.
+// (lulesh.cc:2249-2546, 2586-2836, 2874-3035), fused per element.
+//
+// The EOS is evaluated rep times per element, as in the reference, but the
+// repetitions are spread over the wave: a wave's 64 elements need
+// sum(rep) evaluations; lane l of round r takes evaluation r*64+l, finds
+// the element it belongs to in the wave's rep prefix sums, and the lane
+// that does an element's last repetition stores its result.  Regions of
+// different cost sit next to each other in element order, so a plain
+// per-lane loop would run every wave to its most expensive element.
+// Every repetition recomputes the same values from the same inputs, so the
+// element's result is that of the reference's sequential loop.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *tiles, int ntiles)
 {
+    __shared__ double s_in[7][256];
+    __shared__ double s_out[5][256];
+    __shared__ int s_start[256];
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const DomDev &D = doms[t.dom];
     GroupCtl *G = D.grp;
     if (group_stopped(*G)) return;
-    const int el = t.start + threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wbase = tid & ~63;
+    const int el = t.start + tid;
     const bool live = el < D.numElem;
-    unsigned long long cbits = dbits(1.0e+20), hbits = dbits(1.0e+20);
+    int rep = 0;
+    double vdov = 0.0;
     if (live) {
         const int nx = D.nx;
         const int k = el / (nx * nx);
@@ -551,7 +800,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         const double phixi = monoq_phi(gxi, xim, xip);
         const double phieta = monoq_phi(geta, etam, etap);
         const double phizeta = monoq_phi(gzeta, zetam, zetap);
-        const double vdov = D.vdov[el];
+        vdov = D.vdov[el];
         const double vnew_raw = D.vnew[el];
         double qlin, qquad;
         if (vdov > 0.) {
@@ -572,26 +821,57 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         }
 
         // --- qstop on the q of the previous cycle (lulesh.cc:2531-2546) ---
-        double q_old = D.q[el];
+        const double q_old = D.q[el];
         if (q_old > G->qstop) raise_status(G, -2);
 
-        // --- EOS ---
+        // --- EOS inputs (vnew clamped to [eosvmin, eosvmax], lulesh.cc:2642-2650) ---
         double vnewc = vnew_raw;
         if (vnewc < Phys::eosvmin) vnewc = Phys::eosvmin;
         if (vnewc > Phys::eosvmax) vnewc = Phys::eosvmax;
-        double e_old = D.e[el], delvc = D.delv[el], p_old = D.p[el];
-        const int rep = D.elemRep[el];
-        EosOut o;
-        o.p = o.e = o.q = o.bvc = o.pbvc = 0.0;
-        for (int r = 0; r < rep; ++r) {
-            asm volatile("" : "+v"(vnewc), "+v"(e_old), "+v"(delvc), "+v"(p_old));
-            asm volatile("" : "+v"(q_old), "+v"(qquad), "+v"(qlin));
-            o = eos_rep(vnewc, e_old, delvc, p_old, q_old, qquad, qlin);
+        s_in[0][tid] = vnewc;
+        s_in[1][tid] = D.e[el];
+        s_in[2][tid] = D.delv[el];
+        s_in[3][tid] = D.p[el];
+        s_in[4][tid] = q_old;
+        s_in[5][tid] = qquad;
+        s_in[6][tid] = qlin;
+        rep = D.elemRep[el];
+    }
+    const int incl = wave_incl_scan(rep, lane);
+    const int total = __shfl(incl, 63, 64);
+    s_start[tid] = incl - rep;
+    __syncthreads();
+
+    // --- EOS repetitions, spread over the wave ---
+    for (int w = lane; w < total; w += 64) {
+        int lo = 0, hi = 63;                       // last lane with start <= w
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_start[wbase + mid] <= w) lo = mid;
+            else hi = mid - 1;
         }
-        D.p[el] = o.p;
-        D.e[el] = o.e;
-        D.q[el] = o.q;
-        const double ss = ss_of(o.pbvc, o.e, vnewc, o.bvc, o.p);
+        const int o = wbase + lo;
+        const EosOut r = eos_rep(s_in[0][o], s_in[1][o], s_in[2][o], s_in[3][o], s_in[4][o],
+                                 s_in[5][o], s_in[6][o]);
+        const int last = (lo == 63 ? total : s_start[o + 1]) - 1;
+        if (w == last) {
+            s_out[0][o] = r.p;
+            s_out[1][o] = r.e;
+            s_out[2][o] = r.q;
+            s_out[3][o] = r.bvc;
+            s_out[4][o] = r.pbvc;
+        }
+    }
+    __syncthreads();
+
+    unsigned long long cbits = dbits(1.0e+20), hbits = dbits(1.0e+20);
+    if (live) {
+        const double p_new = s_out[0][tid], e_new = s_out[1][tid];
+        const double vnewc = s_in[0][tid];
+        D.p[el] = p_new;
+        D.e[el] = e_new;
+        D.q[el] = s_out[2][tid];
+        const double ss = ss_of(s_out[4][tid], e_new, vnewc, s_out[3][tid], p_new);
         D.ss[el] = ss;
         double vfin = vnewc;
         if (fabs(vfin - 1.0) < Phys::v_cut) vfin = 1.0;
@@ -605,8 +885,8 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
             if (vdov < 0.) dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;
             dtf = sqrt(dtf);
             dtf = arealg / dtf;
-            const unsigned long long b = dbits(dtf);
-            cbits = b < cbits ? b : cbits;
+            const unsigned long long bb = dbits(dtf);
+            cbits = bb < cbits ? bb : cbits;
             const double dtdvov = Phys::dvovmax / (fabs(vdov) + 1.e-20);
             const unsigned long long h = dbits(dtdvov);
             hbits = h < hbits ? h : hbits;
@@ -615,7 +895,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     // positive doubles order like their bit patterns: MIN via integer MIN
     cbits = wave_min_u64(cbits);
     hbits = wave_min_u64(hbits);
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
         DomCtl &C = *D.ctl;
         if (cbits < dbits(1.0e+20)) atomicMin(&C.dtcourant_bits, cbits);
         if (hbits < dbits(1.0e+20)) atomicMin(&C.dthydro_bits, hbits);
@@ -692,6 +972,13 @@ hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt,
     return hipGetLastError();
 }
 
+hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, hipStream_t s)
+{
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_force_march, dim3(nt), dim3(kMT * kMT), 0, s, d, t, nt, kchunk);
+    return hipGetLastError();
+}
+
 hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
@@ -744,6 +1031,21 @@ __global__ void k_probe_sqrt_div(const double *a, const double *b, double *out, 
         out[2 * t] = sqrt(a[t]);
         out[2 * t + 1] = a[t] / b[t];
     }
+}
+
+// streaming copy, the bandwidth yardstick of the profiling runs
+__global__ __launch_bounds__(256) void k_probe_copy(const double2 *__restrict__ a, double2 *__restrict__ b,
+                                                    size_t n2)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += stride) b[t] = a[t];
+}
+
+hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s)
+{
+    const size_t n2 = n / 2;
+    hipLaunchKernelGGL(k_probe_copy, dim3(4096), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n2);
+    return hipGetLastError();
 }
 
 hipError_t launch_probe_cbrt(const double *in, double *out, int n, hipStream_t s)

@@ -189,13 +189,14 @@ void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int bala
     h.deltatime0 = (.5 * cbrt(h.volo[0])) / sqrt(2.0 * einit);
 }
 
-size_t domain_device_bytes(int nx)
+size_t domain_device_bytes(int nx, bool corner_array)
 {
     const size_t ne = (size_t)nx * nx * nx, nn = (size_t)(nx + 1) * (nx + 1) * (nx + 1);
     const size_t all = ne + 6 * (size_t)nx * nx;
     // 10 node fields, 11 element fields, 3 ghosted gradients, 3 plain
-    // gradients, 48 corner components, elemRep; 256-byte alignment slack
-    return 8 * (10 * nn + 11 * ne + 3 * all + 3 * ne + 48 * ne) + 4 * ne + 64 * 256;
+    // gradients, [48 corner components], elemRep; 256-byte alignment slack
+    return 8 * (10 * nn + 11 * ne + 3 * all + 3 * ne + (corner_array ? 48 * ne : 0)) + 4 * ne +
+           64 * 256;
 }
 
 }  // namespace lulesh

@@ -19,6 +19,6 @@ struct HostDomain {
 };
 
 void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost);
-size_t domain_device_bytes(int nx);
+size_t domain_device_bytes(int nx, bool corner_array);
 
 }  // namespace lulesh

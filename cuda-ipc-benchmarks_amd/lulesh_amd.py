@@ -82,6 +82,7 @@ SIGNATURES = {
     "lulesh_server_now": (_d, [_vp]),
     "lulesh_server_times": (_i, [_vp, _i, _dp, _dp]),
     "lulesh_probe_cbrt": (_i, [_dp, _dp, _i]),
+    "lulesh_probe_copy": (_d, [_sz, _i]),
     "lulesh_probe_sqrt_div": (_i, [_dp, _dp, _dp, _i]),
 }
 
@@ -387,6 +388,15 @@ def probe_cbrt(x):
     out = np.empty_like(a)
     _chk(load().lulesh_probe_cbrt(_p(a), _p(out), a.size), "probe_cbrt")
     return out
+
+
+def probe_copy_ms(n_doubles, reps=20):
+    """Average ms of a device-to-device streaming copy of n doubles."""
+    init()
+    ms = load().lulesh_probe_copy(n_doubles, reps)
+    if ms < 0:
+        raise LuleshError(ENOMEM, "probe_copy")
+    return ms
 
 
 def probe_sqrt_div(a, b):

@@ -268,6 +268,9 @@ int lulesh_server_times(lulesh_server *s, int id, double *t_admit, double *t_don
 /* ------------------------------------------------------------------ */
 /* out[i] = correctly rounded cube root of in[i], computed on the GPU */
 int lulesh_probe_cbrt(const double *in, double *out, int n);
+/* Average milliseconds of a streaming device copy of n doubles (reps
+ * launches): the measured-bandwidth yardstick of the profiling runs. */
+double lulesh_probe_copy(size_t n, int reps);
 /* out2[2i] = sqrt(a[i]), out2[2i+1] = a[i] / b[i], computed on the GPU */
 int lulesh_probe_sqrt_div(const double *a, const double *b, double *out2, int n);
 

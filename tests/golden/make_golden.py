@@ -93,6 +93,9 @@ def run(args, threads):
         "printed_energy": [ln.split("=")[1].strip() for ln in out.splitlines()
                            if "Final Origin Energy" in ln][0],
         "sha256": {f: sha(v) for f, v in fields.items()},
+        # tolerance fingerprints for runs whose fields differ by roundoff
+        "sum": {f: float(np.sum(v)).hex() for f, v in fields.items()},
+        "l2": {f: float(np.sqrt(np.sum(v * v))).hex() for f, v in fields.items()},
     }
     return rec, fields
 

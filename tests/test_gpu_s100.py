@@ -1,0 +1,49 @@
+"""The headline workload, Sedov -s 100 -r 11 -b 1 -c 1 on one GPU, against
+the reference CPU code's results (tests/golden/ref_omp.json, written from the
+compiled unmodified omp_4.0 sources): origin energy within 1e-10 relative,
+TotalAbsDiff within 1e-8, and field fingerprints within roundoff."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+DB = json.load(open(os.path.join(GOLDEN, "ref_omp.json")))
+
+
+def _check_record(g, rec, rel_fields=1e-10):
+    rc, cyc, t, dt = g.sync()
+    assert rc == 0
+    assert cyc == rec["cycle"]
+    assert abs(t - float.fromhex(rec["time"])) <= 1e-12 * float.fromhex(rec["time"])
+    v = g.verify()
+    e_ref = float.fromhex(rec["origin_energy"])
+    assert abs(v["origin_energy"] - e_ref) / abs(e_ref) <= 1e-10
+    assert abs(v["total_abs_diff"] - float.fromhex(rec["total_abs_diff"])) <= 1e-8
+    for f in ["e", "x", "xd", "p", "v"]:
+        a = g.field(f)
+        s_ref = float.fromhex(rec["sum"][f])
+        l2_ref = float.fromhex(rec["l2"][f])
+        assert abs(np.sum(a) - s_ref) <= rel_fields * max(abs(s_ref), l2_ref), f
+        assert abs(np.sqrt(np.sum(a * a)) - l2_ref) <= rel_fields * l2_ref, f
+
+
+def test_s100_200_cycles_vs_reference(amd):
+    rec = DB["-s 100 -i 200"]
+    g = amd.Group(100)
+    g.set_max_cycles(200)
+    assert g.run() == 0
+    _check_record(g, rec)
+
+
+def test_s100_full_run_vs_reference(amd):
+    rec = DB.get("-s 100 full")
+    if rec is None:
+        pytest.skip("full-run -s 100 fixture not generated")
+    g = amd.Group(100)
+    assert g.run() == 0
+    _check_record(g, rec)
