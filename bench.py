@@ -164,6 +164,144 @@ def bench_plain(args):
     return line
 
 
+def problem_mix(n, seed, s_lo, s_hi, r_lo, r_hi, balances=(1,), costs=(1,)):
+    """Seeded problem mix (numpy PCG64)."""
+    rng = np.random.default_rng(seed)
+    nx = rng.integers(s_lo, s_hi + 1, n)
+    nr = rng.integers(r_lo, r_hi + 1, n)
+    b = rng.choice(np.array(balances), n)
+    c = rng.choice(np.array(costs), n)
+    return nx.astype(np.int32), nr.astype(np.int32), b.astype(np.int32), c.astype(np.int32)
+
+
+def run_sequential(mix, idx):
+    """Plain single-problem runs, one after the other (the comparison
+    baseline of the batched and served forms).  Returns (zone-cycles, s,
+    per-problem (cycles, origin energy))."""
+    nx, nr, b, c = mix
+    zc, t, res = 0, 0.0, {}
+    for i in idx:
+        g = lulesh_amd.Group(int(nx[i]), 1, int(nr[i]), int(b[i]), int(c[i]))
+        t0 = time.perf_counter()
+        rc = g.run(64)
+        t += time.perf_counter() - t0
+        if rc != 0:
+            raise SystemExit(f"sequential run {i} failed: {rc}")
+        cyc = g.sync()[1]
+        res[int(i)] = (cyc, g.verify()["origin_energy"])
+        zc += int(nx[i]) ** 3 * cyc
+        g.close()
+    return zc, t, res
+
+
+def bench_batched(args):
+    mix = problem_mix(64, args.seed, 16, 64, 1, 16, (1, 2, 4), (1, 2))
+    nx = mix[0]
+    # autotune on a throwaway copy of the ensemble
+    tb = lulesh_amd.Batch(*mix)
+    tune = tb.autotune(5)
+    tb.close()
+    b = lulesh_amd.Batch(*mix)
+    b.set_tiles(tune["tile_elems"], tune["tile_nodes"])
+    b.set_kchunk(tune["kchunk"])
+    b.enqueue(args.chunk)          # capture outside the timed region (real cycles)
+    b.synchronize()
+    c0, _ = b.results()
+    t0 = time.perf_counter()
+    rc = b.run(args.chunk)
+    t1 = time.perf_counter()
+    cyc, st = b.results()
+    if rc != 0 or np.any(st != 0):
+        raise SystemExit(f"batched run failed: {rc} {st}")
+    zc = int(np.sum(nx.astype(np.int64) ** 3 * (cyc - c0)))
+    value = zc / (t1 - t0)
+    energies = [b.group(i).verify()["origin_energy"] for i in range(len(nx))]
+    # sequential plain runs of the same mix: every problem, to stoptime
+    szc, st_s, seq = run_sequential(mix, range(len(nx)))
+    seq_rate = szc / st_s
+    worst = max(abs(energies[i] - seq[i][1]) / abs(seq[i][1]) for i in range(len(nx)))
+    same_cycles = all(int(cyc[i]) == seq[i][0] for i in range(len(nx)))
+    return {
+        "metric": "batched Sedov ensemble aggregate zone-cycles/s vs sequential plain runs",
+        "value": value, "unit": "zone-cycles/s", "n_gpus": 1, "steps": int(np.max(cyc)),
+        "warmup": args.chunk, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": f"synthetic: 64 Sedov problems, seed {args.seed}",
+        "config": {"workload": "64 problems, s~U{16..64}, r~U{1..16}, b in {1,2,4}, c in {1,2}, "
+                               "to stoptime", "zone_cycles": zc, "autotune": tune},
+        "sequential_plain": {"value": seq_rate, "zone_cycles": szc, "seconds": st_s},
+        "speedup_vs_sequential": value / seq_rate,
+        "max_origin_energy_rel_diff_vs_plain": worst, "cycles_match_plain": same_cycles,
+    }
+
+
+def bench_serve(args):
+    n = args.requests
+    mix = problem_mix(n, args.seed, 20, 80, 1, 11)
+    rng = np.random.default_rng(args.seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, n))
+    srv = lulesh_amd.Server(0, args.slots)
+    t_start = srv.now()
+    ids = [srv.submit(int(mix[0][i]), int(mix[1][i]), 1, 1, float(t_start + arrivals[i]))
+           for i in range(n)]
+    done = 0
+    last = time.time()
+    while done < n:
+        done += srv.poll(args.chunk)
+        if time.time() - last > 20:
+            print(f"# serve: {done}/{n} done, {srv.stats()}", file=sys.stderr, flush=True)
+            last = time.time()
+    t_end = srv.now()
+    res = [srv.result(i) for i in ids]
+    lat = np.array([r["t_done"] - (t_start + arrivals[i]) for i, r in enumerate(res)])
+    zc = int(sum(int(mix[0][i]) ** 3 * r["cycles"] for i, r in enumerate(res)))
+    bad = [i for i, r in enumerate(res) if r["status"] != 0]
+    srv.close()
+    # sequential plain runs of a sample of the same mix (rate per zone-cycle)
+    sample = list(range(0, n, max(1, n // args.seq_sample)))[: args.seq_sample]
+    szc, st_s, seq = run_sequential(mix, sample)
+    worst = max(abs(res[i]["origin_energy"] - seq[i][1]) / abs(seq[i][1]) for i in sample)
+    return {
+        "metric": "served Sedov stream: aggregate zone-cycles/s, p50/p99 completion latency",
+        "value": zc / (t_end - t_start), "unit": "zone-cycles/s", "n_gpus": 1, "steps": n,
+        "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": f"synthetic: {n} Sedov requests, Poisson rate {args.rate}/s, seed {args.seed}",
+        "config": {"workload": "s~U{20..80}, r~U{1..11}, to stoptime, continuous batching",
+                   "slots": args.slots, "zone_cycles": zc},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "mean": float(lat.mean())},
+        "wall_s": t_end - t_start, "failed": len(bad),
+        "sequential_plain_sample": {"requests": len(sample), "value": szc / st_s},
+        "speedup_vs_sequential": (zc / (t_end - t_start)) / (szc / st_s),
+        "max_origin_energy_rel_diff_vs_plain": worst,
+    }
+
+
+def bench_multidomain(args):
+    g = lulesh_amd.Group(45, 2, 11, 1, 1)
+    g.enqueue(args.warmup)
+    g.enqueue(args.chunk)
+    rc, c0, _, _ = g.sync()
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = min(args.chunk, args.steps - done)
+        g.enqueue(k)
+        done += k
+    rc, c1, _, _ = g.sync()
+    t1 = time.perf_counter()
+    ms = g.profile(5) / 5
+    E = 8 * 45 ** 3
+    return {
+        "metric": "8 subdomains (2x2x2 of -s 45) on one GPU, zone-cycles/s incl. on-device halo exchange",
+        "value": E * (c1 - c0) / (t1 - t0), "unit": "zone-cycles/s", "n_gpus": 1,
+        "steps": c1 - c0, "warmup": args.warmup, "ms_per_step": (t1 - t0) * 1e3 / (c1 - c0),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: Sedov blast, 8-rank decomposition on one GPU",
+        "config": {"workload": "2x2x2 subdomains, -s 45 each, device_local comm backend"},
+        "kernels_ms": {KERNEL_NAMES[k]: round(float(ms[k]), 4) for k in range(7) if ms[k] > 0},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,7 +309,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=100)
     ap.add_argument("--chunk", type=int, default=50, help="cycles per captured graph launch")
-    ap.add_argument("--mode", default="plain", choices=["plain", "copy"])
+    ap.add_argument("--mode", default="plain",
+                    choices=["plain", "copy", "batched", "serve", "multidomain"])
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=8.0, help="serve: Poisson arrivals per second")
+    ap.add_argument("--slots", type=int, default=64, help="serve: max resident problems")
+    ap.add_argument("--seq-sample", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -186,7 +330,14 @@ def main():
         print(json.dumps({"mode": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n,
                           "ms": ms, "GBs": 16 * n / (ms * 1e-3) / 1e9}), flush=True)
         return
-    line = bench_plain(args)
+    if args.mode == "batched":
+        line = bench_batched(args)
+    elif args.mode == "serve":
+        line = bench_serve(args)
+    elif args.mode == "multidomain":
+        line = bench_multidomain(args)
+    else:
+        line = bench_plain(args)
     print(json.dumps(line), flush=True)
 
 

@@ -441,6 +441,55 @@ int lulesh_batch_profile(lulesh_batch *b, int ncycles, double ms[7])
     return engine_profile(b->engine, ncycles, b->stream, ms);
 }
 
+// Tile-shape autotuner: times `probe_cycles` cycles of the ensemble (one
+// captured graph each) for every candidate (element tile, node tile, column
+// chunk) and installs the fastest.  The probe cycles are real cycles of the
+// problems; results do not depend on the tile shapes.
+int lulesh_batch_autotune(lulesh_batch *b, int probe_cycles, int *best_tile_e, int *best_tile_n,
+                          int *best_kchunk, double *best_ms)
+{
+    if (!b || probe_cycles < 1) return LULESH_EINVAL;
+    const int te[] = {64, 128, 256};
+    const int tn[] = {128, 256};
+    const int kc[] = {0, 4, 8};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e30;
+    int be = 256, bn = 256, bk = 0;
+    for (int a : te)
+        for (int n : tn)
+            for (int k : kc) {
+                int rc = engine_set_tiles(b->engine, a, n);
+                if (!rc) rc = engine_set_kchunk(b->engine, k);
+                if (!rc) rc = engine_enqueue(b->engine, probe_cycles, b->stream);   // capture + warm
+                if (rc) return rc;
+                HIPCHK(hipEventRecord(e0, b->stream));
+                rc = engine_enqueue(b->engine, probe_cycles, b->stream);
+                if (rc) return rc;
+                HIPCHK(hipEventRecord(e1, b->stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) { best = ms; be = a; bn = n; bk = k; }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    int rc = engine_set_tiles(b->engine, be, bn);
+    if (!rc) rc = engine_set_kchunk(b->engine, bk);
+    if (best_tile_e) *best_tile_e = be;
+    if (best_tile_n) *best_tile_n = bn;
+    if (best_kchunk) *best_kchunk = bk ? bk : b->engine->kchunk_used;
+    if (best_ms) *best_ms = best / probe_cycles;
+    return rc;
+}
+
+int lulesh_batch_set_kchunk(lulesh_batch *b, int kchunk)
+{
+    if (!b) return LULESH_EINVAL;
+    return engine_set_kchunk(b->engine, kchunk);
+}
+
 int lulesh_batch_synchronize(lulesh_batch *b)
 {
     if (!b) return LULESH_EINVAL;

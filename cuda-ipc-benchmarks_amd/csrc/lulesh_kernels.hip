@@ -892,13 +892,27 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
             hbits = h < hbits ? h : hbits;
         }
     }
-    // positive doubles order like their bit patterns: MIN via integer MIN
+    // positive doubles order like their bit patterns: MIN via integer MIN;
+    // reduce the workgroup first, then one atomic per workgroup and only
+    // when it can lower the stored minimum (all workgroups of a subdomain
+    // target the same two words)
+    __shared__ unsigned long long s_min[2][4];
     cbits = wave_min_u64(cbits);
     hbits = wave_min_u64(hbits);
     if (lane == 0) {
+        s_min[0][tid >> 6] = cbits;
+        s_min[1][tid >> 6] = hbits;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 1; w < nw; ++w) {
+            cbits = s_min[0][w] < cbits ? s_min[0][w] : cbits;
+            hbits = s_min[1][w] < hbits ? s_min[1][w] : hbits;
+        }
         DomCtl &C = *D.ctl;
-        if (cbits < dbits(1.0e+20)) atomicMin(&C.dtcourant_bits, cbits);
-        if (hbits < dbits(1.0e+20)) atomicMin(&C.dthydro_bits, hbits);
+        if (cbits < *(volatile unsigned long long *)&C.dtcourant_bits) atomicMin(&C.dtcourant_bits, cbits);
+        if (hbits < *(volatile unsigned long long *)&C.dthydro_bits) atomicMin(&C.dthydro_bits, hbits);
     }
 }
 

@@ -183,36 +183,42 @@ __device__ __forceinline__ void node_normals(const double x[8], const double y[8
     }
 }
 
-// VoluDer + CalcElemVolumeDerivative, lulesh.cc:717-789; row j holds the
-// node arguments the reference passes for corner j
-__device__ __forceinline__ void volume_derivs(const double x[8], const double y[8], const double z[8],
-                                              double dvdx[8], double dvdy[8], double dvdz[8])
+// VoluDer + CalcElemVolumeDerivative, lulesh.cc:717-789: the derivative
+// of the element volume with respect to corner j's position; row j of A
+// holds the node arguments the reference passes for corner j
+__device__ __forceinline__ void volume_deriv_corner(const double x[8], const double y[8],
+                                                    const double z[8], int j, double &dvdx,
+                                                    double &dvdy, double &dvdz)
 {
     constexpr int A[8][6] = {{1, 2, 3, 4, 5, 7}, {2, 3, 0, 5, 6, 4}, {3, 0, 1, 6, 7, 5},
                              {0, 1, 2, 7, 4, 6}, {7, 6, 5, 0, 3, 1}, {4, 7, 6, 1, 0, 2},
                              {5, 4, 7, 2, 1, 3}, {6, 5, 4, 3, 2, 0}};
     const double twelfth = 1.0 / 12.0;
+    const double x0 = x[A[j][0]], x1 = x[A[j][1]], x2 = x[A[j][2]];
+    const double x3 = x[A[j][3]], x4 = x[A[j][4]], x5 = x[A[j][5]];
+    const double y0 = y[A[j][0]], y1 = y[A[j][1]], y2 = y[A[j][2]];
+    const double y3 = y[A[j][3]], y4 = y[A[j][4]], y5 = y[A[j][5]];
+    const double z0 = z[A[j][0]], z1 = z[A[j][1]], z2 = z[A[j][2]];
+    const double z3 = z[A[j][3]], z4 = z[A[j][4]], z5 = z[A[j][5]];
+    const double dx = (y1 + y2) * (z0 + z1) - (y0 + y1) * (z1 + z2) +
+                      (y0 + y4) * (z3 + z4) - (y3 + y4) * (z0 + z4) -
+                      (y2 + y5) * (z3 + z5) + (y3 + y5) * (z2 + z5);
+    const double dy = -(x1 + x2) * (z0 + z1) + (x0 + x1) * (z1 + z2) -
+                      (x0 + x4) * (z3 + z4) + (x3 + x4) * (z0 + z4) +
+                      (x2 + x5) * (z3 + z5) - (x3 + x5) * (z2 + z5);
+    const double dz = -(y1 + y2) * (x0 + x1) + (y0 + y1) * (x1 + x2) -
+                      (y0 + y4) * (x3 + x4) + (y3 + y4) * (x0 + x4) +
+                      (y2 + y5) * (x3 + x5) - (y3 + y5) * (x2 + x5);
+    dvdx = dx * twelfth;
+    dvdy = dy * twelfth;
+    dvdz = dz * twelfth;
+}
+
+__device__ __forceinline__ void volume_derivs(const double x[8], const double y[8], const double z[8],
+                                              double dvdx[8], double dvdy[8], double dvdz[8])
+{
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const double x0 = x[A[j][0]], x1 = x[A[j][1]], x2 = x[A[j][2]];
-        const double x3 = x[A[j][3]], x4 = x[A[j][4]], x5 = x[A[j][5]];
-        const double y0 = y[A[j][0]], y1 = y[A[j][1]], y2 = y[A[j][2]];
-        const double y3 = y[A[j][3]], y4 = y[A[j][4]], y5 = y[A[j][5]];
-        const double z0 = z[A[j][0]], z1 = z[A[j][1]], z2 = z[A[j][2]];
-        const double z3 = z[A[j][3]], z4 = z[A[j][4]], z5 = z[A[j][5]];
-        const double dx = (y1 + y2) * (z0 + z1) - (y0 + y1) * (z1 + z2) +
-                          (y0 + y4) * (z3 + z4) - (y3 + y4) * (z0 + z4) -
-                          (y2 + y5) * (z3 + z5) + (y3 + y5) * (z2 + z5);
-        const double dy = -(x1 + x2) * (z0 + z1) + (x0 + x1) * (z1 + z2) -
-                          (x0 + x4) * (z3 + z4) + (x3 + x4) * (z0 + z4) +
-                          (x2 + x5) * (z3 + z5) - (x3 + x5) * (z2 + z5);
-        const double dz = -(y1 + y2) * (x0 + x1) + (y0 + y1) * (x1 + x2) -
-                          (y0 + y4) * (x3 + x4) + (y3 + y4) * (x0 + x4) +
-                          (y2 + y5) * (x3 + x5) - (y3 + y5) * (x2 + x5);
-        dvdx[j] = dx * twelfth;
-        dvdy[j] = dy * twelfth;
-        dvdz[j] = dz * twelfth;
-    }
+    for (int j = 0; j < 8; ++j) volume_deriv_corner(x, y, z, j, dvdx[j], dvdy[j], dvdz[j]);
 }
 
 // AreaFace, lulesh.cc:1725-1744

@@ -73,6 +73,8 @@ SIGNATURES = {
     "lulesh_batch_set_tiles": (_i, [_vp, _i, _i]),
     "lulesh_batch_enqueue": (_i, [_vp, _i]),
     "lulesh_batch_synchronize": (_i, [_vp]),
+    "lulesh_batch_set_kchunk": (_i, [_vp, _i]),
+    "lulesh_batch_autotune": (_i, [_vp, _i, _ip, _ip, _ip, _dp]),
     "lulesh_server_create": (_i, [_sz, _i, _pvp]),
     "lulesh_server_destroy": (_i, [_vp]),
     "lulesh_server_submit": (_i, [_vp, _i, _i, _i, _i, _d]),
@@ -310,6 +312,16 @@ class Batch:
 
     def set_tiles(self, te, tn):
         _chk(load().lulesh_batch_set_tiles(self.h, te, tn), "tiles")
+
+    def set_kchunk(self, k):
+        _chk(load().lulesh_batch_set_kchunk(self.h, k), "kchunk")
+
+    def autotune(self, probe_cycles=5):
+        a, n, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = ctypes.c_double()
+        _chk(load().lulesh_batch_autotune(self.h, probe_cycles, ctypes.byref(a), ctypes.byref(n),
+                                          ctypes.byref(k), ctypes.byref(ms)), "autotune")
+        return dict(tile_elems=a.value, tile_nodes=n.value, kchunk=k.value, ms_per_cycle=ms.value)
 
     def run(self, cycles_per_sync=0):
         return load().lulesh_batch_run(self.h, cycles_per_sync, None)

@@ -228,6 +228,13 @@ int lulesh_batch_results(lulesh_batch *b, int *cycles, int *status);
  * tile_nodes for node kernels (64, 128, 192 or 256); 0 keeps the current
  * value. */
 int lulesh_batch_set_tiles(lulesh_batch *b, int tile_elems, int tile_nodes);
+/* Node planes per column tile of the fused force kernel (0 = automatic). */
+int lulesh_batch_set_kchunk(lulesh_batch *b, int kchunk);
+/* Autotuner: times `probe_cycles` real cycles of the ensemble for each
+ * candidate element tile {64,128,256}, node tile {128,256} and column chunk
+ * {auto,4,8}, installs the fastest and reports it (ms per cycle). */
+int lulesh_batch_autotune(lulesh_batch *b, int probe_cycles, int *tile_elems, int *tile_nodes,
+                          int *kchunk, double *ms_per_cycle);
 /* Enqueue `ncycles` cycles of the whole ensemble without waiting / wait. */
 int lulesh_batch_enqueue(lulesh_batch *b, int ncycles);
 int lulesh_batch_synchronize(lulesh_batch *b);
