@@ -37,7 +37,7 @@ import lulesh_amd  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 # algorithmic (compulsory) bytes per launch of each kernel class, per
-# element E and node N of the launch; see DESIGN.md
+# element E and node N of the launch (E elements, N nodes of the mesh)
 KERNEL_NAMES = ["time_increment", "force_gather", "node_gather_update", "node_update",
                 "kinematics", "qeos", "halo_exchange"]
 
@@ -46,19 +46,44 @@ TWOPASS = os.environ.get("LULESH_FORCE_PATH") == "twopass"
 
 
 def kernel_bytes(k, E, N):
-    if k == 1 and not TWOPASS:   # fused force+gather: x..zd, 6 elem doubles in, f out
-        return 48 * N + 48 * E + 24 * N
+    if k == 1 and not TWOPASS:   # fused force+gather: x..zd, p q v volo hgcoef in, f out
+        return 48 * N + 40 * E + 24 * N
     if k == 1:   # x..zd of 8 nodes (once per node), 6 elem doubles in, 48 corner doubles out
         return 48 * N + 48 * E + 384 * E
     if k == 2:   # 48 corner doubles in, nodalMass + x..zd in, x..zd out
         return 384 * E + 8 * N + 48 * N + 48 * N
     if k == 4:   # x..zd in, volo v in, 10 doubles out
         return 48 * N + 16 * E + 80 * E
-    if k == 5:   # 15 doubles + elemRep in, 5 doubles out
-        return 120 * E + 4 * E + 40 * E
+    if k == 5:   # 15 doubles + elemRep in; p e q ss v hgcoef out
+        return 120 * E + 4 * E + 48 * E
     if k == 3:
         return 24 * N + 8 * N + 48 * N + 48 * N
     return 0
+
+
+KERNEL_SYMBOLS = {1: "k_force_march", 3: "k_node_update", 4: "k_kinematics", 5: "k_qeos"}
+
+
+def measured_traffic(k, nx):
+    """HBM bytes per launch of kernel class k at -s nx, as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (copy-
+    calibrated); None when no such measurement is committed."""
+    sym = KERNEL_SYMBOLS.get(k)
+    if sym is None or TWOPASS:
+        return None
+    pdir = os.path.join(REPO, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, fn)) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 4 and w[0] == "traffic_bytes_per_launch" and w[1] == sym \
+                        and w[2] == f"s{nx}":
+                    return int(w[3])
+    return None
 
 
 def cpu_baseline(nx, target_s=15.0):
@@ -155,7 +180,7 @@ def bench_plain(args):
                    "elements": E, "nodes": N, "graph_cycles_per_launch": chunk},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": b},
+                     "traffic": measured_traffic(dom, nx), "algorithmic_bytes": b},
         "kernels": kern,
     }
     g.close()
@@ -319,7 +344,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(subdomains of a decomposition share one GPU's HBM instead)", file=sys.stderr)
         raise SystemExit(2)
     lulesh_amd.init(0)

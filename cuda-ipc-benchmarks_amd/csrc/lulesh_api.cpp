@@ -58,7 +58,7 @@ int domain_alloc(lulesh_domain *d)
         {&D.e, ne}, {&D.p, ne}, {&D.q, ne}, {&D.v, ne}, {&D.volo, ne}, {&D.delv, ne},
         {&D.vdov, ne}, {&D.arealg, ne}, {&D.ss, ne}, {&D.elemMass, ne}, {&D.vnew, ne},
         {&D.delv_xi, all}, {&D.delv_eta, all}, {&D.delv_zeta, all},
-        {&D.delx_xi, ne}, {&D.delx_eta, ne}, {&D.delx_zeta, ne},
+        {&D.delx_xi, ne}, {&D.delx_eta, ne}, {&D.delx_zeta, ne}, {&D.hgcoef, ne},
         {&D.corner, corner_path_enabled() ? (size_t)kCornerComps * ne : 0}};
     size_t bytes = 0;
     for (auto &s : slots) bytes += align256(s.n * sizeof(double));
@@ -81,17 +81,24 @@ int domain_upload(lulesh_domain *d)
 {
     const HostDomain &h = d->host;
     DomDev &D = d->dev;
-    const size_t nn = h.numNode, ne = h.numElem;
-    HIPCHK(hipMemcpy(D.x, h.x.data(), nn * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(D.y, h.y.data(), nn * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(D.z, h.z.data(), nn * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(D.nodalMass, h.nodalMass.data(), nn * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(D.volo, h.volo.data(), ne * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(D.elemMass, h.volo.data(), ne * 8, hipMemcpyHostToDevice));
-    std::vector<double> ones(ne, 1.0);
-    HIPCHK(hipMemcpy(D.v, ones.data(), ne * 8, hipMemcpyHostToDevice));
+    D.nx = h.nx; D.numElem = h.numElem; D.numNode = h.numNode; D.allElem = h.allElem;
+    D.col = h.col; D.row = h.row; D.plane = h.plane; D.tp = h.tp;
+    for (int f = 0; f < 6; ++f) D.ghost[f] = h.ghost[f];
+    // the arena was cleared on the null stream; the set-up kernels follow
+    // it there, and the blocking copies below wait for them
+    HIPCHK(launch_setup(D, nullptr));
     HIPCHK(hipMemcpy(D.e, &h.e0, 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy((void *)D.elemRep, h.elemRep.data(), ne * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void *)D.elemRep, h.elemRep.data(), (size_t)h.numElem * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(nullptr));
+    return LULESH_OK;
+}
+
+// recompute the hourglass coefficients after set-up or a host write of one
+// of their inputs
+int domain_refresh_hgcoef(lulesh_domain *d, hipStream_t s)
+{
+    HIPCHK(launch_hg_coef(d->dev.hgcoef, d->dev.ss, d->dev.elemMass, d->dev.volo, d->dev.v,
+                          d->host.numElem, s));
     return LULESH_OK;
 }
 
@@ -314,26 +321,18 @@ int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **o
     for (int r = 0; r < g->ndom; ++r) {
         lulesh_domain *d = g->doms[r];
         DomDev &D = d->dev;
-        const HostDomain &h = d->host;
-        D.nx = h.nx; D.numElem = h.numElem; D.numNode = h.numNode; D.allElem = h.allElem;
-        D.col = h.col; D.row = h.row; D.plane = h.plane; D.tp = h.tp;
-        for (int f = 0; f < 6; ++f) D.ghost[f] = h.ghost[f];
         D.ctl = g->dctl + r;
         D.grp = g->gctl;
+    }
+    for (auto *d : g->doms) {
+        rc = domain_refresh_hgcoef(d, g->stream);
+        if (rc) return rc;
     }
     // initial nodalMass exchange (lulesh.cc:3147-3161)
     const int fm = LULESH_NODAL_MASS;
     rc = group_exchange(g.get(), LULESH_MSG_COMM_SBN, 1, &fm, g->stream);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(g->stream));
-    // set-up arrays no longer needed on the host
-    for (auto *d : g->doms) {
-        d->host.x = std::vector<double>();
-        d->host.y = std::vector<double>();
-        d->host.z = std::vector<double>();
-        d->host.volo = std::vector<double>();
-        d->host.nodalMass = std::vector<double>();
-    }
     *out = g.release();
     return LULESH_OK;
 }
@@ -597,6 +596,11 @@ int lulesh_domain_set_field(lulesh_domain *d, int f, const double *host, size_t 
     if (!p || n < len) return LULESH_EINVAL;
     HIPCHK(hipStreamSynchronize(d->g->stream));
     HIPCHK(hipMemcpy(p, host, len * 8, hipMemcpyHostToDevice));
+    if (f == LULESH_SS || f == LULESH_V || f == LULESH_VOLO || f == LULESH_ELEM_MASS) {
+        const int rc = domain_refresh_hgcoef(d, d->g->stream);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(d->g->stream));
+    }
     return (int)len;
 }
 

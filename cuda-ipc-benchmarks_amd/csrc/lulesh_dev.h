@@ -25,6 +25,10 @@ struct DomDev {
     double *e, *p, *q, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass, *vnew;
     double *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
     double *corner;
+    // FB hourglass coefficient -hgcoef*0.01*ss*elemMass/cbrt(volo*v) of
+    // the coming force computation (CalcFBHourglassForceForElems,
+    // lulesh.cc:1012-1052), produced where ss and v are produced
+    double *hgcoef;
     const int *elemRep;
     int nx, numElem, numNode, allElem;
     int col, row, plane, tp;
@@ -60,11 +64,14 @@ struct Tile {
     int start;
 };
 
-// column tile of the fused force kernel: 15x15 nodes from (i0, j0), node
-// planes [k0, k0 + kchunk)
+// column tile of the fused force kernel: (ex-1) x (ey-1) nodes from
+// (i0, j0), node planes [k0, k0 + kchunk); its ex x ey threads are the
+// elements of the tile plus one halo row and column on the low side
 struct ColTile {
     int dom;
     int i0, j0, k0;
+    int ex, ey;
+    int pad[2];
 };
 
 // physics constants of Domain::Domain (LULESH/omp_4.0/lulesh-init.cc:20-38)

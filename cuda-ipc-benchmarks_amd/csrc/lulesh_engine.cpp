@@ -53,21 +53,59 @@ static void engine_drop_graph(Engine *e)
     e->graph_cycles = 0;
 }
 
-// node planes per column tile: enough workgroups to fill 256 CUs twice over
-// a few times, but never fewer than 4 planes (the bottom element plane of a
-// chunk is recomputed by its owner and the chunk below)
+// Column-tile shape for a mesh of nx elements per edge: ex x ey element
+// slots (<= 256) covering (ex-1) x (ey-1) nodes each; pick the shape with
+// the fewest slots over the whole node plane (boundary tiles are partly
+// empty, so 16 x 16 is not always best for small meshes).
+static void tile_shape(int nx, int &ex, int &ey)
+{
+    const int s1 = nx + 1;
+    long best_w = -1, best_s = -1;
+    ex = ey = 16;
+    for (int a = 4; a <= 64; ++a)
+        for (int b = 4; b <= 64 && a * b <= 256; ++b) {
+            const long tiles = (long)((s1 + a - 2) / (a - 1)) * ((s1 + b - 2) / (b - 1));
+            const long waves = tiles * ((a * b + 63) / 64);   // busy waves per plane
+            const long slots = tiles * a * b;
+            if (best_w < 0 || waves < best_w || (waves == best_w && slots < best_s)) {
+                best_w = waves;
+                best_s = slots;
+                ex = a;
+                ey = b;
+            }
+        }
+}
+
+// node planes per column tile.  The force kernel runs two 256-thread
+// workgroups per CU; a chunk of kc node planes computes kc + 1 element
+// planes, so the kernel takes about ceil(tiles / (2 * CUs)) * (kc + 1)
+// plane steps.  Pick the kc in [3, 32] that minimises that (the smaller kc
+// on a tie, for the shorter tail).
 static int auto_kchunk(const Engine *e)
 {
-    long cols = 0, planes = 0;
-    for (auto *g : e->groups)
-        for (auto *d : g->doms) {
-            const int c = (d->host.nx + 1 + 14) / 15;
-            cols += (long)c * c;
-            planes = std::max<long>(planes, d->host.nx + 1);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const long slots = 2L * std::max(cus, 1);
+    long best_cost = -1;
+    int best = 4;
+    for (int kc = 3; kc <= 32; ++kc) {
+        long tiles = 0;
+        for (auto *g : e->groups)
+            for (auto *d : g->doms) {
+                int ex, ey;
+                tile_shape(d->host.nx, ex, ey);
+                const int s1 = d->host.nx + 1;
+                tiles += (long)((s1 + ex - 2) / (ex - 1)) * ((s1 + ey - 2) / (ey - 1)) *
+                         ((s1 + kc - 1) / kc);
+            }
+        const long cost = ((tiles + slots - 1) / slots) * (kc + 1);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = kc;
         }
-    const long target = 2048;
-    long k = (cols * planes + target - 1) / target;
-    return (int)std::max<long>(4, std::min<long>(k, 32));
+    }
+    return best;
 }
 
 static int build_tiles(Engine *e)
@@ -78,9 +116,12 @@ static int build_tiles(Engine *e)
     for (int di = 0, gi = 0; gi < (int)e->groups.size(); ++gi)
         for (auto *d : e->groups[gi]->doms) {
             const int s1 = d->host.nx + 1;
+            int ex, ey;
+            tile_shape(d->host.nx, ex, ey);
             for (int k0 = 0; k0 < s1; k0 += kc)
-                for (int j0 = 0; j0 < s1; j0 += 15)
-                    for (int i0 = 0; i0 < s1; i0 += 15) ct.push_back(ColTile{di, i0, j0, k0});
+                for (int j0 = 0; j0 < s1; j0 += ey - 1)
+                    for (int i0 = 0; i0 < s1; i0 += ex - 1)
+                        ct.push_back(ColTile{di, i0, j0, k0, ex, ey, {0, 0}});
             ++di;
         }
     e->kchunk_used = kc;

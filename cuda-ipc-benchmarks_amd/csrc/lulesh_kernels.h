@@ -19,6 +19,11 @@ hipError_t launch_pack(const PackEnt *e, int n, const double *const *fptr, int n
                        const GroupCtl *gate, hipStream_t s);
 hipError_t launch_unpack(const UnpackTgt *t, int n, const UnpackSrc *src, double *const *fptr,
                          int nfields, int mode, const GroupCtl *gate, hipStream_t s);
+// Sedov set-up of one subdomain on the device: coordinates, volo,
+// elemMass, v = 1, nodal masses
+hipError_t launch_setup(const DomDev &D, hipStream_t s);
+hipError_t launch_hg_coef(double *coef, const double *ss, const double *mass, const double *volo,
+                          const double *v, int n, hipStream_t s);
 hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s);
 hipError_t launch_probe_cbrt(const double *in, double *out, int n, hipStream_t s);
 hipError_t launch_probe_sqrt_div(const double *a, const double *b, double *out, int n, hipStream_t s);

@@ -179,8 +179,7 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
         for (int c = 0; c < 8; ++c)
             hg[c][m] = hg_gamma(m, c) - volinv * (dvdx[c] * hx + dvdy[c] * hy + dvdz[c] * hz);
     }
-    const double volume13 = cbrt_cr(determ);
-    const double coefficient = -Phys::hgcoef * 0.01 * D.ss[el] * D.elemMass[el] / volume13;
+    const double coefficient = D.hgcoef[el];
     const double *vel[3] = {D.xd, D.yd, D.zd};
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
@@ -204,9 +203,10 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
 // ---------------------------------------------------------------------------
 // Fused force + ordered node gather, z-marching (the production path).
 //
-// A workgroup owns a column of 15 x 15 nodes and a chunk of node planes
-// [k0, k1).  Its 16 x 16 threads are the elements of the column's halo-
-// extended element tile; the block walks element planes c = k0-1 .. k1-1.
+// A workgroup owns a column of (ex-1) x (ey-1) nodes and a chunk of node
+// planes [k0, k1).  Its ex x ey threads (<= 256; 16 x 16 for large meshes,
+// shapes fitted to the mesh for small ones) are the elements of the
+// column's halo-extended element tile; the block walks element planes c = k0-1 .. k1-1.
 // At each plane every thread computes its element's 8 stress corner forces
 // and 8 hourglass corner forces (the same arithmetic as k_elem_force) into
 // LDS, and each node thread adds its four neighbouring elements' corners:
@@ -219,8 +219,7 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
 // Elements outside the mesh contribute +0.0, which leaves a sum that
 // started from +0.0 unchanged.
 // ---------------------------------------------------------------------------
-constexpr int kMT = 16;        // element tile edge = threads per row
-constexpr int kMN = kMT - 1;   // node tile edge
+constexpr int kMThreads = 256;   // threads (element slots) per column tile
 
 __device__ __forceinline__ void load_elem_xyz(const DomDev &D, const int n[8], double x[8], double y[8],
                                               double z[8])
@@ -254,7 +253,7 @@ struct HgModes {
 };
 
 __device__ __forceinline__ void hg_modes(const double x[8], const double y[8], const double z[8],
-                                         double determ, double ss, double mass, HgModes &H)
+                                         double determ, double coefficient, HgModes &H)
 {
     double dvdx[8], dvdy[8], dvdz[8];
     volume_derivs(x, y, z, dvdx, dvdy, dvdz);
@@ -272,8 +271,14 @@ __device__ __forceinline__ void hg_modes(const double x[8], const double y[8], c
         for (int c = 0; c < 8; ++c)
             H.g[c][m] = hg_gamma(m, c) - volinv * (dvdx[c] * hx + dvdy[c] * hy + dvdz[c] * hz);
     }
-    const double volume13 = cbrt_cr(determ);
-    H.coefficient = -Phys::hgcoef * 0.01 * ss * mass / volume13;
+    H.coefficient = coefficient;
+}
+
+// coefficient = - hourg * 0.01 * ss * mass / cbrt(volo * v), lulesh.cc:1012-1052
+__device__ __forceinline__ double hg_coefficient(double ss, double mass, double volo, double v)
+{
+    const double volume13 = cbrt_cr(volo * v);
+    return -Phys::hgcoef * 0.01 * ss * mass / volume13;
 }
 
 __device__ __forceinline__ void hg_force(const HgModes &H, const double vl[8], double out[8])
@@ -294,23 +299,25 @@ __device__ __forceinline__ void hg_force(const HgModes &H, const double vl[8], d
 __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, const ColTile *tiles,
                                                         int ntiles, int kchunk)
 {
-    __shared__ double lds[24][kMT * kMT];
+    __shared__ double lds[24][kMThreads];
     const ColTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const DomDev &D = doms[t.dom];
     GroupCtl *G = D.grp;
     if (group_stopped(*G)) return;
     const int nx = D.nx, s1 = nx + 1, s2 = s1 * s1;
     const int tid = threadIdx.x;
-    const int ta = tid & (kMT - 1), tb = tid / kMT;
+    const int ex = t.ex;
+    const int ta = tid % ex, tb = tid / ex;
+    const bool slot_ok = tb < t.ey;
     const int a = t.i0 - 1 + ta, b = t.j0 - 1 + tb;
-    const bool col_ok = a >= 0 && a < nx && b >= 0 && b < nx;
+    const bool col_ok = slot_ok && a >= 0 && a < nx && b >= 0 && b < nx;
     const int ni = t.i0 + ta, nj = t.j0 + tb;
-    const bool node_ok = ta < kMN && tb < kMN && ni <= nx && nj <= nx;
+    const bool node_ok = ta < ex - 1 && tb < t.ey - 1 && ni <= nx && nj <= nx;
     const int k0 = t.k0;
     const int k1 = min(t.k0 + kchunk, nx + 1);
     // lower / upper corner slots seen from the node thread:
     // own element, +x, +y, +x+y neighbours
-    const int t1 = tid + 1, t2 = tid + kMT, t3 = tid + kMT + 1;
+    const int t1 = tid + 1, t2 = tid + ex, t3 = tid + ex + 1;
     double ps[3] = {0.0, 0.0, 0.0}, ph[3] = {0.0, 0.0, 0.0};   // node plane c+1 partials
     double fs[3] = {0.0, 0.0, 0.0};                              // node plane c stress sums
 
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                 const double vv = D.v[el];
                 if (vv <= 0.0) raise_status(G, -1);
                 HgModes H;
-                hg_modes(x, y, z, D.volo[el] * vv, D.ss[el], D.elemMass[el], H);
+                hg_modes(x, y, z, D.volo[el] * vv, D.hgcoef[el], H);
                 double vl[8];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) vl[q] = D.xd[n[q]];
@@ -876,6 +883,8 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         double vfin = vnewc;
         if (fabs(vfin - 1.0) < Phys::v_cut) vfin = 1.0;
         D.v[el] = vfin;
+        // the next cycle's hourglass coefficient needs exactly these ss, v
+        D.hgcoef[el] = hg_coefficient(ss, D.elemMass[el], D.volo[el], vfin);
 
         // --- time constraints ---
         if (vdov != 0.) {
@@ -989,7 +998,7 @@ hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt,
 hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_force_march, dim3(nt), dim3(kMT * kMT), 0, s, d, t, nt, kchunk);
+    hipLaunchKernelGGL(k_force_march, dim3(nt), dim3(kMThreads), 0, s, d, t, nt, kchunk);
     return hipGetLastError();
 }
 
@@ -1028,6 +1037,81 @@ hipError_t launch_unpack(const UnpackTgt *t, int n, const UnpackSrc *src, double
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_unpack, dim3(grid_of(n, 256)), dim3(256), 0, s, t, n, src, fptr, nfields, mode,
                        gate);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Sedov set-up on the device (Domain::Domain, lulesh-init.cc:16-184)
+// ---------------------------------------------------------------------------
+
+// BuildMesh coordinates, lulesh-init.cc:192-212: 1.125 * (loc*nx + i) / (tp*nx)
+__global__ __launch_bounds__(256) void k_setup_coords(DomDev D)
+{
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= D.numNode) return;
+    const int nx = D.nx, s1 = nx + 1;
+    const int k = nd / (s1 * s1), rem = nd - k * s1 * s1, j = rem / s1, i = rem - j * s1;
+    const double mesh = (double)(D.tp * nx);
+    D.x[nd] = 1.125 * (double)(D.col * nx + i) / mesh;
+    D.y[nd] = 1.125 * (double)(D.row * nx + j) / mesh;
+    D.z[nd] = 1.125 * (double)(D.plane * nx + k) / mesh;
+}
+
+// element volumes (lulesh-init.cc:151-162): volo = elemMass = volume, v = 1
+__global__ __launch_bounds__(256) void k_setup_volumes(DomDev D)
+{
+    const int el = blockIdx.x * blockDim.x + threadIdx.x;
+    if (el >= D.numElem) return;
+    int n[8], i, j, k;
+    elem_nodes(D.nx, el, n, i, j, k);
+    double x[8], y[8], z[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) { x[c] = D.x[n[c]]; y[c] = D.y[n[c]]; z[c] = D.z[n[c]]; }
+    const double vol = elem_volume(x, y, z);
+    D.volo[el] = vol;
+    D.elemMass[el] = vol;
+    D.v[el] = 1.0;
+}
+
+// nodal masses (lulesh-init.cc:163-167): the reference adds volume/8 of
+// each element, elements in ascending order, onto a node mass that starts
+// at 0.0; per node that is the ordered sum over its elements
+__global__ __launch_bounds__(256) void k_setup_nodal_mass(DomDev D)
+{
+    const int nd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= D.numNode) return;
+    const int nx = D.nx, s1 = nx + 1;
+    const int k = nd / (s1 * s1), rem = nd - k * s1 * s1, j = rem / s1, i = rem - j * s1;
+    double m = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int a = i - 1 + (q & 1), b = j - 1 + ((q >> 1) & 1), c = k - 1 + (q >> 2);
+        if (a < 0 || a >= nx || b < 0 || b >= nx || c < 0 || c >= nx) continue;
+        m += D.volo[(c * nx + b) * nx + a] / 8.0;
+    }
+    D.nodalMass[nd] = m;
+}
+
+hipError_t launch_setup(const DomDev &D, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_setup_coords, dim3(grid_of(D.numNode, 256)), dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_setup_volumes, dim3(grid_of(D.numElem, 256)), dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_setup_nodal_mass, dim3(grid_of(D.numNode, 256)), dim3(256), 0, s, D);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_hg_coef(double *coef, const double *ss, const double *mass,
+                                                 const double *volo, const double *v, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) coef[t] = hg_coefficient(ss[t], mass[t], volo[t], v[t]);
+}
+
+hipError_t launch_hg_coef(double *coef, const double *ss, const double *mass, const double *volo,
+                          const double *v, int n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hg_coef, dim3(grid_of(n, 256)), dim3(256), 0, s, coef, ss, mass, volo, v, n);
     return hipGetLastError();
 }
 

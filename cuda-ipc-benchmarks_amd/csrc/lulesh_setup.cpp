@@ -1,9 +1,10 @@
 // lulesh_setup.cpp -- host-side Sedov set-up of one subdomain.
 //
-// Reproduces Domain::Domain and its helpers (LULESH/omp_4.0/lulesh-init.cc)
-// so the initial state the GPU starts from is the reference's, bit for bit:
-// the same coordinate formula, the same element volumes in the same order,
-// the same nodal-mass accumulation, glibc rand() for the region layout.
+// Reproduces the host half of Domain::Domain and its helpers
+// (LULESH/omp_4.0/lulesh-init.cc): decomposition, comm-buffer sizes, ghost
+// offsets, glibc rand() for the region layout, e0 and dt0.  Coordinates,
+// volumes and nodal masses are produced on the device (k_setup_*), with
+// the same formulas and the same summation order.
 #include "lulesh_setup.h"
 
 #include <math.h>
@@ -139,63 +140,37 @@ void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int bala
         else h.ghost[f] = -1;
     }
 
-    // BuildMesh, lulesh-init.cc:189-212
-    h.x.resize(h.numNode);
-    h.y.resize(h.numNode);
-    h.z.resize(h.numNode);
-    const int meshEdgeElems = tp * nx;
-    int nidx = 0;
-    double tz = 1.125 * (double)(h.plane * nx) / (double)meshEdgeElems;
-    for (int plane = 0; plane < en; ++plane) {
-        double ty = 1.125 * (double)(h.row * nx) / (double)meshEdgeElems;
-        for (int row = 0; row < en; ++row) {
-            double tx = 1.125 * (double)(h.col * nx) / (double)meshEdgeElems;
-            for (int col = 0; col < en; ++col) {
-                h.x[nidx] = tx;
-                h.y[nidx] = ty;
-                h.z[nidx] = tz;
-                ++nidx;
-                tx = 1.125 * (double)(h.col * nx + col + 1) / (double)meshEdgeElems;
-            }
-            ty = 1.125 * (double)(h.row * nx + row + 1) / (double)meshEdgeElems;
-        }
-        tz = 1.125 * (double)(h.plane * nx + plane + 1) / (double)meshEdgeElems;
-    }
-
     create_regions(h, nr, balance, cost);
 
-    // initial volumes and masses, lulesh-init.cc:151-168 (element order)
-    h.volo.resize(h.numElem);
-    h.nodalMass.assign(h.numNode, 0.0);
-    const int s2 = en * en;
-    for (int k = 0, el = 0; k < nx; ++k)
-        for (int j = 0; j < nx; ++j)
-            for (int i = 0; i < nx; ++i, ++el) {
-                const int n0 = k * s2 + j * en + i;
-                const int nl[8] = {n0, n0 + 1, n0 + en + 1, n0 + en,
-                                   n0 + s2, n0 + s2 + 1, n0 + s2 + en + 1, n0 + s2 + en};
-                double xl[8], yl[8], zl[8];
-                for (int c = 0; c < 8; ++c) { xl[c] = h.x[nl[c]]; yl[c] = h.y[nl[c]]; zl[c] = h.z[nl[c]]; }
-                const double volume = host_elem_volume(xl, yl, zl);
-                h.volo[el] = volume;
-                for (int c = 0; c < 8; ++c) h.nodalMass[nl[c]] += volume / 8.0;
-            }
+    // volume of element 0 for dt0: its corner coordinates by the BuildMesh
+    // formula (lulesh-init.cc:192-212)
+    {
+        const double mesh = (double)(tp * nx);
+        double xl[8], yl[8], zl[8];
+        for (int c = 0; c < 8; ++c) {
+            const int di = ((c + 1) >> 1) & 1, dj = (c >> 1) & 1, dk = c >> 2;
+            xl[c] = 1.125 * (double)(h.col * nx + di) / mesh;
+            yl[c] = 1.125 * (double)(h.row * nx + dj) / mesh;
+            zl[c] = 1.125 * (double)(h.plane * nx + dk) / mesh;
+        }
+        h.volo0 = host_elem_volume(xl, yl, zl);
+    }
 
     // initial energy and dt0, lulesh-init.cc:170-180
     const double ebase = 3.948746e+7;
     const double scale = (nx * tp) / 45.0;
     const double einit = ebase * scale * scale * scale;
     h.e0 = (h.row + h.col + h.plane == 0) ? einit : 0.0;
-    h.deltatime0 = (.5 * cbrt(h.volo[0])) / sqrt(2.0 * einit);
+    h.deltatime0 = (.5 * cbrt(h.volo0)) / sqrt(2.0 * einit);
 }
 
 size_t domain_device_bytes(int nx, bool corner_array)
 {
     const size_t ne = (size_t)nx * nx * nx, nn = (size_t)(nx + 1) * (nx + 1) * (nx + 1);
     const size_t all = ne + 6 * (size_t)nx * nx;
-    // 10 node fields, 11 element fields, 3 ghosted gradients, 3 plain
+    // 10 node fields, 12 element fields, 3 ghosted gradients, 3 plain
     // gradients, [48 corner components], elemRep; 256-byte alignment slack
-    return 8 * (10 * nn + 11 * ne + 3 * all + 3 * ne + (corner_array ? 48 * ne : 0)) + 4 * ne +
+    return 8 * (10 * nn + 12 * ne + 3 * all + 3 * ne + (corner_array ? 48 * ne : 0)) + 4 * ne +
            64 * 256;
 }
 

@@ -13,11 +13,12 @@ struct HostDomain {
     bool planeMin = false, planeMax = false;
     int maxPlaneSize = 0, maxEdgeSize = 0, comBufSize = 0;
     int ghost[6] = {-1, -1, -1, -1, -1, -1};
-    std::vector<double> x, y, z, volo, nodalMass;
     std::vector<int> elemRep, region;
-    double e0 = 0.0, deltatime0 = 0.0;
+    double e0 = 0.0, deltatime0 = 0.0, volo0 = 0.0;
 };
 
+// decomposition, comm-buffer sizes, region layout (glibc rand()), e0 and
+// dt0; the mesh, volumes and nodal masses are built on the device
 void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost);
 size_t domain_device_bytes(int nx, bool corner_array);
 

@@ -73,7 +73,7 @@ enum lulesh_field {
 /* library                                                             */
 /* ------------------------------------------------------------------ */
 
-/* Selects the device (0 on a one-GPU box) and checks it is gfx950. */
+/* Selects the device (0 on a one-GPU machine) and checks it is gfx950. */
 int lulesh_init(int device);
 const char *lulesh_status_string(int status);
 const char *lulesh_version(void);
