@@ -63,6 +63,7 @@ int domain_alloc(lulesh_domain *d)
     size_t bytes = 0;
     for (auto &s : slots) bytes += align256(s.n * sizeof(double));
     bytes += align256(ne * sizeof(int));
+    bytes += align256((size_t)h.evtotal * sizeof(unsigned short));
     void *arena = nullptr;
     if (hipMalloc(&arena, bytes) != hipSuccess) return LULESH_ENOMEM;
     d->arena = arena;
@@ -73,6 +74,8 @@ int domain_alloc(lulesh_domain *d)
         p += align256(s.n * sizeof(double));
     }
     D.elemRep = (const int *)p;
+    p += align256(ne * sizeof(int));
+    D.evtab = (const unsigned short *)p;
     HIPCHK(hipMemset(arena, 0, bytes));
     return LULESH_OK;
 }
@@ -89,7 +92,9 @@ int domain_upload(lulesh_domain *d)
     HIPCHK(launch_setup(D, nullptr));
     HIPCHK(hipMemcpy(D.e, &h.e0, 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy((void *)D.elemRep, h.elemRep.data(), (size_t)h.numElem * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy((void *)D.evtab, h.evtab.data(), (size_t)h.evtotal * 2, hipMemcpyHostToDevice));
     HIPCHK(hipStreamSynchronize(nullptr));
+    d->host.evtab = std::vector<unsigned short>();   // the device copy is the one used
     return LULESH_OK;
 }
 
@@ -287,8 +292,7 @@ int group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, hipS
 // ---------------------------------------------------------------------------
 int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **out)
 {
-    if (nx < 1 || tp < 1 || tp > 4 || nr < 1 || balance < 0 || cost < 0 || !out) return LULESH_EINVAL;
-    if ((long long)nx * nx * nx * 8 > 2147483647LL) return LULESH_EINVAL;
+    if (!out || !problem_ok(nx, tp, nr, balance, cost)) return LULESH_EINVAL;
     std::unique_ptr<lulesh_group> g(new lulesh_group());
     g->nx = nx;
     g->tp = tp;

@@ -30,6 +30,9 @@ struct DomDev {
     // lulesh.cc:1012-1052), produced where ss and v are produced
     double *hgcoef;
     const int *elemRep;
+    // one entry per EOS evaluation, elements in ascending order, each
+    // repeated elemRep times (entry format: kEvLast / kEvMask)
+    const unsigned short *evtab;
     int nx, numElem, numNode, allElem;
     int col, row, plane, tp;
     // ghost element offsets of the velocity-gradient arrays for the six
@@ -59,10 +62,20 @@ struct GroupCtl {
     int ndom;
 };
 
+// element or node tile: [start, start + blockDim) of subdomain dom; for
+// element tiles eoff/ecount locate the tile's EOS evaluations in the
+// subdomain's evaluation table
 struct Tile {
     int dom;
     int start;
+    long long eoff;
+    int ecount;
+    int pad;
 };
+
+// EOS evaluation table entry: element index mod 2^15 in the low 15 bits,
+// bit 15 set on the element's last repetition
+constexpr unsigned kEvLast = 0x8000u, kEvMask = 0x7fffu;
 
 // column tile of the fused force kernel: (ex-1) x (ey-1) nodes from
 // (i0, j0), node planes [k0, k0 + kchunk); its ex x ey threads are the

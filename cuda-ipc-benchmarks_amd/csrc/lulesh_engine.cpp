@@ -3,12 +3,15 @@
 //
 // An Engine steps any set of groups together.  One cycle is
 //   k_time_increment  (all groups)
-//   k_elem_force      (all subdomains)
-//   k_node_gather     (fused with the node update when no group has halos)
-//   [SBN exchange, k_node_update, PosVel exchange]   multi-subdomain groups
+//   k_force_march     (stress + hourglass forces and the ordered node gather,
+//                      all subdomains; LULESH_FORCE_PATH=twopass selects
+//                      k_elem_force + k_node_gather instead)
+//   [SBN exchange]    multi-subdomain groups
+//   k_node_update
+//   [PosVel exchange]
 //   k_kinematics
 //   [MonoQ exchange]
-//   k_qeos
+//   k_qeos            (MonoQ region, qstop, EOS x rep, dt constraints)
 // and `n` cycles are captured once into a HIP graph and replayed, so the
 // host launches one graph per n cycles and never waits inside a cycle: dt,
 // time, the stop test and error codes all live in device control blocks.
@@ -79,8 +82,10 @@ static void tile_shape(int nx, int &ex, int &ey)
 // node planes per column tile.  The force kernel runs two 256-thread
 // workgroups per CU; a chunk of kc node planes computes kc + 1 element
 // planes, so the kernel takes about ceil(tiles / (2 * CUs)) * (kc + 1)
-// plane steps.  Pick the kc in [3, 32] that minimises that (the smaller kc
-// on a tie, for the shorter tail).
+// plane steps.  Pick the kc in [1, 32] that minimises that (the smaller kc
+// on a tie, for the shorter tail): small meshes get short chunks and more
+// workgroups, large ones the least recomputation that still fills the
+// chip in whole rounds.
 static int auto_kchunk(const Engine *e)
 {
     int dev = 0, cus = 256;
@@ -89,7 +94,7 @@ static int auto_kchunk(const Engine *e)
     const long slots = 2L * std::max(cus, 1);
     long best_cost = -1;
     int best = 4;
-    for (int kc = 3; kc <= 32; ++kc) {
+    for (int kc = 1; kc <= 32; ++kc) {
         long tiles = 0;
         for (auto *g : e->groups)
             for (auto *d : g->doms) {
@@ -132,19 +137,42 @@ static int build_tiles(Engine *e)
     if (!ct.empty()) HIPCHK(hipMemcpy(e->d_ctiles, ct.data(), ct.size() * sizeof(ColTile), hipMemcpyHostToDevice));
     for (int di = 0, gi = 0; gi < (int)e->groups.size(); ++gi)
         for (auto *d : e->groups[gi]->doms) {
-            for (int s = 0; s < d->host.numElem; s += e->tile_e) et.push_back(Tile{di, s});
-            for (int s = 0; s < d->host.numNode; s += e->tile_n) nt.push_back(Tile{di, s});
+            const HostDomain &h = d->host;
+            for (int s = 0; s < h.numElem; s += e->tile_e) {
+                const int end = std::min(s + e->tile_e, h.numElem);
+                const long long a = h.evprefix64[s / 64], b = h.evprefix64[(end + 63) / 64];
+                et.push_back(Tile{di, s, a, (int)(b - a), 0});
+            }
+            for (int s = 0; s < h.numNode; s += e->tile_n) nt.push_back(Tile{di, s, 0, 0, 0});
             ++di;
         }
+    // EOS launch order: the kernels map workgroup b to tile chunk b % 8 (one
+    // chunk of consecutive tiles per XCD), slot b / 8; inside each chunk the
+    // tiles with the most EOS evaluations go first, so the long workgroups
+    // do not start last and run alone at the end of the launch
+    std::vector<Tile> qt = et;
+    {
+        const size_t n = qt.size(), q = n / 8, r = n % 8;
+        size_t a = 0;
+        for (size_t x = 0; x < 8; ++x) {
+            const size_t len = x < r ? q + 1 : q;
+            std::stable_sort(qt.begin() + a, qt.begin() + a + len,
+                             [](const Tile &u, const Tile &v) { return u.ecount > v.ecount; });
+            a += len;
+        }
+    }
     if (e->d_etiles) (void)hipFree(e->d_etiles);
     if (e->d_ntiles) (void)hipFree(e->d_ntiles);
-    e->d_etiles = e->d_ntiles = nullptr;
+    if (e->d_qtiles) (void)hipFree(e->d_qtiles);
+    e->d_etiles = e->d_ntiles = e->d_qtiles = nullptr;
     e->n_etiles = (int)et.size();
     e->n_ntiles = (int)nt.size();
     HIPCHK(hipMalloc((void **)&e->d_etiles, std::max<size_t>(1, et.size()) * sizeof(Tile)));
     HIPCHK(hipMalloc((void **)&e->d_ntiles, std::max<size_t>(1, nt.size()) * sizeof(Tile)));
+    HIPCHK(hipMalloc((void **)&e->d_qtiles, std::max<size_t>(1, qt.size()) * sizeof(Tile)));
     if (!et.empty()) HIPCHK(hipMemcpy(e->d_etiles, et.data(), et.size() * sizeof(Tile), hipMemcpyHostToDevice));
     if (!nt.empty()) HIPCHK(hipMemcpy(e->d_ntiles, nt.data(), nt.size() * sizeof(Tile), hipMemcpyHostToDevice));
+    if (!qt.empty()) HIPCHK(hipMemcpy(e->d_qtiles, qt.data(), qt.size() * sizeof(Tile), hipMemcpyHostToDevice));
     return LULESH_OK;
 }
 
@@ -197,6 +225,7 @@ void engine_destroy(Engine *e)
     if (e->d_doms) (void)hipFree(e->d_doms);
     if (e->d_groups) (void)hipFree(e->d_groups);
     if (e->d_etiles) (void)hipFree(e->d_etiles);
+    if (e->d_qtiles) (void)hipFree(e->d_qtiles);
     if (e->d_ntiles) (void)hipFree(e->d_ntiles);
     if (e->d_ctiles) (void)hipFree(e->d_ctiles);
     delete e;
@@ -290,7 +319,7 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
         STEP(6, rc = exchange_all(e, LULESH_MSG_MONOQ, 3, kMonoQ, s));
         if (rc) return rc;
     }
-    STEP(5, HIPCHK(launch_qeos(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
+    STEP(5, HIPCHK(launch_qeos(e->d_doms, e->d_qtiles, e->n_etiles, e->tile_e, s)));
     return LULESH_OK;
 }
 
@@ -561,7 +590,11 @@ struct Result {
 
 // device footprint of one admitted single-domain problem: its field arena
 // plus control block and tile-table slack
-size_t request_bytes(int nx) { return domain_device_bytes(nx, corner_path_enabled()) + 4096; }
+size_t request_bytes(int nx, int nr, int cost)
+{
+    const long long ne = (long long)nx * nx * nx;
+    return domain_device_bytes(nx, corner_path_enabled(), ne * max_elem_rep(nr, cost)) + 4096;
+}
 
 }  // namespace
 
@@ -618,12 +651,12 @@ double lulesh_server_now(lulesh_server *s) { return s ? s->now() : 0.0; }
 
 int lulesh_server_submit(lulesh_server *s, int nx, int nr, int balance, int cost, double arrival)
 {
-    if (!s || nx < 1 || nr < 1) return LULESH_EINVAL;
+    if (!s || !problem_ok(nx, 1, nr, balance, cost)) return LULESH_EINVAL;
     Request r;
     r.id = (int)s->results.size();
     r.nx = nx; r.nr = nr; r.balance = balance; r.cost = cost;
     r.arrival = arrival;
-    r.bytes = request_bytes(nx);
+    r.bytes = request_bytes(nx, nr, cost);
     if (r.bytes > s->budget) return LULESH_ENOMEM;
     s->queue.push_back(r);
     s->results.push_back(Result());

@@ -63,7 +63,7 @@ struct Engine {
     DomDev *d_doms = nullptr;
     int ndoms = 0;
     GroupCtl **d_groups = nullptr;
-    Tile *d_etiles = nullptr, *d_ntiles = nullptr;
+    Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_qtiles = nullptr;
     int n_etiles = 0, n_ntiles = 0;
     int tile_e = 256, tile_n = 256;
     // fused force + gather (z-marching column tiles) or the two-pass

@@ -321,17 +321,26 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     double ps[3] = {0.0, 0.0, 0.0}, ph[3] = {0.0, 0.0, 0.0};   // node plane c+1 partials
     double fs[3] = {0.0, 0.0, 0.0};                              // node plane c stress sums
 
+    double x[8], y[8], z[8];
+    bool have_lo = false;   // x..z[4..7] hold the previous element's top corners
     for (int c = k0 - 1; c < k1; ++c) {
         const bool el_ok = col_ok && c >= 0 && c < nx;
         const int el = (c * nx + b) * nx + a;
         int n[8];
-        double x[8], y[8], z[8];
         if (el_ok) {
             const int n0 = c * s2 + b * s1 + a;
             n[0] = n0; n[1] = n0 + 1; n[2] = n0 + s1 + 1; n[3] = n0 + s1;
             n[4] = n0 + s2; n[5] = n0 + s2 + 1; n[6] = n0 + s2 + s1 + 1; n[7] = n0 + s2 + s1;
-            load_elem_xyz(D, n, x, y, z);
+            if (have_lo) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { x[q] = x[q + 4]; y[q] = y[q + 4]; z[q] = z[q + 4]; }
+#pragma unroll
+                for (int q = 4; q < 8; ++q) { x[q] = D.x[n[q]]; y[q] = D.y[n[q]]; z[q] = D.z[n[q]]; }
+            } else {
+                load_elem_xyz(D, n, x, y, z);
+            }
         }
+        have_lo = el_ok;
         // ---- stress corners ----
         {
             double fx[8], fy[8], fz[8];
@@ -748,44 +757,33 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
     return v;
 }
 
-__device__ __forceinline__ int wave_incl_scan(int v, int lane)
-{
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_up(v, off, 64);
-        if (lane >= off) v += o;
-    }
-    return v;
-}
-
 // ---------------------------------------------------------------------------
 // CalcMonotonicQRegionForElems + the qstop scan + EvalEOSForElems x rep +
 // CalcCourantConstraintForElems + CalcHydroConstraintForElems
 // (lulesh.cc:2249-2546, 2586-2836, 2874-3035), fused per element.
 //
 // The EOS is evaluated rep times per element, as in the reference, but the
-// repetitions are spread over the wave: a wave's 64 elements need
-// sum(rep) evaluations; lane l of round r takes evaluation r*64+l, finds
-// the element it belongs to in the wave's rep prefix sums, and the lane
-// that does an element's last repetition stores its result.  Regions of
-// different cost sit next to each other in element order, so a plain
-// per-lane loop would run every wave to its most expensive element.
-// Every repetition recomputes the same values from the same inputs, so the
-// element's result is that of the reference's sequential loop.
+// repetitions are spread over the workgroup: the tile's entries of the
+// subdomain's evaluation table (element i listed elemRep[i] times, in
+// element order) are dealt round-robin to the lanes, and the lane holding
+// an element's last entry stores its result.  Regions of different cost
+// sit next to each other in element order, so a per-lane loop would run
+// every wave to its most expensive element, and a per-wave balance leaves
+// the workgroup waiting on its most expensive wave.  Every repetition
+// recomputes the same values from the same inputs, so the element's result
+// is that of the reference's sequential loop.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     __shared__ double s_in[7][256];
     __shared__ double s_out[5][256];
-    __shared__ int s_start[256];
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const DomDev &D = doms[t.dom];
     GroupCtl *G = D.grp;
     if (group_stopped(*G)) return;
-    const int tid = threadIdx.x, lane = tid & 63, wbase = tid & ~63;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int el = t.start + tid;
     const bool live = el < D.numElem;
-    int rep = 0;
     double vdov = 0.0;
     if (live) {
         const int nx = D.nx;
@@ -842,31 +840,28 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         s_in[4][tid] = q_old;
         s_in[5][tid] = qquad;
         s_in[6][tid] = qlin;
-        rep = D.elemRep[el];
     }
-    const int incl = wave_incl_scan(rep, lane);
-    const int total = __shfl(incl, 63, 64);
-    s_start[tid] = incl - rep;
     __syncthreads();
 
-    // --- EOS repetitions, spread over the wave ---
-    for (int w = lane; w < total; w += 64) {
-        int lo = 0, hi = 63;                       // last lane with start <= w
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_start[wbase + mid] <= w) lo = mid;
-            else hi = mid - 1;
-        }
-        const int o = wbase + lo;
-        const EosOut r = eos_rep(s_in[0][o], s_in[1][o], s_in[2][o], s_in[3][o], s_in[4][o],
-                                 s_in[5][o], s_in[6][o]);
-        const int last = (lo == 63 ? total : s_start[o + 1]) - 1;
-        if (w == last) {
-            s_out[0][o] = r.p;
-            s_out[1][o] = r.e;
-            s_out[2][o] = r.q;
-            s_out[3][o] = r.bvc;
-            s_out[4][o] = r.pbvc;
+    // --- EOS repetitions, dealt over the workgroup ---
+    {
+        const unsigned short *tab = D.evtab + t.eoff;
+        const int total = t.ecount, bs = blockDim.x;
+        const unsigned base = (unsigned)t.start & kEvMask;
+        unsigned ent = tid < total ? tab[tid] : 0u;
+        for (int w = tid; w < total; w += bs) {
+            const unsigned next = w + bs < total ? tab[w + bs] : 0u;   // in flight meanwhile
+            const int o = (int)(((ent & kEvMask) - base) & kEvMask);
+            const EosOut r = eos_rep(s_in[0][o], s_in[1][o], s_in[2][o], s_in[3][o], s_in[4][o],
+                                     s_in[5][o], s_in[6][o]);
+            if (ent & kEvLast) {
+                s_out[0][o] = r.p;
+                s_out[1][o] = r.e;
+                s_out[2][o] = r.q;
+                s_out[3][o] = r.bvc;
+                s_out[4][o] = r.pbvc;
+            }
+            ent = next;
         }
     }
     __syncthreads();

@@ -12,6 +12,8 @@
 
 namespace lulesh {
 
+static const unsigned kEvLastHost = 0x8000u, kEvMaskHost = 0x7fffu;
+
 // CalcElemVolume (LULESH/omp_4.0/lulesh.cc:1627-1709), host copy for set-up
 static double host_elem_volume(const double x[8], const double y[8], const double z[8])
 {
@@ -95,6 +97,43 @@ static void create_regions(HostDomain &h, int nr, int balance, int cost)
     }
 }
 
+long long max_elem_rep(int nr, int cost)
+{
+    return (nr + 15) / 20 >= 1 ? 10 * (1 + (long long)cost) : 1 + (long long)cost;
+}
+
+bool problem_ok(int nx, int tp, int nr, int balance, int cost)
+{
+    if (nx < 1 || tp < 1 || tp > 4 || nr < 1 || balance < 0 || cost < 0) return false;
+    const long long ne = (long long)nx * nx * nx;
+    if (ne * 8 > 2147483647LL) return false;
+    return ne * max_elem_rep(nr, cost) <= 2147483647LL;
+}
+
+// The EOS work list: element i appears elemRep[i] times, in element order,
+// its last appearance flagged.  A workgroup of the EOS kernel deals the
+// entries of its tile round-robin over its lanes, so every lane does the
+// same number of evaluations whatever the region layout.
+static void build_eval_table(HostDomain &h)
+{
+    const int ne = h.numElem;
+    long long total = 0;
+    h.evprefix64.assign((size_t)(ne + 63) / 64 + 1, 0);
+    for (int i = 0; i < ne; ++i) {
+        if (i % 64 == 0) h.evprefix64[i / 64] = total;
+        total += h.elemRep[i];
+    }
+    h.evprefix64.back() = total;
+    h.evtotal = total;
+    h.evtab.resize((size_t)total);
+    size_t w = 0;
+    for (int i = 0; i < ne; ++i) {
+        const unsigned short id = (unsigned short)(i & kEvMaskHost);
+        for (int r = 1; r < h.elemRep[i]; ++r) h.evtab[w++] = id;
+        h.evtab[w++] = (unsigned short)(id | kEvLastHost);
+    }
+}
+
 void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost)
 {
     h.nx = nx;
@@ -141,6 +180,7 @@ void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int bala
     }
 
     create_regions(h, nr, balance, cost);
+    build_eval_table(h);
 
     // volume of element 0 for dt0: its corner coordinates by the BuildMesh
     // formula (lulesh-init.cc:192-212)
@@ -164,14 +204,14 @@ void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int bala
     h.deltatime0 = (.5 * cbrt(h.volo0)) / sqrt(2.0 * einit);
 }
 
-size_t domain_device_bytes(int nx, bool corner_array)
+size_t domain_device_bytes(int nx, bool corner_array, long long evtotal)
 {
     const size_t ne = (size_t)nx * nx * nx, nn = (size_t)(nx + 1) * (nx + 1) * (nx + 1);
     const size_t all = ne + 6 * (size_t)nx * nx;
     // 10 node fields, 12 element fields, 3 ghosted gradients, 3 plain
     // gradients, [48 corner components], elemRep; 256-byte alignment slack
     return 8 * (10 * nn + 12 * ne + 3 * all + 3 * ne + (corner_array ? 48 * ne : 0)) + 4 * ne +
-           64 * 256;
+           2 * (size_t)evtotal + 64 * 256;
 }
 
 }  // namespace lulesh

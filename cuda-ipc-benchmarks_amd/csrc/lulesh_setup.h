@@ -14,12 +14,22 @@ struct HostDomain {
     int maxPlaneSize = 0, maxEdgeSize = 0, comBufSize = 0;
     int ghost[6] = {-1, -1, -1, -1, -1, -1};
     std::vector<int> elemRep, region;
+    // EOS evaluation table (uploaded, then released) and the number of
+    // evaluations before element 64*i, i = 0 .. ceil(numElem/64)
+    std::vector<unsigned short> evtab;
+    std::vector<long long> evprefix64;
+    long long evtotal = 0;
     double e0 = 0.0, deltatime0 = 0.0, volo0 = 0.0;
 };
 
 // decomposition, comm-buffer sizes, region layout (glibc rand()), e0 and
 // dt0; the mesh, volumes and nodal masses are built on the device
 void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost);
-size_t domain_device_bytes(int nx, bool corner_array);
+size_t domain_device_bytes(int nx, bool corner_array, long long evtotal);
+// largest elemRep the region layout of (nr, cost) can assign
+long long max_elem_rep(int nr, int cost);
+// problem parameters the library accepts: 32-bit element and node
+// indexing, at most 4x4x4 subdomains, an EOS table under 2^31 entries
+bool problem_ok(int nx, int tp, int nr, int balance, int cost);
 
 }  // namespace lulesh
