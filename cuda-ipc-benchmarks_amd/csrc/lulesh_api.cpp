@@ -52,8 +52,14 @@ int domain_alloc(lulesh_domain *d)
     const size_t nn = h.numNode, ne = h.numElem, all = h.allElem;
     struct Slot { double **p; size_t n; };
     DomDev &D = d->dev;
+    // a one-subdomain group double-buffers its node state (no halo
+    // exchange touches it between the node update and its readers)
+    const bool two = d->g->ndom == 1;
+    NodeState &a = D.ns[0], &b = D.ns[1];
+    const size_t nn2 = two ? nn : 0;
     std::vector<Slot> slots = {
-        {&D.x, nn}, {&D.y, nn}, {&D.z, nn}, {&D.xd, nn}, {&D.yd, nn}, {&D.zd, nn},
+        {&a.x, nn}, {&a.y, nn}, {&a.z, nn}, {&a.xd, nn}, {&a.yd, nn}, {&a.zd, nn},
+        {&b.x, nn2}, {&b.y, nn2}, {&b.z, nn2}, {&b.xd, nn2}, {&b.yd, nn2}, {&b.zd, nn2},
         {&D.fx, nn}, {&D.fy, nn}, {&D.fz, nn}, {&D.nodalMass, nn},
         {&D.e, ne}, {&D.p, ne}, {&D.q, ne}, {&D.v, ne}, {&D.volo, ne}, {&D.delv, ne},
         {&D.vdov, ne}, {&D.arealg, ne}, {&D.ss, ne}, {&D.elemMass, ne}, {&D.vnew, ne},
@@ -76,6 +82,9 @@ int domain_alloc(lulesh_domain *d)
     D.elemRep = (const int *)p;
     p += align256(ne * sizeof(int));
     D.evtab = (const unsigned short *)p;
+    if (!two) b = a;
+    D.cur = a;   // cycle 0: the set-up state is in buffer 0
+    D.old = b;
     HIPCHK(hipMemset(arena, 0, bytes));
     return LULESH_OK;
 }
@@ -107,17 +116,35 @@ int domain_refresh_hgcoef(lulesh_domain *d, hipStream_t s)
     return LULESH_OK;
 }
 
+// the node-state buffer holding the current positions and velocities:
+// buffer (cycles done) & 1 when the subdomain double-buffers them; -1 when
+// the control block cannot be read
+static int node_buf(lulesh_domain *d)
+{
+    const DomDev &D = d->dev;
+    if (D.ns[0].x == D.ns[1].x || !D.ctl) return 0;
+    DomCtl c;
+    if (hipStreamSynchronize(d->g->stream) != hipSuccess) return -1;
+    if (hipMemcpy(&c, D.ctl, sizeof(DomCtl), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return c.cycle & 1;
+}
+
 double *field_dev_ptr(lulesh_domain *d, int f, size_t *len)
 {
     DomDev &D = d->dev;
     const size_t nn = d->host.numNode, ne = d->host.numElem, all = d->host.allElem;
+    if (f >= LULESH_X && f <= LULESH_ZD) {
+        const int b = node_buf(d);
+        if (b < 0) {
+            *len = 0;
+            return nullptr;
+        }
+        const NodeState &S = D.ns[b];
+        double *const p[6] = {S.x, S.y, S.z, S.xd, S.yd, S.zd};
+        *len = nn;
+        return p[f - LULESH_X];
+    }
     switch (f) {
-    case LULESH_X: *len = nn; return D.x;
-    case LULESH_Y: *len = nn; return D.y;
-    case LULESH_Z: *len = nn; return D.z;
-    case LULESH_XD: *len = nn; return D.xd;
-    case LULESH_YD: *len = nn; return D.yd;
-    case LULESH_ZD: *len = nn; return D.zd;
     case LULESH_FX: *len = nn; return D.fx;
     case LULESH_FY: *len = nn; return D.fy;
     case LULESH_FZ: *len = nn; return D.fz;
@@ -368,12 +395,18 @@ int group_reset_ctl(lulesh_group *g)
 
 int group_set_param(lulesh_group *g)
 {
-    // stoptime / qstop / max_cycles live in the device GroupCtl
+    // stoptime / qstop / max_cycles live in the device GroupCtl; let the
+    // group's queued cycles finish before rewriting it
+    HIPCHK(hipStreamSynchronize(g->stream));
     GroupCtl G;
     HIPCHK(hipMemcpy(&G, g->gctl, sizeof(G), hipMemcpyDeviceToHost));
     G.stoptime = g->stoptime;
     G.qstop = g->qstop;
     G.max_cycles = g->max_cycles;
+    // the stop test is re-evaluated at the next TimeIncrement, as the
+    // reference's loop condition is on every iteration: a raised cap or
+    // stop time resumes a group that had reached the old one
+    G.done = 0;
     HIPCHK(hipMemcpy(g->gctl, &G, sizeof(G), hipMemcpyHostToDevice));
     return LULESH_OK;
 }
@@ -492,6 +525,15 @@ int lulesh_group_set_max_cycles(lulesh_group *g, int m)
     return group_set_param(g);
 }
 
+int lulesh_group_set_keep_forces(lulesh_group *g, int keep)
+{
+    if (!g) return LULESH_EINVAL;
+    g->keep_forces = keep != 0;
+    // the captured cycle depends on it
+    if (g->engine) { engine_destroy(g->engine); g->engine = nullptr; }
+    return LULESH_OK;
+}
+
 int lulesh_group_enqueue(lulesh_group *g, int ncycles, lulesh_stream_t stream)
 {
     if (!g || ncycles < 0) return LULESH_EINVAL;
@@ -584,6 +626,8 @@ int lulesh_domain_sizes(const lulesh_domain *d, int *nx, int *ne, int *nn, int *
 int lulesh_domain_get_field(lulesh_domain *d, int f, double *host, size_t n)
 {
     if (!d || !host) return LULESH_EINVAL;
+    // a one-subdomain group stores its forces only when asked to
+    if (f >= LULESH_FX && f <= LULESH_FZ && d->g->ndom == 1 && !d->g->keep_forces) return LULESH_EINVAL;
     size_t len;
     double *p = field_dev_ptr(d, f, &len);
     if (!p || n < len) return LULESH_EINVAL;

@@ -1,7 +1,10 @@
 // lulesh_dev.h -- device-visible data layout of the MI355X LULESH step.
 //
 // Layout in HBM (per subdomain, structure of arrays, FP64):
-//   node fields   x y z xd yd zd fx fy fz nodalMass        [(nx+1)^3]
+//   node state    x y z xd yd zd, two buffers for a one-subdomain group
+//                 (cycle n reads buffer (n-1)&1 and writes buffer n&1, so a
+//                 kernel can advance nodes that other workgroups still read)
+//   node fields   fx fy fz nodalMass                       [(nx+1)^3]
 //   element       e p q v volo delv vdov arealg ss elemMass vnew
 //                 delx_xi delx_eta delx_zeta                [nx^3]
 //   gradients     delv_xi delv_eta delv_zeta   [nx^3 + 6 nx^2 ghost planes]
@@ -19,9 +22,21 @@ namespace lulesh {
 
 constexpr int kCornerComps = 48;     // 6 components x 8 corners
 
+// node positions and velocities
+struct NodeState {
+    double *x, *y, *z, *xd, *yd, *zd;
+};
+
 // per-subdomain descriptor, read by every kernel through the tile table
 struct DomDev {
-    double *x, *y, *z, *xd, *yd, *zd, *fx, *fy, *fz, *nodalMass;
+    // ns[b]: buffer b of the node state; ns[1] aliases ns[0] when the
+    // subdomain updates its nodes in place (groups with halo exchanges)
+    NodeState ns[2];
+    // the buffers the running cycle reads (old: the state it starts from)
+    // and writes (cur); k_time_increment sets them in the stepping
+    // engine's copy of this descriptor when it counts the cycle
+    NodeState old, cur;
+    double *fx, *fy, *fz, *nodalMass;
     double *e, *p, *q, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass, *vnew;
     double *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
     double *corner;

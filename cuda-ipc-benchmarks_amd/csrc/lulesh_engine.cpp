@@ -4,10 +4,12 @@
 // An Engine steps any set of groups together.  One cycle is
 //   k_time_increment  (all groups)
 //   k_force_march     (stress + hourglass forces and the ordered node gather,
-//                      all subdomains; LULESH_FORCE_PATH=twopass selects
-//                      k_elem_force + k_node_gather instead)
+//                      all subdomains; for one-subdomain groups also the
+//                      node update, old node-state buffer -> new one;
+//                      LULESH_FORCE_PATH=twopass selects k_elem_force +
+//                      k_node_gather instead)
 //   [SBN exchange]    multi-subdomain groups
-//   k_node_update
+//   [k_node_update]   when not done by k_force_march
 //   [PosVel exchange]
 //   k_kinematics
 //   [MonoQ exchange]
@@ -188,9 +190,21 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
     if (kc) e->kchunk = atoi(kc);
     std::vector<DomDev> doms;
     std::vector<GroupCtl *> gptr;
+    std::vector<int> dom0;
     for (auto *g : groups) {
         if (g->ndom > 1) e->fused = false;
-        for (auto *d : g->doms) doms.push_back(d->dev);
+        // node-state buffers as of the group's last counted cycle
+        GroupCtl G;
+        DomCtl D0;
+        int rc0 = group_read_ctl(g, &G, &D0, g->stream);
+        if (rc0) { engine_destroy(e); return rc0; }
+        dom0.push_back((int)doms.size());
+        for (auto *d : g->doms) {
+            DomDev D = d->dev;
+            D.cur = D.ns[D0.cycle & 1];
+            D.old = D.ns[(D0.cycle & 1) ^ 1];
+            doms.push_back(D);
+        }
         gptr.push_back(g->gctl);
         if (g->ndom > 1) {
             // build the exchange plans now: no allocation may happen while a
@@ -203,8 +217,20 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
         }
     }
     e->ndoms = (int)doms.size();
+    {
+        const char *fu = getenv("LULESH_NO_FUSED_UPDATE");
+        e->fuse_update = e->fused && e->march && !(fu && fu[0] == '1');
+        for (const DomDev &D : doms)
+            if (D.ns[0].x == D.ns[1].x) e->fuse_update = false;
+        // forces go to memory when an exchange or a separate update reads
+        // them, or a member group asked to keep them
+        e->store_forces = !e->fused || (e->march && !e->fuse_update);
+        for (auto *g : groups)
+            if (g->keep_forces) e->store_forces = true;
+    }
     if (hipMalloc((void **)&e->d_doms, std::max<size_t>(1, doms.size()) * sizeof(DomDev)) != hipSuccess ||
-        hipMalloc((void **)&e->d_groups, std::max<size_t>(1, gptr.size()) * sizeof(GroupCtl *)) != hipSuccess) {
+        hipMalloc((void **)&e->d_groups, std::max<size_t>(1, gptr.size()) * sizeof(GroupCtl *)) != hipSuccess ||
+        hipMalloc((void **)&e->d_dom0, std::max<size_t>(1, dom0.size()) * sizeof(int)) != hipSuccess) {
         engine_destroy(e);
         return LULESH_ENOMEM;
     }
@@ -212,6 +238,8 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
         (void)hipMemcpy(e->d_doms, doms.data(), doms.size() * sizeof(DomDev), hipMemcpyHostToDevice);
     if (!gptr.empty())
         (void)hipMemcpy(e->d_groups, gptr.data(), gptr.size() * sizeof(GroupCtl *), hipMemcpyHostToDevice);
+    if (!dom0.empty())
+        (void)hipMemcpy(e->d_dom0, dom0.data(), dom0.size() * sizeof(int), hipMemcpyHostToDevice);
     const int rc = build_tiles(e);
     if (rc) { engine_destroy(e); return rc; }
     *out = e;
@@ -224,6 +252,7 @@ void engine_destroy(Engine *e)
     engine_drop_graph(e);
     if (e->d_doms) (void)hipFree(e->d_doms);
     if (e->d_groups) (void)hipFree(e->d_groups);
+    if (e->d_dom0) (void)hipFree(e->d_dom0);
     if (e->d_etiles) (void)hipFree(e->d_etiles);
     if (e->d_qtiles) (void)hipFree(e->d_qtiles);
     if (e->d_ntiles) (void)hipFree(e->d_ntiles);
@@ -293,22 +322,26 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
 {
     const int ng = (int)e->groups.size();
     int rc = LULESH_OK;
-    STEP(0, HIPCHK(launch_time_increment(e->d_groups, ng, s)));
+    STEP(0, HIPCHK(launch_time_increment(e->d_groups, ng, e->d_doms, e->d_dom0, s)));
     if (e->march) {
-        STEP(1, HIPCHK(launch_force_march(e->d_doms, e->d_ctiles, e->n_ctiles, e->kchunk_used, s)));
+        STEP(1, HIPCHK(launch_force_march(e->d_doms, e->d_ctiles, e->n_ctiles, e->kchunk_used,
+                                          e->fuse_update, e->store_forces, s)));
         if (!e->fused) {
             STEP(6, rc = exchange_all(e, LULESH_MSG_COMM_SBN, 3, kSbn, s));
             if (rc) return rc;
         }
-        STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+        if (!e->fuse_update)
+            STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
     } else {
         STEP(1, HIPCHK(launch_elem_force(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
-        STEP(2, HIPCHK(launch_node_gather(e->fused, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+        const bool gather_update = e->fused && !e->store_forces;
+        STEP(2, HIPCHK(launch_node_gather(gather_update, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
         if (!e->fused) {
             STEP(6, rc = exchange_all(e, LULESH_MSG_COMM_SBN, 3, kSbn, s));
             if (rc) return rc;
-            STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
         }
+        if (!gather_update)
+            STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
     }
     if (!e->fused) {
         STEP(6, rc = exchange_all(e, LULESH_MSG_SYNC_POS_VEL, 6, kPosVel, s));
@@ -430,6 +463,7 @@ int lulesh_batch_destroy(lulesh_batch *b)
 int lulesh_batch_set_max_cycles(lulesh_batch *b, int m)
 {
     if (!b) return LULESH_EINVAL;
+    HIPCHK(hipStreamSynchronize(b->stream));
     for (auto *g : b->groups) {
         g->max_cycles = m;
         const int rc = group_set_param(g);

@@ -45,6 +45,7 @@ struct lulesh_group {
     lulesh::DomCtl *dctl = nullptr;
     double stoptime = 1.0e-2, qstop = 1.0e+12;
     int max_cycles = -1;
+    bool keep_forces = false;   // store fx..fz even when no exchange needs them
     const lulesh_comm_backend_ops *backend = nullptr;
     void *backend_ctx = nullptr;
     std::map<std::string, std::unique_ptr<lulesh::DevPlan>> plans;
@@ -63,6 +64,7 @@ struct Engine {
     DomDev *d_doms = nullptr;
     int ndoms = 0;
     GroupCtl **d_groups = nullptr;
+    int *d_dom0 = nullptr;      // index in d_doms of each group's first subdomain
     Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_qtiles = nullptr;
     int n_etiles = 0, n_ntiles = 0;
     int tile_e = 256, tile_n = 256;
@@ -73,6 +75,11 @@ struct Engine {
     int kchunk_used = 8;
     ColTile *d_ctiles = nullptr;
     int n_ctiles = 0;
+    // node update done by the force march as each node plane completes
+    // (every member double-buffers its node state, no halo exchange sits
+    // between force and update); LULESH_NO_FUSED_UPDATE=1 keeps them apart
+    bool fuse_update = true;
+    bool store_forces = true;   // fx..fz written (exchange or keep_forces)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_cycles = 0;

@@ -6,11 +6,15 @@
 
 namespace lulesh {
 
-hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, hipStream_t s);
+hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, DomDev *doms, const int *dom0,
+                                 hipStream_t s);
 hipError_t launch_elem_force(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 // fused force + ordered gather (16x16-thread column tiles, kchunk node planes)
-hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, hipStream_t s);
+// advance: also the node update (old -> new node-state buffer) as each
+// node plane completes; store_f: write fx..fz
+hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, bool advance,
+                              bool store_f, hipStream_t s);
 hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_kinematics(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);

@@ -39,6 +39,12 @@ __device__ __forceinline__ bool group_stopped(const GroupCtl &G)
     return (G.done | G.status) != 0;
 }
 
+// node state at the start of the running cycle and the buffer the cycle's
+// node update writes (the same buffer when the subdomain does not
+// double-buffer its node state)
+__device__ __forceinline__ NodeState ns_old(const DomDev &D) { return D.old; }
+__device__ __forceinline__ NodeState ns_new(const DomDev &D) { return D.cur; }
+
 __device__ __forceinline__ void raise_status(GroupCtl *G, int code)
 {
     atomicCAS(&G->status, 0, code);
@@ -55,7 +61,10 @@ __device__ __forceinline__ unsigned long long dbits(double v)
 // and reduced with MIN across the group (the MPI_Allreduce); every
 // subdomain then applies the ratio clamps to its own deltatime.
 // ---------------------------------------------------------------------------
-__global__ void k_time_increment(GroupCtl *const *groups, int ngroups)
+// doms/dom0: the engine's subdomain descriptors and the index of each
+// group's first one; the counted cycle n reads node-state buffer (n-1)&1
+// and writes buffer n&1
+__global__ void k_time_increment(GroupCtl *const *groups, int ngroups, DomDev *doms, const int *dom0)
 {
     const int gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= ngroups) return;
@@ -97,6 +106,9 @@ __global__ void k_time_increment(GroupCtl *const *groups, int ngroups)
         // CalcTimeConstraintsForElems restarts from 1e20 (lulesh.cc:3019-3020)
         d.dtcourant_bits = dbits(1.0e+20);
         d.dthydro_bits = dbits(1.0e+20);
+        DomDev &dd = doms[dom0[gi] + r];
+        dd.cur = dd.ns[d.cycle & 1];
+        dd.old = dd.ns[(d.cycle & 1) ^ 1];
     }
 }
 
@@ -138,10 +150,11 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
     int n[8], i, j, k;
     elem_nodes(D.nx, el, n, i, j, k);
     const size_t E = (size_t)D.numElem;
+    const NodeState S = ns_old(D);
 
     double x[8], y[8], z[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) { x[c] = D.x[n[c]]; y[c] = D.y[n[c]]; z[c] = D.z[n[c]]; }
+    for (int c = 0; c < 8; ++c) { x[c] = S.x[n[c]]; y[c] = S.y[n[c]]; z[c] = S.z[n[c]]; }
 
     // stress part
     {
@@ -180,7 +193,7 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
             hg[c][m] = hg_gamma(m, c) - volinv * (dvdx[c] * hx + dvdy[c] * hy + dvdz[c] * hz);
     }
     const double coefficient = D.hgcoef[el];
-    const double *vel[3] = {D.xd, D.yd, D.zd};
+    const double *vel[3] = {S.xd, S.yd, S.zd};
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
         double vl[8], h[4];
@@ -198,6 +211,37 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
             D.corner[((3 + comp) * 8 + c) * E + el] =
                 coefficient * (hg[c][0] * h[0] + hg[c][1] * h[1] + hg[c][2] * h[2] + hg[c][3] * h[3]);
     }
+}
+
+// CalcAccelerationForNodes + ApplyAccelerationBoundaryConditionsForNodes +
+// CalcVelocityForNodes + CalcPositionForNodes (lulesh.cc:1415-1550) for
+// node (i,j,k) of subdomain D
+__device__ __forceinline__ void node_advance(const DomDev &D, const NodeState &src, const NodeState &dst,
+                                             int nd, int i, int j, int k, double fx, double fy, double fz,
+                                             double dt)
+{
+    const double one_over_nMass = 1. / D.nodalMass[nd];
+    double ax = fx * one_over_nMass, ay = fy * one_over_nMass, az = fz * one_over_nMass;
+    // symmetry planes exist on the subdomains at the mesh origin
+    // (SetupSymmetryPlanes, lulesh-init.cc:487-508)
+    if (D.col == 0 && i == 0) ax = 0.0;
+    if (D.row == 0 && j == 0) ay = 0.0;
+    if (D.plane == 0 && k == 0) az = 0.0;
+    double vx = src.xd[nd] + ax * dt;
+    if (fabs(vx) < Phys::u_cut) vx = 0.0;
+    double vy = src.yd[nd] + ay * dt;
+    if (fabs(vy) < Phys::u_cut) vy = 0.0;
+    double vz = src.zd[nd] + az * dt;
+    if (fabs(vz) < Phys::u_cut) vz = 0.0;
+    const double px = src.x[nd] + vx * dt;
+    const double py = src.y[nd] + vy * dt;
+    const double pz = src.z[nd] + vz * dt;
+    dst.xd[nd] = vx;
+    dst.yd[nd] = vy;
+    dst.zd[nd] = vz;
+    dst.x[nd] = px;
+    dst.y[nd] = py;
+    dst.z[nd] = pz;
 }
 
 // ---------------------------------------------------------------------------
@@ -221,11 +265,11 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
 // ---------------------------------------------------------------------------
 constexpr int kMThreads = 256;   // threads (element slots) per column tile
 
-__device__ __forceinline__ void load_elem_xyz(const DomDev &D, const int n[8], double x[8], double y[8],
+__device__ __forceinline__ void load_elem_xyz(const NodeState &S, const int n[8], double x[8], double y[8],
                                               double z[8])
 {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) { x[c] = D.x[n[c]]; y[c] = D.y[n[c]]; z[c] = D.z[n[c]]; }
+    for (int c = 0; c < 8; ++c) { x[c] = S.x[n[c]]; y[c] = S.y[n[c]]; z[c] = S.z[n[c]]; }
 }
 
 // stress corner forces -(sigma B) from the face normals
@@ -297,7 +341,7 @@ __device__ __forceinline__ void hg_force(const HgModes &H, const double vl[8], d
 }
 
 __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, const ColTile *tiles,
-                                                        int ntiles, int kchunk)
+                                                        int ntiles, int kchunk, int advance, int store_f)
 {
     __shared__ double lds[24][kMThreads];
     const ColTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -318,6 +362,8 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     // lower / upper corner slots seen from the node thread:
     // own element, +x, +y, +x+y neighbours
     const int t1 = tid + 1, t2 = tid + ex, t3 = tid + ex + 1;
+    const NodeState S = ns_old(D);
+    const double dt = D.ctl->deltatime;
     double ps[3] = {0.0, 0.0, 0.0}, ph[3] = {0.0, 0.0, 0.0};   // node plane c+1 partials
     double fs[3] = {0.0, 0.0, 0.0};                              // node plane c stress sums
 
@@ -335,9 +381,9 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
 #pragma unroll
                 for (int q = 0; q < 4; ++q) { x[q] = x[q + 4]; y[q] = y[q + 4]; z[q] = z[q + 4]; }
 #pragma unroll
-                for (int q = 4; q < 8; ++q) { x[q] = D.x[n[q]]; y[q] = D.y[n[q]]; z[q] = D.z[n[q]]; }
+                for (int q = 4; q < 8; ++q) { x[q] = S.x[n[q]]; y[q] = S.y[n[q]]; z[q] = S.z[n[q]]; }
             } else {
-                load_elem_xyz(D, n, x, y, z);
+                load_elem_xyz(S, n, x, y, z);
             }
         }
         have_lo = el_ok;
@@ -392,13 +438,13 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                 hg_modes(x, y, z, D.volo[el] * vv, D.hgcoef[el], H);
                 double vl[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) vl[q] = D.xd[n[q]];
+                for (int q = 0; q < 8; ++q) vl[q] = S.xd[n[q]];
                 hg_force(H, vl, hx);
 #pragma unroll
-                for (int q = 0; q < 8; ++q) vl[q] = D.yd[n[q]];
+                for (int q = 0; q < 8; ++q) vl[q] = S.yd[n[q]];
                 hg_force(H, vl, hy);
 #pragma unroll
-                for (int q = 0; q < 8; ++q) vl[q] = D.zd[n[q]];
+                for (int q = 0; q < 8; ++q) vl[q] = S.zd[n[q]];
                 hg_force(H, vl, hz);
             } else {
 #pragma unroll
@@ -435,40 +481,18 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             }
             if (c >= k0) {
                 const int nd = c * s2 + nj * s1 + ni;
-                D.fx[nd] = f[0];
-                D.fy[nd] = f[1];
-                D.fz[nd] = f[2];
+                if (store_f) {
+                    D.fx[nd] = f[0];
+                    D.fy[nd] = f[1];
+                    D.fz[nd] = f[2];
+                }
+                // the node is complete: advance it into the new buffer
+                // (other workgroups only read the old one)
+                if (advance) node_advance(D, S, ns_new(D), nd, ni, nj, c, f[0], f[1], f[2], dt);
             }
         }
         __syncthreads();
     }
-}
-
-// CalcAccelerationForNodes + ApplyAccelerationBoundaryConditionsForNodes +
-// CalcVelocityForNodes + CalcPositionForNodes (lulesh.cc:1415-1550) for
-// node (i,j,k) of subdomain D
-__device__ __forceinline__ void node_advance(const DomDev &D, int nd, int i, int j, int k,
-                                             double fx, double fy, double fz, double dt)
-{
-    const double one_over_nMass = 1. / D.nodalMass[nd];
-    double ax = fx * one_over_nMass, ay = fy * one_over_nMass, az = fz * one_over_nMass;
-    // symmetry planes exist on the subdomains at the mesh origin
-    // (SetupSymmetryPlanes, lulesh-init.cc:487-508)
-    if (D.col == 0 && i == 0) ax = 0.0;
-    if (D.row == 0 && j == 0) ay = 0.0;
-    if (D.plane == 0 && k == 0) az = 0.0;
-    double vx = D.xd[nd] + ax * dt;
-    if (fabs(vx) < Phys::u_cut) vx = 0.0;
-    double vy = D.yd[nd] + ay * dt;
-    if (fabs(vy) < Phys::u_cut) vy = 0.0;
-    double vz = D.zd[nd] + az * dt;
-    if (fabs(vz) < Phys::u_cut) vz = 0.0;
-    D.xd[nd] = vx;
-    D.yd[nd] = vy;
-    D.zd[nd] = vz;
-    D.x[nd] = D.x[nd] + vx * dt;
-    D.y[nd] = D.y[nd] + vy * dt;
-    D.z[nd] = D.z[nd] + vz * dt;
 }
 
 // ---------------------------------------------------------------------------
@@ -525,7 +549,7 @@ __global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const T
     }
     const double fx = sx + hx, fy = sy + hy, fz = sz + hz;
     if (FUSE_UPDATE) {
-        node_advance(D, nd, i, j, k, fx, fy, fz, D.ctl->deltatime);
+        node_advance(D, ns_old(D), ns_new(D), nd, i, j, k, fx, fy, fz, D.ctl->deltatime);
     } else {
         D.fx[nd] = fx;
         D.fy[nd] = fy;
@@ -545,7 +569,7 @@ __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const T
     const int rem = nd - k * s1 * s1;
     const int j = rem / s1;
     const int i = rem - j * s1;
-    node_advance(D, nd, i, j, k, D.fx[nd], D.fy[nd], D.fz[nd], D.ctl->deltatime);
+    node_advance(D, ns_old(D), ns_new(D), nd, i, j, k, D.fx[nd], D.fy[nd], D.fz[nd], D.ctl->deltatime);
 }
 
 // ---------------------------------------------------------------------------
@@ -555,24 +579,10 @@ __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const T
 // MonoQ gradients.  The deviatoric strains of the reference are not
 // stored: nothing reads them.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Tile *tiles, int ntiles)
+__device__ __forceinline__ void kin_element(const DomDev &D, GroupCtl *G, int el, double dt,
+                                            const double x[8], const double y[8], const double z[8],
+                                            const double xv[8], const double yv[8], const double zv[8])
 {
-    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const DomDev &D = doms[t.dom];
-    GroupCtl *G = D.grp;
-    if (group_stopped(*G)) return;
-    const int el = t.start + threadIdx.x;
-    if (el >= D.numElem) return;
-    int n[8], i, j, k;
-    elem_nodes(D.nx, el, n, i, j, k);
-    const double dt = D.ctl->deltatime;
-
-    double x[8], y[8], z[8], xv[8], yv[8], zv[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        x[c] = D.x[n[c]]; y[c] = D.y[n[c]]; z[c] = D.z[n[c]];
-        xv[c] = D.xd[n[c]]; yv[c] = D.yd[n[c]]; zv[c] = D.zd[n[c]];
-    }
     const double volo = D.volo[el];
     const double volume = elem_volume(x, y, z);
     const double vnew = volume / volo;
@@ -642,6 +652,28 @@ __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Ti
     dyv = -0.25 * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
     dzv = -0.25 * ((zv[0] + zv[1] + zv[5] + zv[4]) - (zv[3] + zv[2] + zv[6] + zv[7]));
     D.delv_eta[el] = ax * dxv + ay * dyv + az * dzv;
+}
+
+__global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Tile *tiles, int ntiles)
+{
+    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const DomDev &D = doms[t.dom];
+    GroupCtl *G = D.grp;
+    if (group_stopped(*G)) return;
+    const int el = t.start + threadIdx.x;
+    if (el >= D.numElem) return;
+    int n[8], i, j, k;
+    elem_nodes(D.nx, el, n, i, j, k);
+    const double dt = D.ctl->deltatime;
+
+    const NodeState S = ns_new(D);
+    double x[8], y[8], z[8], xv[8], yv[8], zv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        x[c] = S.x[n[c]]; y[c] = S.y[n[c]]; z[c] = S.z[n[c]];
+        xv[c] = S.xd[n[c]]; yv[c] = S.yd[n[c]]; zv[c] = S.zd[n[c]];
+    }
+    kin_element(D, G, el, dt, x, y, z, xv, yv, zv);
 }
 
 // one direction of CalcMonotonicQRegionForElems (lulesh.cc:2307-2340):
@@ -969,9 +1001,11 @@ __global__ __launch_bounds__(256) void k_unpack(const UnpackTgt *tg, int n, cons
 // ---------------------------------------------------------------------------
 static inline int grid_of(int n, int b) { return (n + b - 1) / b; }
 
-hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, hipStream_t s)
+hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, DomDev *doms, const int *dom0,
+                                 hipStream_t s)
 {
-    hipLaunchKernelGGL(k_time_increment, dim3(grid_of(ngroups, 64)), dim3(64), 0, s, groups, ngroups);
+    hipLaunchKernelGGL(k_time_increment, dim3(grid_of(ngroups, 64)), dim3(64), 0, s, groups, ngroups, doms,
+                       dom0);
     return hipGetLastError();
 }
 
@@ -990,10 +1024,12 @@ hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt,
     return hipGetLastError();
 }
 
-hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, hipStream_t s)
+hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, bool advance,
+                              bool store_f, hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_force_march, dim3(nt), dim3(kMThreads), 0, s, d, t, nt, kchunk);
+    hipLaunchKernelGGL(k_force_march, dim3(nt), dim3(kMThreads), 0, s, d, t, nt, kchunk, advance ? 1 : 0,
+                       store_f ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -1047,9 +1083,9 @@ __global__ __launch_bounds__(256) void k_setup_coords(DomDev D)
     const int nx = D.nx, s1 = nx + 1;
     const int k = nd / (s1 * s1), rem = nd - k * s1 * s1, j = rem / s1, i = rem - j * s1;
     const double mesh = (double)(D.tp * nx);
-    D.x[nd] = 1.125 * (double)(D.col * nx + i) / mesh;
-    D.y[nd] = 1.125 * (double)(D.row * nx + j) / mesh;
-    D.z[nd] = 1.125 * (double)(D.plane * nx + k) / mesh;
+    D.ns[0].x[nd] = 1.125 * (double)(D.col * nx + i) / mesh;
+    D.ns[0].y[nd] = 1.125 * (double)(D.row * nx + j) / mesh;
+    D.ns[0].z[nd] = 1.125 * (double)(D.plane * nx + k) / mesh;
 }
 
 // element volumes (lulesh-init.cc:151-162): volo = elemMass = volume, v = 1
@@ -1061,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_setup_volumes(DomDev D)
     elem_nodes(D.nx, el, n, i, j, k);
     double x[8], y[8], z[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) { x[c] = D.x[n[c]]; y[c] = D.y[n[c]]; z[c] = D.z[n[c]]; }
+    for (int c = 0; c < 8; ++c) { x[c] = D.ns[0].x[n[c]]; y[c] = D.ns[0].y[n[c]]; z[c] = D.ns[0].z[n[c]]; }
     const double vol = elem_volume(x, y, z);
     D.volo[el] = vol;
     D.elemMass[el] = vol;

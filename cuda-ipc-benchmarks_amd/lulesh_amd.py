@@ -40,6 +40,7 @@ SIGNATURES = {
     "lulesh_group_set_stoptime": (_i, [_vp, _d]),
     "lulesh_group_set_qstop": (_i, [_vp, _d]),
     "lulesh_group_set_max_cycles": (_i, [_vp, _i]),
+    "lulesh_group_set_keep_forces": (_i, [_vp, _i]),
     "lulesh_group_run": (_i, [_vp, _i, _vp]),
     "lulesh_group_enqueue": (_i, [_vp, _i, _vp]),
     "lulesh_group_sync": (_i, [_vp, _ip, _dp, _dp, _vp]),
@@ -245,6 +246,10 @@ class Group:
 
     def set_max_cycles(self, m):
         _chk(load().lulesh_group_set_max_cycles(self.h, m), "max_cycles")
+
+    def set_keep_forces(self, keep=True):
+        """Store fx..fz every cycle (one-subdomain groups skip it otherwise)."""
+        _chk(load().lulesh_group_set_keep_forces(self.h, 1 if keep else 0), "keep_forces")
 
     def run(self, cycles_per_sync=0):
         """Runs to the stop condition; returns the status (0 / -1 / -2)."""

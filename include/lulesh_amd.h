@@ -98,6 +98,13 @@ int lulesh_group_set_stoptime(lulesh_group *g, double stoptime);
 int lulesh_group_set_qstop(lulesh_group *g, double qstop);
 /* Cap on the cycle count, the reference's -i option (default: none). */
 int lulesh_group_set_max_cycles(lulesh_group *g, int max_cycles);
+/* Forces (LULESH_FX..FZ) of the last cycle.  A group with one subdomain
+ * advances each node as soon as its force is complete and does not store
+ * the force unless asked to with keep = 1 (costs one extra node-array
+ * write per component per cycle); lulesh_domain_get_field of FX..FZ on
+ * such a group fails with LULESH_EINVAL otherwise.  Groups with several
+ * subdomains always store them: the CommSBN exchange sums them. */
+int lulesh_group_set_keep_forces(lulesh_group *g, int keep);
 
 /* Runs the time loop of main (lulesh.cc:3222-3235) on the device until
  * time >= stoptime, the cycle cap, or an error.  cycles_per_sync cycles are
@@ -139,7 +146,11 @@ int lulesh_domain_sizes(const lulesh_domain *d, int *nx, int *num_elem,
 /* Host copies of a field (n = number of doubles the buffer holds). */
 int lulesh_domain_get_field(lulesh_domain *d, int field, double *host, size_t n);
 int lulesh_domain_set_field(lulesh_domain *d, int field, const double *host, size_t n);
-/* Device pointer of a field (stays valid for the life of the group). */
+/* Device pointer of a field (the allocation stays valid for the life of
+ * the group).  A group with one subdomain keeps its node positions and
+ * velocities (X..ZD) in two buffers that alternate every cycle: the pointer
+ * returned for those fields holds the current state until the next cycle
+ * runs.  Ask again after stepping. */
 int lulesh_domain_field_ptr(lulesh_domain *d, int field, double **dev_ptr);
 /* Region bookkeeping of CreateRegionIndexSets (lulesh-init.cc:377-484) and
  * the per-element repetition count of main (lulesh.cc:3171-3192). */

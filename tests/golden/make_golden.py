@@ -12,6 +12,10 @@ the two smallest meshes the fields themselves.
 
     python tests/golden/make_golden.py            # small configurations
     python tests/golden/make_golden.py --s100     # adds -s 100 -i 200 (minutes)
+
+The full-length -s 100 record was made from a separate run of the
+reference (LULESH_REF_DUMP=dump.bin oracle/_ref/lulesh_ref -s 100 > out.txt):
+    python tests/golden/make_golden.py --from-dump "-s 100 full" "-s 100" dump.bin out.txt
 """
 import argparse
 import hashlib
@@ -80,7 +84,12 @@ def run(args, threads):
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), LULESH_REF_DUMP=dump)
         out = subprocess.run([REF] + args.split(), env=env, capture_output=True, text=True,
                              check=True).stdout
-        nx, cyc, t, dt, fields = read_dump(dump)
+        return record(args, dump, out)
+
+
+def record(args, dump, out):
+    """Golden record of one reference run from its dump file and stdout."""
+    nx, cyc, t, dt, fields = read_dump(dump)
     mad, tad, mrd = symmetry(fields["e"], nx)
     kw = dict(zip(args.split()[::2], args.split()[1::2]))
     rec = {
@@ -104,7 +113,19 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--s100", action="store_true")
     ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--from-dump", nargs=4, metavar=("KEY", "ARGS", "DUMP", "STDOUT"),
+                    help="record an earlier run of the reference (e.g. the full -s 100 run, "
+                         "about an hour on 8 cores) from its dump file and saved stdout")
     a = ap.parse_args()
+    path = os.path.join(HERE, "ref_omp.json")
+    if a.from_dump:
+        key, args, dump, out = a.from_dump
+        db = json.load(open(path))
+        rec, _ = record(args, dump, open(out).read())
+        db[key] = rec
+        json.dump(db, open(path, "w"), indent=1, sort_keys=True)
+        print(key, rec["cycle"], rec["origin_energy_dec"])
+        return
     if not os.path.exists(REF):
         subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
     path = os.path.join(HERE, "ref_omp.json")

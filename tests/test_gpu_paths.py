@@ -1,0 +1,78 @@
+"""Every stepping path of the library gives the oracle's state bit for bit:
+the fused force + node-update march (default for one-subdomain groups),
+the same with a separate node update, the two-pass corner-array path,
+other z-chunk depths, eager launches instead of graphs; node-state buffer
+alternation across separate run calls and host field writes; the forces
+kept on request."""
+import numpy as np
+import pytest
+
+from conftest import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+NODE = ["x", "y", "z", "xd", "yd", "zd"]
+ELEM = ["e", "p", "q", "v", "ss", "delv", "vdov", "arealg", "vnew"]
+
+
+def _compare(g, o, what):
+    for f in NODE + ELEM:
+        assert_bitwise(g.field(f), o.field(f), f"{what} {f}")
+
+
+@pytest.mark.parametrize("env", [
+    {"LULESH_NO_FUSED_UPDATE": "1"},
+    {"LULESH_FORCE_PATH": "twopass"},
+    {"LULESH_KCHUNK": "1"},
+    {"LULESH_KCHUNK": "7"},
+    {"LULESH_NO_GRAPH": "1"},
+])
+def test_paths_bitwise(amd, oracle_mod, monkeypatch, env):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    o = oracle_mod.Oracle(12, cbrt_mode=oracle_mod.CBRT_EXACT)
+    assert o.run() == 0
+    g = amd.Group(12)
+    assert g.run() == 0
+    _compare(g, o, str(env))
+    g.close()
+    o.close()
+
+
+def test_buffers_across_runs_and_field_writes(amd, oracle_mod):
+    o = oracle_mod.Oracle(10, cbrt_mode=oracle_mod.CBRT_EXACT)
+    g = amd.Group(10)
+    o.run(7)
+    g.set_max_cycles(7)
+    assert g.run() == 0
+    _compare(g, o, "after 7 cycles")
+    # perturb the velocities on an odd cycle: the write must land in the
+    # buffer the next cycle starts from
+    for f in ["xd", "zd"]:
+        v = o.field(f) * 1.5
+        o.set_field(f, v)
+        g.domains[0].set_field(f, v)
+    o.run(6)
+    g.set_max_cycles(13)
+    assert g.run() == 0
+    _compare(g, o, "after 13 cycles")
+
+
+def test_keep_forces(amd, oracle_mod):
+    o = oracle_mod.Oracle(16, cbrt_mode=oracle_mod.CBRT_EXACT)
+    o.run(25)
+    g = amd.Group(16)
+    g.set_keep_forces(True)
+    g.set_max_cycles(25)
+    assert g.run() == 0
+    for f in ["fx", "fy", "fz"]:
+        assert_bitwise(g.field(f), o.field(f), f)
+    _compare(g, o, "keep_forces")
+
+
+def test_forces_not_kept_rejected(amd):
+    g = amd.Group(8)
+    g.set_max_cycles(3)
+    assert g.run() == 0
+    with pytest.raises(amd.LuleshError):
+        g.field("fx")

@@ -47,3 +47,27 @@ def test_s100_full_run_vs_reference(amd):
     g = amd.Group(100)
     assert g.run() == 0
     _check_record(g, rec)
+
+
+ORACLE = json.load(open(os.path.join(GOLDEN, "oracle_digests.json")))
+
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("key", ["-s 100 -i 200", "-s 100 full"])
+def test_s100_bitwise_vs_oracle_digests(amd, key):
+    """Every final field of the -s 100 run equals, bit for bit, the CPU
+    oracle's with the correctly rounded cube root (sha256 of each field)."""
+    rec = ORACLE[key]
+    g = amd.Group(100)
+    if rec["its"] >= 0:
+        g.set_max_cycles(rec["its"])
+    assert g.run() == 0
+    rc, cyc, t, dt = g.sync()
+    assert cyc == rec["cycle"]
+    assert t.hex() == rec["time"]
+    for f, h in rec["sha256"].items():
+        assert _sha(g.field(f)) == h, f
