@@ -59,8 +59,9 @@ static void engine_drop_graph(Engine *e)
 }
 
 // Column-tile shape for a mesh of nx elements per edge: ex x ey element
-// slots (<= 256) covering (ex-1) x (ey-1) nodes each; pick the shape with
-// the fewest slots over the whole node plane (boundary tiles are partly
+// slots (<= 256) covering (ex-1) x (ey-1) nodes each, with an
+// (ex+1) x (ey+1) node patch per plane in LDS; pick the shape with the
+// fewest busy waves over the whole node plane (boundary tiles are partly
 // empty, so 16 x 16 is not always best for small meshes).
 static void tile_shape(int nx, int &ex, int &ey)
 {
@@ -69,6 +70,7 @@ static void tile_shape(int nx, int &ex, int &ey)
     ex = ey = 16;
     for (int a = 4; a <= 64; ++a)
         for (int b = 4; b <= 64 && a * b <= 256; ++b) {
+            if ((a + 1) * (b + 1) > kForcePatch) continue;   // node patch held in LDS
             const long tiles = (long)((s1 + a - 2) / (a - 1)) * ((s1 + b - 2) / (b - 1));
             const long waves = tiles * ((a * b + 63) / 64);   // busy waves per plane
             const long slots = tiles * a * b;
@@ -117,6 +119,12 @@ static int auto_kchunk(const Engine *e)
 
 static int build_tiles(Engine *e)
 {
+    for (auto *g : e->groups)
+        for (auto *d : g->doms) {
+            int ex, ey;
+            tile_shape(d->host.nx, ex, ey);
+            if (ex * ey > 256 || (ex + 1) * (ey + 1) > kForcePatch) return LULESH_EINVAL;
+        }
     std::vector<Tile> et, nt;
     std::vector<ColTile> ct;
     const int kc = e->kchunk > 0 ? e->kchunk : auto_kchunk(e);

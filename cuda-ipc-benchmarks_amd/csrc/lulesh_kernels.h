@@ -11,6 +11,8 @@ hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, DomDev *d
 hipError_t launch_elem_force(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 // fused force + ordered gather (16x16-thread column tiles, kchunk node planes)
+// node slots per plane of the force kernel's LDS patch: (ex+1)*(ey+1) must fit
+constexpr int kForcePatch = 320;
 // advance: also the node update (old -> new node-state buffer) as each
 // node plane completes; store_f: write fx..fz
 hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, bool advance,

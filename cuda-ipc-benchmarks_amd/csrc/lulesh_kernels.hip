@@ -264,6 +264,7 @@ __device__ __forceinline__ void node_advance(const DomDev &D, const NodeState &s
 // started from +0.0 unchanged.
 // ---------------------------------------------------------------------------
 constexpr int kMThreads = 256;   // threads (element slots) per column tile
+constexpr int kPatch = kForcePatch;   // node slots per plane: (ex+1)*(ey+1)
 
 __device__ __forceinline__ void load_elem_xyz(const NodeState &S, const int n[8], double x[8], double y[8],
                                               double z[8])
@@ -299,9 +300,9 @@ struct HgModes {
 __device__ __forceinline__ void hg_modes(const double x[8], const double y[8], const double z[8],
                                          double determ, double coefficient, HgModes &H)
 {
-    double dvdx[8], dvdy[8], dvdz[8];
-    volume_derivs(x, y, z, dvdx, dvdy, dvdz);
-    const double volinv = 1.0 / determ;
+    // the four modes' coordinate sums first, so x..z can die before the
+    // volume derivatives and the mode vectors are live together
+    double hmx[4], hmy[4], hmz[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         double hx = x[0] * hg_gamma(m, 0), hy = y[0] * hg_gamma(m, 0), hz = z[0] * hg_gamma(m, 0);
@@ -311,10 +312,18 @@ __device__ __forceinline__ void hg_modes(const double x[8], const double y[8], c
             hy = hy + y[c] * hg_gamma(m, c);
             hz = hz + z[c] * hg_gamma(m, c);
         }
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-            H.g[c][m] = hg_gamma(m, c) - volinv * (dvdx[c] * hx + dvdy[c] * hy + dvdz[c] * hz);
+        hmx[m] = hx;
+        hmy[m] = hy;
+        hmz[m] = hz;
     }
+    double dvdx[8], dvdy[8], dvdz[8];
+    volume_derivs(x, y, z, dvdx, dvdy, dvdz);
+    const double volinv = 1.0 / determ;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            H.g[c][m] = hg_gamma(m, c) - volinv * (dvdx[c] * hmx[m] + dvdy[c] * hmy[m] + dvdz[c] * hmz[m]);
     H.coefficient = coefficient;
 }
 
@@ -343,20 +352,21 @@ __device__ __forceinline__ void hg_force(const HgModes &H, const double vl[8], d
 __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, const ColTile *tiles,
                                                         int ntiles, int kchunk, int advance, int store_f)
 {
-    __shared__ double lds[24][kMThreads];
+    __shared__ double lds[24][kMThreads];       // corner forces of one phase
+    __shared__ double sn[2][6][kPatch];          // node planes (x y z xd yd zd), by plane parity
     const ColTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     const DomDev &D = doms[t.dom];
     GroupCtl *G = D.grp;
     if (group_stopped(*G)) return;
     const int nx = D.nx, s1 = nx + 1, s2 = s1 * s1;
     const int tid = threadIdx.x;
-    const int ex = t.ex;
+    const int ex = t.ex, ey = t.ey, px = ex + 1;
     const int ta = tid % ex, tb = tid / ex;
-    const bool slot_ok = tb < t.ey;
+    const bool slot_ok = tb < ey;
     const int a = t.i0 - 1 + ta, b = t.j0 - 1 + tb;
     const bool col_ok = slot_ok && a >= 0 && a < nx && b >= 0 && b < nx;
     const int ni = t.i0 + ta, nj = t.j0 + tb;
-    const bool node_ok = ta < ex - 1 && tb < t.ey - 1 && ni <= nx && nj <= nx;
+    const bool node_ok = ta < ex - 1 && tb < ey - 1 && ni <= nx && nj <= nx;
     const int k0 = t.k0;
     const int k1 = min(t.k0 + kchunk, nx + 1);
     // lower / upper corner slots seen from the node thread:
@@ -364,46 +374,100 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     const int t1 = tid + 1, t2 = tid + ex, t3 = tid + ex + 1;
     const NodeState S = ns_old(D);
     const double dt = D.ctl->deltatime;
+
+    // Node patch: slot (pi, pj) holds node (i0-1+pi, j0-1+pj) of a node
+    // plane.  The element's corners 0..3 are slots e0, e0+1, e0+px+1,
+    // e0+px of its lower plane, 4..7 the same slots of the upper one.  A
+    // thread loads the slot of the node it owns, (ta+1, tb+1) = corner 2
+    // of its element; the first ex+ey+1 threads also load one slot of the
+    // patch's row 0 / column 0.
+    const int e0 = tb * px + ta;
+    const int cs[4] = {e0, e0 + 1, e0 + px + 1, e0 + px};
+    const int own_sl = e0 + px + 1;
+    const int own_i = ni, own_j = nj;
+    const bool own_in = slot_ok && own_i <= nx && own_j <= nx;
+    int xt_sl = -1, xt_i = 0, xt_j = 0;
+    if (tid < ex + ey + 1) {
+        const int pi = tid < px ? tid : 0, pj = tid < px ? 0 : tid - px + 1;
+        xt_sl = pj * px + pi;
+        xt_i = t.i0 - 1 + pi;
+        xt_j = t.j0 - 1 + pj;
+        if (xt_i < 0 || xt_i > nx || xt_j < 0 || xt_j > nx) xt_sl = -1;
+    }
+    double pf_own[6], pf_xt[6];
+    auto fetch = [&](int kn) {           // issue the loads of node plane kn
+        if (kn < 0 || kn > nx) return;
+        if (own_in) {
+            const int nd = kn * s2 + own_j * s1 + own_i;
+            pf_own[0] = S.x[nd]; pf_own[1] = S.y[nd]; pf_own[2] = S.z[nd];
+            pf_own[3] = S.xd[nd]; pf_own[4] = S.yd[nd]; pf_own[5] = S.zd[nd];
+        }
+        if (xt_sl >= 0) {
+            const int nd = kn * s2 + xt_j * s1 + xt_i;
+            pf_xt[0] = S.x[nd]; pf_xt[1] = S.y[nd]; pf_xt[2] = S.z[nd];
+            pf_xt[3] = S.xd[nd]; pf_xt[4] = S.yd[nd]; pf_xt[5] = S.zd[nd];
+        }
+    };
+    auto stage = [&](int kn) {           // write the fetched plane kn to LDS
+        if (kn < 0 || kn > nx) return;
+        double (*P)[kPatch] = sn[kn & 1];
+        if (own_in)
+#pragma unroll
+            for (int f = 0; f < 6; ++f) P[f][own_sl] = pf_own[f];
+        if (xt_sl >= 0)
+#pragma unroll
+            for (int f = 0; f < 6; ++f) P[f][xt_sl] = pf_xt[f];
+    };
+    // element fields of plane c (p, q, v, volo, hgcoef), fetched a plane ahead
+    double ef[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ef_next[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    auto fetch_elem = [&](int c, double out[5]) {
+        if (!(col_ok && c >= 0 && c < nx)) return;
+        const int el = (c * nx + b) * nx + a;
+        out[0] = D.p[el]; out[1] = D.q[el]; out[2] = D.v[el]; out[3] = D.volo[el]; out[4] = D.hgcoef[el];
+    };
+
+    fetch(k0 - 1);
+    stage(k0 - 1);
+    fetch(k0);
+    stage(k0);
+    fetch_elem(k0 - 1, ef);
+    __syncthreads();
+
     double ps[3] = {0.0, 0.0, 0.0}, ph[3] = {0.0, 0.0, 0.0};   // node plane c+1 partials
     double fs[3] = {0.0, 0.0, 0.0};                              // node plane c stress sums
-
-    double x[8], y[8], z[8];
-    bool have_lo = false;   // x..z[4..7] hold the previous element's top corners
     for (int c = k0 - 1; c < k1; ++c) {
         const bool el_ok = col_ok && c >= 0 && c < nx;
         const int el = (c * nx + b) * nx + a;
-        int n[8];
-        if (el_ok) {
-            const int n0 = c * s2 + b * s1 + a;
-            n[0] = n0; n[1] = n0 + 1; n[2] = n0 + s1 + 1; n[3] = n0 + s1;
-            n[4] = n0 + s2; n[5] = n0 + s2 + 1; n[6] = n0 + s2 + s1 + 1; n[7] = n0 + s2 + s1;
-            if (have_lo) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) { x[q] = x[q + 4]; y[q] = y[q + 4]; z[q] = z[q + 4]; }
-#pragma unroll
-                for (int q = 4; q < 8; ++q) { x[q] = S.x[n[q]]; y[q] = S.y[n[q]]; z[q] = S.z[n[q]]; }
-            } else {
-                load_elem_xyz(S, n, x, y, z);
-            }
-        }
-        have_lo = el_ok;
+        const double (*Lo)[kPatch] = sn[c & 1];
+        const double (*Up)[kPatch] = sn[(c + 1) & 1];
+        // loads for the next plane run while this one is computed
+        if (c + 2 <= k1) fetch(c + 2);
+        fetch_elem(c + 1, ef_next);
+        double nmass = 0.0;
+        if (advance && node_ok && c >= k0) nmass = D.nodalMass[c * s2 + nj * s1 + ni];
+
         // ---- stress corners ----
-        {
-            double fx[8], fy[8], fz[8];
-            if (el_ok) {
-                const double sig = -D.p[el] - D.q[el];
-                const double vol = elem_stress(x, y, z, sig, fx, fy, fz);
-                if (vol <= 0.0) raise_status(G, -1);
-            } else {
+        if (el_ok) {
+            double x[8], y[8], z[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) fx[q] = fy[q] = fz[q] = 0.0;
+            for (int q = 0; q < 4; ++q) {
+                x[q] = Lo[0][cs[q]]; y[q] = Lo[1][cs[q]]; z[q] = Lo[2][cs[q]];
+                x[q + 4] = Up[0][cs[q]]; y[q + 4] = Up[1][cs[q]]; z[q + 4] = Up[2][cs[q]];
             }
+            const double sig = -ef[0] - ef[1];
+            const Jac J = elem_jacobian(x, y, z);
+            if (J.volume <= 0.0) raise_status(G, -1);
+            double bx[8], by[8], bz[8];
+            node_normals(x, y, z, bx, by, bz);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                lds[0 * 8 + q][tid] = fx[q];
-                lds[1 * 8 + q][tid] = fy[q];
-                lds[2 * 8 + q][tid] = fz[q];
+                lds[0 * 8 + q][tid] = -(sig * bx[q]);
+                lds[1 * 8 + q][tid] = -(sig * by[q]);
+                lds[2 * 8 + q][tid] = -(sig * bz[q]);
             }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 24; ++q) lds[q][tid] = 0.0;
         }
         __syncthreads();
         if (node_ok) {
@@ -429,33 +493,34 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         }
         __syncthreads();
         // ---- hourglass corners ----
-        {
-            double hx[8], hy[8], hz[8];
-            if (el_ok) {
-                const double vv = D.v[el];
-                if (vv <= 0.0) raise_status(G, -1);
-                HgModes H;
-                hg_modes(x, y, z, D.volo[el] * vv, D.hgcoef[el], H);
-                double vl[8];
+        if (el_ok) {
+            const double vv = ef[2];
+            if (vv <= 0.0) raise_status(G, -1);
+            HgModes H;
+            {
+                double x[8], y[8], z[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) vl[q] = S.xd[n[q]];
-                hg_force(H, vl, hx);
-#pragma unroll
-                for (int q = 0; q < 8; ++q) vl[q] = S.yd[n[q]];
-                hg_force(H, vl, hy);
-#pragma unroll
-                for (int q = 0; q < 8; ++q) vl[q] = S.zd[n[q]];
-                hg_force(H, vl, hz);
-            } else {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) hx[q] = hy[q] = hz[q] = 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    x[q] = Lo[0][cs[q]]; y[q] = Lo[1][cs[q]]; z[q] = Lo[2][cs[q]];
+                    x[q + 4] = Up[0][cs[q]]; y[q + 4] = Up[1][cs[q]]; z[q + 4] = Up[2][cs[q]];
+                }
+                hg_modes(x, y, z, ef[3] * vv, ef[4], H);
             }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                lds[0 * 8 + q][tid] = hx[q];
-                lds[1 * 8 + q][tid] = hy[q];
-                lds[2 * 8 + q][tid] = hz[q];
+            for (int d = 0; d < 3; ++d) {
+                double vl[8], h[8];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    vl[q] = Lo[3 + d][cs[q]];
+                    vl[q + 4] = Up[3 + d][cs[q]];
+                }
+                hg_force(H, vl, h);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) lds[d * 8 + q][tid] = h[q];
             }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 24; ++q) lds[q][tid] = 0.0;
         }
         __syncthreads();
         if (node_ok) {
@@ -486,11 +551,36 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                     D.fy[nd] = f[1];
                     D.fz[nd] = f[2];
                 }
-                // the node is complete: advance it into the new buffer
-                // (other workgroups only read the old one)
-                if (advance) node_advance(D, S, ns_new(D), nd, ni, nj, c, f[0], f[1], f[2], dt);
+                if (advance) {
+                    // the node is complete: advance it from its old state
+                    // (its own slot of the lower plane) into the new buffer
+                    const double one_over_nMass = 1. / nmass;
+                    double ax = f[0] * one_over_nMass, ay = f[1] * one_over_nMass,
+                           az = f[2] * one_over_nMass;
+                    if (D.col == 0 && ni == 0) ax = 0.0;
+                    if (D.row == 0 && nj == 0) ay = 0.0;
+                    if (D.plane == 0 && c == 0) az = 0.0;
+                    double vx = Lo[3][own_sl] + ax * dt;
+                    if (fabs(vx) < Phys::u_cut) vx = 0.0;
+                    double vy = Lo[4][own_sl] + ay * dt;
+                    if (fabs(vy) < Phys::u_cut) vy = 0.0;
+                    double vz = Lo[5][own_sl] + az * dt;
+                    if (fabs(vz) < Phys::u_cut) vz = 0.0;
+                    const NodeState N = ns_new(D);
+                    N.xd[nd] = vx;
+                    N.yd[nd] = vy;
+                    N.zd[nd] = vz;
+                    N.x[nd] = Lo[0][own_sl] + vx * dt;
+                    N.y[nd] = Lo[1][own_sl] + vy * dt;
+                    N.z[nd] = Lo[2][own_sl] + vz * dt;
+                }
             }
         }
+        // the lower plane is done with (this thread's own slot was read
+        // just above): it takes node plane c+2
+        if (c + 2 <= k1) stage(c + 2);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) ef[q] = ef_next[q];
         __syncthreads();
     }
 }
