@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <map>
 #include <memory>
 #include <string>
@@ -224,6 +225,7 @@ int group_plan(lulesh_group *g, int msgType, int nf, const int *fields, bool doF
                DevPlan **out)
 {
     if (nf < 1 || nf > 6) return LULESH_EINVAL;
+    if (g->comm_down && g->ndom > 1) return LULESH_EINVAL;   // receive buffers released
     const std::string key = plan_key(msgType, nf, fields, doFlag, planeOnly);
     auto it = g->plans.find(key);
     if (it != g->plans.end()) { *out = it->second.get(); return LULESH_OK; }
@@ -336,8 +338,14 @@ int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **o
         if (rc) return rc;
         rc = domain_upload(d);
         if (rc) return rc;
+        // the comm-setup phase: the backend's whole receive-side set-up,
+        // per subdomain (lulesh.cu:515-527 times commAllocRecv per rank)
+        const auto t0 = std::chrono::steady_clock::now();
         rc = g->backend->alloc_recv(d, (size_t)d->host.comBufSize, &d->recv, g->backend_ctx);
         if (rc) return rc;
+        HIPCHK(hipDeviceSynchronize());
+        g->phase_setup_ms = std::max(
+            g->phase_setup_ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     }
     // control blocks: GroupCtl then ndom DomCtl in one allocation
     const size_t cbytes = sizeof(GroupCtl) + (size_t)g->ndom * sizeof(DomCtl);
@@ -785,19 +793,78 @@ int lulesh_group_set_comm_backend(lulesh_group *g, const lulesh_comm_backend_ops
         return LULESH_EINVAL;
     HIPCHK(hipStreamSynchronize(g->stream));
     // receive buffers belong to the backend: hand them over
+    g->phase_setup_ms = 0.0;
     for (auto *d : g->doms) {
         double *old = d->recv;
         double *fresh = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
         int rc = ops->alloc_recv(d, (size_t)d->host.comBufSize, &fresh, ctx);
         if (rc) return rc;
+        HIPCHK(hipDeviceSynchronize());
+        g->phase_setup_ms = std::max(
+            g->phase_setup_ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
         if (old) g->backend->teardown(d, old, g->backend_ctx);
         d->recv = fresh;
     }
+    g->comm_down = false;
     g->backend = ops;
     g->backend_ctx = ctx;
     g->plans.clear();
     if (g->engine) { engine_destroy(g->engine); g->engine = nullptr; }
     return LULESH_OK;
+}
+
+int lulesh_group_comm_teardown(lulesh_group *g)
+{
+    if (!g) return LULESH_EINVAL;
+    if (g->comm_down) return LULESH_OK;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    if (g->engine) { engine_destroy(g->engine); g->engine = nullptr; }
+    g->plans.clear();
+    g->phase_free_ms = 0.0;
+    for (auto *d : g->doms) {
+        // the comm-free phase, per subdomain (lulesh.cu:4909-4913)
+        const auto t0 = std::chrono::steady_clock::now();
+        if (d->recv) {
+            const int rc = g->backend->teardown(d, d->recv, g->backend_ctx);
+            if (rc) return rc;
+        }
+        HIPCHK(hipDeviceSynchronize());
+        d->recv = nullptr;
+        g->phase_free_ms = std::max(
+            g->phase_free_ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    g->comm_down = true;
+    return LULESH_OK;
+}
+
+int lulesh_group_phase_times(const lulesh_group *g, double out[4])
+{
+    if (!g || !out) return LULESH_EINVAL;
+    out[0] = g->phase_setup_ms;
+    out[1] = 0.0;   // window construction: no window in the device-local backend
+    out[2] = 0.0;   // peer query / mapping: none either
+    out[3] = g->phase_free_ms;
+    return LULESH_OK;
+}
+
+int lulesh_domain_write_solution(lulesh_domain *d, const char *path)
+{
+    if (!d || !path) return LULESH_EINVAL;
+    const size_t nn = (size_t)d->host.numNode;
+    std::vector<double> x(nn), y(nn), z(nn);
+    int rc = lulesh_domain_get_field(d, LULESH_X, x.data(), nn);
+    if (rc < 0) return rc;
+    rc = lulesh_domain_get_field(d, LULESH_Y, y.data(), nn);
+    if (rc < 0) return rc;
+    rc = lulesh_domain_get_field(d, LULESH_Z, z.data(), nn);
+    if (rc < 0) return rc;
+    FILE *f = fopen(path, "wb");
+    if (!f) return LULESH_EINVAL;
+    // write_solution's record per node (lulesh.cu:4584-4603)
+    for (size_t i = 0; i < nn; ++i)
+        fprintf(f, "%10d\n%.10f\n%.10f\n%.10f\n", (int)i, x[i], y[i], z[i]);
+    return fclose(f) == 0 ? LULESH_OK : LULESH_EINVAL;
 }
 
 int lulesh_group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, lulesh_stream_t stream)

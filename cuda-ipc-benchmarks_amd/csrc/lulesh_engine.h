@@ -46,6 +46,9 @@ struct lulesh_group {
     double stoptime = 1.0e-2, qstop = 1.0e+12;
     int max_cycles = -1;
     bool keep_forces = false;   // store fx..fz even when no exchange needs them
+    // lifecycle phases (max over subdomains), milliseconds
+    double phase_setup_ms = 0.0, phase_free_ms = 0.0;
+    bool comm_down = false;     // receive buffers released
     const lulesh_comm_backend_ops *backend = nullptr;
     void *backend_ctx = nullptr;
     std::map<std::string, std::unique_ptr<lulesh::DevPlan>> plans;

@@ -10,6 +10,11 @@
 //   -t <tp>     subdomains per edge; tp^3 subdomains share the GPU (default 1)
 //   -p          print progress every 100 cycles
 //   -q          quiet
+//   -w <0|1>    write subdomain 0's final coordinates to xyz.asc, as the
+//               CUDA reference's write_solution does (default 1)
+//
+// After the run it prints the reference's PHASE_* lifecycle lines
+// (LULESH/cuda/src/lulesh.cu:4918-4941).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -29,7 +34,8 @@ static int parse_int(const char *s, int *out)
 
 int main(int argc, char **argv)
 {
-    int its = -1, nx = 30, nr = 11, bal = 1, cost = 1, tp = 1, prog = 0, quiet = 0;
+    const auto t_app = std::chrono::steady_clock::now();
+    int its = -1, nx = 30, nr = 11, bal = 1, cost = 1, tp = 1, prog = 0, quiet = 0, wsol = 1;
     for (int i = 1; i < argc; ++i) {
         int *dst = nullptr;
         if (!strcmp(argv[i], "-s")) dst = &nx;
@@ -38,6 +44,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "-b")) dst = &bal;
         else if (!strcmp(argv[i], "-c")) dst = &cost;
         else if (!strcmp(argv[i], "-t")) dst = &tp;
+        else if (!strcmp(argv[i], "-w")) dst = &wsol;
         else if (!strcmp(argv[i], "-p")) { prog = 1; continue; }
         else if (!strcmp(argv[i], "-q")) { quiet = 1; continue; }
         else {
@@ -101,6 +108,25 @@ int main(int argc, char **argv)
         printf("FOM                  = %10.8g (z/s)\n\n", 1000.0 / grind2);
         printf("FullPrecision OriginEnergy = %.17e\n", v[0]);
     }
+    if (wsol) {
+        lulesh_domain *d0 = nullptr;
+        if (lulesh_group_domain(g, 0, &d0) == 0) {
+            rc = lulesh_domain_write_solution(d0, "xyz.asc");
+            if (rc) fprintf(stderr, "write_solution: %s\n", lulesh_status_string(rc));
+        }
+    }
+    rc = lulesh_group_comm_teardown(g);
+    if (rc) fprintf(stderr, "comm_teardown: %s\n", lulesh_status_string(rc));
+    double ph[4] = {0, 0, 0, 0};
+    lulesh_group_phase_times(g, ph);
     lulesh_group_destroy(g);
+    const double app_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_app).count();
+    printf("PHASE_COMM_SETUP_ms   %.4f\n", ph[0]);
+    printf("PHASE_WIN_ALLOCATE_ms %.4f\n", ph[1]);
+    printf("PHASE_PEER_QUERY_ms   %.4f\n", ph[2]);
+    printf("PHASE_COMM_FREE_ms    %.4f\n", ph[3]);
+    printf("PHASE_APP_TOTAL_ms    %.4f\n", app_ms);
+    fflush(stdout);
     return 0;
 }

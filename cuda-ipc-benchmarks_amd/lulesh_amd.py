@@ -41,6 +41,9 @@ SIGNATURES = {
     "lulesh_group_set_qstop": (_i, [_vp, _d]),
     "lulesh_group_set_max_cycles": (_i, [_vp, _i]),
     "lulesh_group_set_keep_forces": (_i, [_vp, _i]),
+    "lulesh_group_comm_teardown": (_i, [_vp]),
+    "lulesh_group_phase_times": (_i, [_vp, _dp]),
+    "lulesh_domain_write_solution": (_i, [_vp, ctypes.c_char_p]),
     "lulesh_group_run": (_i, [_vp, _i, _vp]),
     "lulesh_group_enqueue": (_i, [_vp, _i, _vp]),
     "lulesh_group_sync": (_i, [_vp, _ip, _dp, _dp, _vp]),
@@ -246,6 +249,18 @@ class Group:
 
     def set_max_cycles(self, m):
         _chk(load().lulesh_group_set_max_cycles(self.h, m), "max_cycles")
+
+    def comm_teardown(self):
+        _chk(load().lulesh_group_comm_teardown(self.h), "comm_teardown")
+
+    def phase_times(self):
+        """(comm set-up, window alloc, peer query, comm teardown) in ms."""
+        out = np.zeros(4, dtype=np.float64)
+        _chk(load().lulesh_group_phase_times(self.h, _p(out)), "phase_times")
+        return tuple(float(v) for v in out)
+
+    def write_solution(self, path, rank=0):
+        _chk(load().lulesh_domain_write_solution(self.domains[rank].h, str(path).encode()), "write_solution")
 
     def set_keep_forces(self, keep=True):
         """Store fx..fz every cycle (one-subdomain groups skip it otherwise)."""

@@ -98,6 +98,21 @@ int lulesh_group_set_stoptime(lulesh_group *g, double stoptime);
 int lulesh_group_set_qstop(lulesh_group *g, double qstop);
 /* Cap on the cycle count, the reference's -i option (default: none). */
 int lulesh_group_set_max_cycles(lulesh_group *g, int max_cycles);
+/* Lifecycle accounting (lulesh.cu:4918-4941, comm_backend.h:33-48).
+ * lulesh_group_comm_teardown releases the backend's receive buffers (after
+ * the last exchange; a multi-subdomain group cannot step afterwards) and
+ * times it.  lulesh_group_phase_times fills, in milliseconds and as the
+ * maximum over subdomains: [0] comm set-up (the backend's alloc_recv at
+ * group creation or backend change), [1] window allocation and [2] peer
+ * query (breakdowns a window-based backend would give; 0 here), [3] comm
+ * teardown. */
+int lulesh_group_comm_teardown(lulesh_group *g);
+int lulesh_group_phase_times(const lulesh_group *g, double out[4]);
+/* The final node coordinates of one subdomain as the reference's
+ * write_solution writes them (xyz.asc, lulesh.cu:4584-4603): per node its
+ * index ("%10d") and x, y, z ("%.10f"), one value per line. */
+int lulesh_domain_write_solution(lulesh_domain *d, const char *path);
+
 /* Forces (LULESH_FX..FZ) of the last cycle.  A group with one subdomain
  * advances each node as soon as its force is complete and does not store
  * the force unless asked to with keep = 1 (costs one extra node-array
