@@ -92,6 +92,31 @@ def measured_traffic(k, nx, fused_update=False):
     return None
 
 
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (an FMA counts as 2)
+
+
+def profiled_value(key, k, nx, fused_update=False):
+    """A per-launch figure of kernel class k at -s nx from the rocprofv3
+    summaries in profiles/ (lines '<key> <kernel> s<nx> <value>')."""
+    sym = KERNEL_SYMBOLS.get(k)
+    if sym is None or TWOPASS:
+        return None
+    if k == 1 and fused_update:
+        sym += "+update"
+    pdir = os.path.join(REPO, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, fn)) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 4 and w[0] == key and w[1] == sym and w[2] == f"s{nx}":
+                    return float(w[3])
+    return None
+
+
 def cpu_baseline(nx, target_s=15.0):
     """The reference's own CPU code on the host cores, bounded sample."""
     ref = os.path.join(REPO, "oracle", "_ref", "lulesh_ref")
@@ -190,6 +215,15 @@ def bench_plain(args):
                      "traffic": measured_traffic(dom, nx, fu), "algorithmic_bytes": b},
         "kernels": kern,
     }
+    # the dominant kernel is FP64-issue bound: its FP64 rate against the
+    # vector peak, from the FLOPs per launch rocprofv3 counted (profiles/)
+    flops = profiled_value("fp64_flops_per_launch", dom, nx, fu)
+    if flops:
+        tf = flops / (per[dom] * 1e-3) / 1e12
+        line["roofline"]["compute"] = {
+            "bound": "fp64 valu", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tf / FP64_PEAK_TFLOPS, "flops_per_launch": flops,
+            "valu_issue_busy": profiled_value("valu_busy", dom, nx, fu)}
     g.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(nx)
