@@ -156,11 +156,35 @@ static int build_tiles(Engine *e)
             for (int s = 0; s < h.numNode; s += e->tile_n) nt.push_back(Tile{di, s, 0, 0, 0});
             ++di;
         }
+    // EOS tiles: 256 elements when the engine has enough elements to give
+    // every CU two such workgroups, else 128 or 64, so that the repetitions
+    // of a small mesh spread over more lanes than it has elements
+    long long etot = 0;
+    for (auto *g : e->groups)
+        for (auto *d : g->doms) etot += d->host.numElem;
+    int cus = 256, dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    e->qeos_te = etot >= 2LL * 256 * cus ? 256 : etot >= 2LL * 128 * cus ? 128 : 64;
+    if (const char *qe = getenv("LULESH_QEOS_TILE")) {
+        const int v = atoi(qe);
+        if (v == 64 || v == 128 || v == 256) e->qeos_te = v;
+    }
+    std::vector<Tile> qt;
+    for (int di = 0, gi = 0; gi < (int)e->groups.size(); ++gi)
+        for (auto *d : e->groups[gi]->doms) {
+            const HostDomain &h = d->host;
+            for (int s = 0; s < h.numElem; s += e->qeos_te) {
+                const int end = std::min(s + e->qeos_te, h.numElem);
+                const long long a = h.evprefix64[s / 64], b = h.evprefix64[(end + 63) / 64];
+                qt.push_back(Tile{di, s, a, (int)(b - a), 0});
+            }
+            ++di;
+        }
     // EOS launch order: the kernels map workgroup b to tile chunk b % 8 (one
     // chunk of consecutive tiles per XCD), slot b / 8; inside each chunk the
     // tiles with the most EOS evaluations go first, so the long workgroups
     // do not start last and run alone at the end of the launch
-    std::vector<Tile> qt = et;
     {
         const size_t n = qt.size(), q = n / 8, r = n % 8;
         size_t a = 0;
@@ -177,6 +201,7 @@ static int build_tiles(Engine *e)
     e->d_etiles = e->d_ntiles = e->d_qtiles = nullptr;
     e->n_etiles = (int)et.size();
     e->n_ntiles = (int)nt.size();
+    e->n_qtiles = (int)qt.size();
     HIPCHK(hipMalloc((void **)&e->d_etiles, std::max<size_t>(1, et.size()) * sizeof(Tile)));
     HIPCHK(hipMalloc((void **)&e->d_ntiles, std::max<size_t>(1, nt.size()) * sizeof(Tile)));
     HIPCHK(hipMalloc((void **)&e->d_qtiles, std::max<size_t>(1, qt.size()) * sizeof(Tile)));
@@ -360,7 +385,7 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
         STEP(6, rc = exchange_all(e, LULESH_MSG_MONOQ, 3, kMonoQ, s));
         if (rc) return rc;
     }
-    STEP(5, HIPCHK(launch_qeos(e->d_doms, e->d_qtiles, e->n_etiles, e->tile_e, s)));
+    STEP(5, HIPCHK(launch_qeos(e->d_doms, e->d_qtiles, e->n_qtiles, e->qeos_te, 256, s)));
     return LULESH_OK;
 }
 

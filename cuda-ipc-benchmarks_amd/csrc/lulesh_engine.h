@@ -69,7 +69,8 @@ struct Engine {
     GroupCtl **d_groups = nullptr;
     int *d_dom0 = nullptr;      // index in d_doms of each group's first subdomain
     Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_qtiles = nullptr;
-    int n_etiles = 0, n_ntiles = 0;
+    int n_etiles = 0, n_ntiles = 0, n_qtiles = 0;
+    int qeos_te = 256;          // elements per EOS tile (256 threads per workgroup)
     int tile_e = 256, tile_n = 256;
     // fused force + gather (z-marching column tiles) or the two-pass
     // corner-array path

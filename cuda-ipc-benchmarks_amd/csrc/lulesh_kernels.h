@@ -19,7 +19,8 @@ hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kch
                               bool store_f, hipStream_t s);
 hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_kinematics(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
-hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+// te elements per tile (<= bs threads per workgroup)
+hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int te, int bs, hipStream_t s);
 // gate: skip the exchange once the group has stopped (NULL: always run)
 hipError_t launch_pack(const PackEnt *e, int n, const double *const *fptr, int nfields,
                        const GroupCtl *gate, hipStream_t s);

@@ -895,7 +895,10 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 // recomputes the same values from the same inputs, so the element's result
 // is that of the reference's sequential loop.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *tiles, int ntiles)
+// A tile holds te <= blockDim.x elements: the first te threads do the
+// per-element work, all threads share the EOS repetitions (small meshes
+// use te = 64 so a tile's repetitions spread over four times the lanes).
+__global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *tiles, int ntiles, int te)
 {
     __shared__ double s_in[7][256];
     __shared__ double s_out[5][256];
@@ -905,7 +908,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     if (group_stopped(*G)) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int el = t.start + tid;
-    const bool live = el < D.numElem;
+    const bool live = tid < te && el < D.numElem;
     double vdov = 0.0;
     if (live) {
         const int nx = D.nx;
@@ -1137,10 +1140,11 @@ hipError_t launch_kinematics(const DomDev *d, const Tile *t, int nt, int bs, hip
     return hipGetLastError();
 }
 
-hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int te, int bs, hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_qeos, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    if (te < 1 || te > bs || bs > 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_qeos, dim3(nt), dim3(bs), 0, s, d, t, nt, te);
     return hipGetLastError();
 }
 

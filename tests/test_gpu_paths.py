@@ -1,7 +1,8 @@
 """Every stepping path of the library gives the oracle's state bit for bit:
 the fused force + node-update march (default for one-subdomain groups),
 the same with a separate node update, the two-pass corner-array path,
-other z-chunk depths, eager launches instead of graphs; node-state buffer
+other z-chunk depths, eager launches instead of graphs, other EOS tile
+sizes (the s=12 default is 64 elements per tile); node-state buffer
 alternation across separate run calls and host field writes; the forces
 kept on request."""
 import numpy as np
@@ -26,6 +27,8 @@ def _compare(g, o, what):
     {"LULESH_KCHUNK": "1"},
     {"LULESH_KCHUNK": "7"},
     {"LULESH_NO_GRAPH": "1"},
+    {"LULESH_QEOS_TILE": "128"},
+    {"LULESH_QEOS_TILE": "256"},
 ])
 def test_paths_bitwise(amd, oracle_mod, monkeypatch, env):
     for k, v in env.items():
