@@ -3,20 +3,23 @@
 //
 // One cycle = TimeIncrement + LagrangeLeapFrog
 // (LULESH/omp_4.0/lulesh.cc:3224-3225, 3040-3084), as the launches
-//   k_time_increment      per group: dt, time, cycle on the device
-//   k_elem_force          per element: stress + FB hourglass corner forces
-//   k_node_gather<fuse>   per node: ordered corner gather; for a single
-//                         subdomain also the acceleration/BC/velocity/position
-//                         update (no halo step in between)
+//   k_time_increment      one wave per group: dt (MIN over its subdomains),
+//                         time, cycle on the device
+//   k_force_march         per column tile, marching up the node planes:
+//                         stress + FB hourglass corner forces, the ordered
+//                         node gather through LDS, and (subdomains without
+//                         halo exchange) the node update
 //   k_node_update         per node (multi-subdomain, after the SBN exchange)
 //   k_kinematics          per element: volume, arealg, vdov, MonoQ gradients
-//   k_qeos                per element: MonoQ region, qstop, EOS x rep,
-//                         Courant/hydro dt minima
-// Every kernel walks a tile table (subdomain, first item) so one launch
-// covers every subdomain of every group on the GPU.  Integer atomics are
-// used only for the dt minima (MIN is exact) and the error code; all force
-// sums run in the reference's element order, so results are bit-for-bit
-// reproducible.
+//   k_qeos                per element tile: MonoQ region, qstop, EOS x rep
+//                         dealt over the workgroup, Courant/hydro dt minima
+// plus the two-pass alternative of the force step (k_elem_force writes
+// corner forces, k_node_gather<fuse> gathers them per node) kept for
+// cross-checking.  Every kernel walks a tile table (subdomain, first item)
+// so one launch covers every subdomain of every group on the GPU.  Integer
+// atomics are used only for the dt minima (MIN is exact) and the error
+// code; all force sums run in the reference's element order, so results
+// are bit-for-bit reproducible.
 #include <hip/hip_runtime.h>
 
 #include "lulesh_dev.h"
@@ -64,52 +67,62 @@ __device__ __forceinline__ unsigned long long dbits(double v)
 // doms/dom0: the engine's subdomain descriptors and the index of each
 // group's first one; the counted cycle n reads node-state buffer (n-1)&1
 // and writes buffer n&1
-__global__ void k_time_increment(GroupCtl *const *groups, int ngroups, DomDev *doms, const int *dom0)
+__global__ __launch_bounds__(256) void k_time_increment(GroupCtl *const *groups, int ngroups, DomDev *doms,
+                                                         const int *dom0)
 {
-    const int gi = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wave per group, lane r = subdomain r (at most 4^3 = 64)
+    const int gi = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int r = threadIdx.x & 63;
     if (gi >= ngroups) return;
     GroupCtl &G = *groups[gi];
     if (group_stopped(G)) return;
     DomCtl *D = G.dctl;
     if (!(D[0].time < G.stoptime) || (G.max_cycles >= 0 && D[0].cycle >= G.max_cycles)) {
-        G.done = 1;
+        if (r == 0) G.done = 1;
         return;
     }
-    double newdt = 1.0e+20;
-    for (int r = 0; r < G.ndom; ++r) {
-        double gnewdt = 1.0e+20;
+    const bool mine = r < G.ndom;
+    // each subdomain's candidate, then the MIN over the group (the
+    // reference's MPI_Allreduce(MIN), lulesh.cc:227-236; MIN is exact, so
+    // any reduction order gives the sequential result)
+    double gnewdt = 1.0e+20;
+    if (mine) {
         const double dc = __longlong_as_double((long long)D[r].dtcourant_bits);
         const double dh = __longlong_as_double((long long)D[r].dthydro_bits);
         if (dc < gnewdt) gnewdt = dc / 2.0;
         if (dh < gnewdt) gnewdt = dh * 2.0 / 3.0;
-        if (r == 0 || gnewdt < newdt) newdt = gnewdt;
     }
-    for (int r = 0; r < G.ndom; ++r) {
-        DomCtl &d = D[r];
-        double targetdt = G.stoptime - d.time;
-        if ((Phys::dtfixed <= 0.0) && (d.cycle != 0)) {
-            const double olddt = d.deltatime;
-            double dt = newdt;
-            const double ratio = dt / olddt;
-            if (ratio >= 1.0) {
-                if (ratio < Phys::deltatimemultlb) dt = olddt;
-                else if (ratio > Phys::deltatimemultub) dt = olddt * Phys::deltatimemultub;
-            }
-            if (dt > Phys::dtmax) dt = Phys::dtmax;
-            d.deltatime = dt;
+    double newdt = mine ? gnewdt : __builtin_inf();
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(newdt, off, 64);
+        newdt = o < newdt ? o : newdt;
+    }
+    if (!mine) return;
+    DomCtl &d = D[r];
+    double targetdt = G.stoptime - d.time;
+    if ((Phys::dtfixed <= 0.0) && (d.cycle != 0)) {
+        const double olddt = d.deltatime;
+        double dt = newdt;
+        const double ratio = dt / olddt;
+        if (ratio >= 1.0) {
+            if (ratio < Phys::deltatimemultlb) dt = olddt;
+            else if (ratio > Phys::deltatimemultub) dt = olddt * Phys::deltatimemultub;
         }
-        if ((targetdt > d.deltatime) && (targetdt < (4.0 * d.deltatime / 3.0)))
-            targetdt = 2.0 * d.deltatime / 3.0;
-        if (targetdt < d.deltatime) d.deltatime = targetdt;
-        d.time += d.deltatime;
-        ++d.cycle;
-        // CalcTimeConstraintsForElems restarts from 1e20 (lulesh.cc:3019-3020)
-        d.dtcourant_bits = dbits(1.0e+20);
-        d.dthydro_bits = dbits(1.0e+20);
-        DomDev &dd = doms[dom0[gi] + r];
-        dd.cur = dd.ns[d.cycle & 1];
-        dd.old = dd.ns[(d.cycle & 1) ^ 1];
+        if (dt > Phys::dtmax) dt = Phys::dtmax;
+        d.deltatime = dt;
     }
+    if ((targetdt > d.deltatime) && (targetdt < (4.0 * d.deltatime / 3.0)))
+        targetdt = 2.0 * d.deltatime / 3.0;
+    if (targetdt < d.deltatime) d.deltatime = targetdt;
+    d.time += d.deltatime;
+    ++d.cycle;
+    // CalcTimeConstraintsForElems restarts from 1e20 (lulesh.cc:3019-3020)
+    d.dtcourant_bits = dbits(1.0e+20);
+    d.dthydro_bits = dbits(1.0e+20);
+    DomDev &dd = doms[dom0[gi] + r];
+    dd.cur = dd.ns[d.cycle & 1];
+    dd.old = dd.ns[(d.cycle & 1) ^ 1];
 }
 
 // node ids of element (i,j,k) in the structured block (BuildMesh,
@@ -1097,7 +1110,8 @@ static inline int grid_of(int n, int b) { return (n + b - 1) / b; }
 hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, DomDev *doms, const int *dom0,
                                  hipStream_t s)
 {
-    hipLaunchKernelGGL(k_time_increment, dim3(grid_of(ngroups, 64)), dim3(64), 0, s, groups, ngroups, doms,
+    // one 64-lane wave per group, four groups per workgroup
+    hipLaunchKernelGGL(k_time_increment, dim3(grid_of(ngroups, 4)), dim3(256), 0, s, groups, ngroups, doms,
                        dom0);
     return hipGetLastError();
 }
