@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <unordered_map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -62,6 +63,7 @@ int domain_alloc(lulesh_domain *d)
         {&a.x, nn}, {&a.y, nn}, {&a.z, nn}, {&a.xd, nn}, {&a.yd, nn}, {&a.zd, nn},
         {&b.x, nn2}, {&b.y, nn2}, {&b.z, nn2}, {&b.xd, nn2}, {&b.yd, nn2}, {&b.zd, nn2},
         {&D.fx, nn}, {&D.fy, nn}, {&D.fz, nn}, {&D.nodalMass, nn},
+        {&d->fpart[0], two ? 0 : nn}, {&d->fpart[1], two ? 0 : nn}, {&d->fpart[2], two ? 0 : nn},
         {&D.e, ne}, {&D.p, ne}, {&D.q, ne}, {&D.v, ne}, {&D.volo, ne}, {&D.delv, ne},
         {&D.vdov, ne}, {&D.arealg, ne}, {&D.ss, ne}, {&D.elemMass, ne}, {&D.vnew, ne},
         {&D.delv_xi, all}, {&D.delv_eta, all}, {&D.delv_zeta, all},
@@ -86,6 +88,12 @@ int domain_alloc(lulesh_domain *d)
     if (!two) b = a;
     D.cur = a;   // cycle 0: the set-up state is in buffer 0
     D.old = b;
+    D.fpx = D.fx;
+    D.fpy = D.fy;
+    D.fpz = D.fz;
+    D.dir_node = D.dir_elem = nullptr;
+    D.dir_sbn = D.dir_ghost = nullptr;
+    D.dom_base = 0;
     HIPCHK(hipMemset(arena, 0, bytes));
     return LULESH_OK;
 }
@@ -298,6 +306,98 @@ int plan_unpack(lulesh_group *g, DevPlan *p, int rank, hipStream_t s, bool gated
 // (lulesh.cc:1400-1408, 1607-1617, 2511-2519)
 static const int kPosVelFields[6] = {LULESH_X, LULESH_Y, LULESH_Z, LULESH_XD, LULESH_YD, LULESH_ZD};
 static const int kMonoQFields[3] = {LULESH_DELV_XI, LULESH_DELV_ETA, LULESH_DELV_ZETA};
+
+DirectTables::~DirectTables()
+{
+    if (node) (void)hipFree(node);
+    if (elem) (void)hipFree(elem);
+    if (sbn) (void)hipFree(sbn);
+    if (ghost) (void)hipFree(ghost);
+}
+
+// The direct halo steps are the device_local exchange plans read the other
+// way round: the plan says which sender index lands at which receive-buffer
+// word (pack) and which words a receiving index takes in slot order
+// (unpack); joining the two on the buffer address gives, per receiving
+// index, its senders in slot order.
+bool group_device_local(const lulesh_group *g) { return g->backend == &kDeviceLocal; }
+
+int group_direct(lulesh_group *g, DirectTables **out)
+{
+    if (g->direct) { *out = g->direct.get(); return LULESH_OK; }
+    if (g->backend != &kDeviceLocal || g->comm_down || g->ndom < 2) return LULESH_EINVAL;
+    auto plan = [g](int nf, bool planeOnly, HostPlan &hp) {
+        PlanInput in;
+        in.tp = g->tp;
+        for (auto *d : g->doms) in.doms.push_back(&d->host);
+        in.nfields = nf;
+        in.doFlag = true;
+        in.planeOnly = planeOnly;
+        in.pack_dest = [g](int from, int to, size_t off) {
+            return g->backend->pack_dest(g->doms[from], g->doms[to], off, g->backend_ctx);
+        };
+        in.recv_base = [g](int dom) -> const double * {
+            return g->backend->recv_base(g->doms[dom], g->backend_ctx);
+        };
+        return build_plan(in, hp);
+    };
+    HostPlan sbn, mq;
+    if (plan(3, false, sbn) || plan(3, true, mq)) return LULESH_EDIRECTION;
+    auto senders = [](const HostPlan &hp) {
+        std::unordered_map<const double *, DirRef> m;
+        for (const PackEnt &e : hp.pack) m[e.dst] = DirRef{e.src_dom, e.src_idx};
+        return m;
+    };
+    const int nd = g->ndom;
+    std::unique_ptr<DirectTables> t(new DirectTables());
+    t->node_off.assign(nd + 1, 0);
+    t->elem_off.assign(nd + 1, 0);
+    for (int r = 0; r < nd; ++r) {
+        t->node_off[r + 1] = t->node_off[r] + (size_t)g->doms[r]->host.numNode;
+        t->elem_off[r + 1] = t->elem_off[r] + (size_t)g->doms[r]->host.numElem;
+    }
+    std::vector<IdxSpan> node(t->node_off[nd], IdxSpan{0, 0}), elem(t->elem_off[nd], IdxSpan{0, 0});
+    std::vector<DirRef> sbn_list, ghost_list;
+    {
+        const auto from = senders(sbn);
+        for (const UnpackTgt &u : sbn.tgt) {
+            node[t->node_off[u.dom] + u.idx] = IdxSpan{(int)sbn_list.size(), u.count};
+            for (int m = 0; m < u.count; ++m) {
+                const auto it = from.find(sbn.src[u.first + m].src);
+                if (it == from.end()) return LULESH_EDIRECTION;
+                sbn_list.push_back(it->second);
+            }
+        }
+    }
+    {
+        const auto from = senders(mq);
+        std::map<std::pair<int, int>, std::vector<DirRef>> fills;   // sender element -> ghost slots
+        for (const UnpackTgt &u : mq.tgt) {
+            if (u.count != 1) return LULESH_EDIRECTION;
+            const auto it = from.find(mq.src[u.first].src);
+            if (it == from.end()) return LULESH_EDIRECTION;
+            fills[{it->second.dom, it->second.idx}].push_back(DirRef{u.dom, u.idx});
+        }
+        for (auto &kv : fills) {
+            elem[t->elem_off[kv.first.first] + kv.first.second] = IdxSpan{(int)ghost_list.size(), (int)kv.second.size()};
+            ghost_list.insert(ghost_list.end(), kv.second.begin(), kv.second.end());
+        }
+    }
+    HIPCHK(hipMalloc((void **)&t->node, node.size() * sizeof(IdxSpan)));
+    HIPCHK(hipMalloc((void **)&t->elem, elem.size() * sizeof(IdxSpan)));
+    HIPCHK(hipMalloc((void **)&t->sbn, std::max<size_t>(1, sbn_list.size()) * sizeof(DirRef)));
+    HIPCHK(hipMalloc((void **)&t->ghost, std::max<size_t>(1, ghost_list.size()) * sizeof(DirRef)));
+    HIPCHK(hipMemcpy(t->node, node.data(), node.size() * sizeof(IdxSpan), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t->elem, elem.data(), elem.size() * sizeof(IdxSpan), hipMemcpyHostToDevice));
+    if (!sbn_list.empty())
+        HIPCHK(hipMemcpy(t->sbn, sbn_list.data(), sbn_list.size() * sizeof(DirRef), hipMemcpyHostToDevice));
+    if (!ghost_list.empty())
+        HIPCHK(hipMemcpy(t->ghost, ghost_list.data(), ghost_list.size() * sizeof(DirRef),
+                         hipMemcpyHostToDevice));
+    g->direct = std::move(t);
+    *out = g->direct.get();
+    return LULESH_OK;
+}
 
 int group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, hipStream_t s, bool gated)
 {
@@ -810,6 +910,7 @@ int lulesh_group_set_comm_backend(lulesh_group *g, const lulesh_comm_backend_ops
     g->backend = ops;
     g->backend_ctx = ctx;
     g->plans.clear();
+    g->direct.reset();
     if (g->engine) { engine_destroy(g->engine); g->engine = nullptr; }
     return LULESH_OK;
 }
@@ -821,6 +922,7 @@ int lulesh_group_comm_teardown(lulesh_group *g)
     HIPCHK(hipStreamSynchronize(g->stream));
     if (g->engine) { engine_destroy(g->engine); g->engine = nullptr; }
     g->plans.clear();
+    g->direct.reset();
     g->phase_free_ms = 0.0;
     for (auto *d : g->doms) {
         // the comm-free phase, per subdomain (lulesh.cu:4909-4913)

@@ -22,6 +22,16 @@ namespace lulesh {
 
 constexpr int kCornerComps = 48;     // 6 components x 8 corners
 
+struct IdxSpan {          // [first, first + count) of a list
+    int first;
+    int count;
+};
+
+struct DirRef {           // an index of another subdomain of the group
+    int dom;
+    int idx;
+};
+
 // node positions and velocities
 struct NodeState {
     double *x, *y, *z, *xd, *yd, *zd;
@@ -37,6 +47,20 @@ struct DomDev {
     // engine's copy of this descriptor when it counts the cycle
     NodeState old, cur;
     double *fx, *fy, *fz, *nodalMass;
+    // where the force kernels store a cycle's forces: fx..fz, or, when the
+    // stepping engine sums shared nodes directly (k_node_update_direct),
+    // separate arrays holding this subdomain's own partial forces
+    double *fpx, *fpy, *fpz;
+    // direct halo steps (device_local backend): per node {first, count} of
+    // its CommSBN sources in receive-slot order, per element {first,
+    // count} of the neighbours' MonoQ ghost slots it fills; null otherwise.
+    // dom of a DirRef is a rank of the group, whose subdomain 0 is at
+    // index dom_base of the engine's descriptor array.
+    const IdxSpan *dir_node;
+    const IdxSpan *dir_elem;
+    const DirRef *dir_sbn;
+    const DirRef *dir_ghost;
+    int dom_base;
     double *e, *p, *q, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass, *vnew;
     double *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
     double *corner;

@@ -251,6 +251,35 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
     }
     e->ndoms = (int)doms.size();
     {
+        const char *nd = getenv("LULESH_NO_DIRECT");
+        e->direct = !e->fused && !(nd && nd[0] == '1');
+        for (auto *g : groups)
+            if (g->ndom > 1 && !group_device_local(g)) e->direct = false;
+        if (e->direct) {
+            for (size_t gi = 0; gi < groups.size(); ++gi) {
+                lulesh_group *g = groups[gi];
+                DirectTables *t = nullptr;
+                if (g->ndom > 1) {
+                    const int rc0 = group_direct(g, &t);
+                    if (rc0) { engine_destroy(e); return rc0; }
+                }
+                for (int r = 0; r < g->ndom; ++r) {
+                    DomDev &D = doms[dom0[gi] + r];
+                    D.dom_base = dom0[gi];
+                    if (!t) continue;
+                    lulesh_domain *d = g->doms[r];
+                    D.fpx = d->fpart[0];
+                    D.fpy = d->fpart[1];
+                    D.fpz = d->fpart[2];
+                    D.dir_node = t->node + t->node_off[r];
+                    D.dir_elem = t->elem + t->elem_off[r];
+                    D.dir_sbn = t->sbn;
+                    D.dir_ghost = t->ghost;
+                }
+            }
+        }
+    }
+    {
         const char *fu = getenv("LULESH_NO_FUSED_UPDATE");
         e->fuse_update = e->fused && e->march && !(fu && fu[0] == '1');
         for (const DomDev &D : doms)
@@ -356,24 +385,26 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
     const int ng = (int)e->groups.size();
     int rc = LULESH_OK;
     STEP(0, HIPCHK(launch_time_increment(e->d_groups, ng, e->d_doms, e->d_dom0, s)));
+    bool updated;   // node update done inside the force step
     if (e->march) {
         STEP(1, HIPCHK(launch_force_march(e->d_doms, e->d_ctiles, e->n_ctiles, e->kchunk_used,
                                           e->fuse_update, e->store_forces, s)));
-        if (!e->fused) {
-            STEP(6, rc = exchange_all(e, LULESH_MSG_COMM_SBN, 3, kSbn, s));
-            if (rc) return rc;
-        }
-        if (!e->fuse_update)
-            STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+        updated = e->fuse_update;
     } else {
         STEP(1, HIPCHK(launch_elem_force(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
         const bool gather_update = e->fused && !e->store_forces;
         STEP(2, HIPCHK(launch_node_gather(gather_update, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+        updated = gather_update;
+    }
+    if (e->direct) {
+        // CommSBN summed straight from the neighbours' partial forces
+        STEP(3, HIPCHK(launch_node_update_direct(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+    } else {
         if (!e->fused) {
             STEP(6, rc = exchange_all(e, LULESH_MSG_COMM_SBN, 3, kSbn, s));
             if (rc) return rc;
         }
-        if (!gather_update)
+        if (!updated)
             STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
     }
     if (!e->fused) {
@@ -381,7 +412,7 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
         if (rc) return rc;
     }
     STEP(4, HIPCHK(launch_kinematics(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
-    if (!e->fused) {
+    if (!e->fused && !e->direct) {   // direct: the kinematics kernel filled the ghosts
         STEP(6, rc = exchange_all(e, LULESH_MSG_MONOQ, 3, kMonoQ, s));
         if (rc) return rc;
     }

@@ -15,6 +15,16 @@ namespace lulesh {
 struct Engine;
 
 // a halo-exchange step of one group, resident on the device
+// device tables of the direct halo steps of one group (device_local
+// backend): see DomDev::dir_*
+struct DirectTables {
+    IdxSpan *node = nullptr;       // sum of numNode over the group's subdomains
+    IdxSpan *elem = nullptr;       // sum of numElem
+    DirRef *sbn = nullptr, *ghost = nullptr;
+    std::vector<size_t> node_off, elem_off;   // per rank
+    ~DirectTables();
+};
+
 struct DevPlan {
     int msgType = 0, nfields = 0, mode = 0;
     PackEnt *pack = nullptr;
@@ -35,6 +45,7 @@ struct lulesh_domain {
     void *arena = nullptr;
     size_t arena_bytes = 0;
     double *recv = nullptr;
+    double *fpart[3] = {nullptr, nullptr, nullptr};   // own partial forces (groups with halos)
 };
 
 struct lulesh_group {
@@ -52,6 +63,7 @@ struct lulesh_group {
     const lulesh_comm_backend_ops *backend = nullptr;
     void *backend_ctx = nullptr;
     std::map<std::string, std::unique_ptr<lulesh::DevPlan>> plans;
+    std::unique_ptr<lulesh::DirectTables> direct;   // built on first use
     lulesh::Engine *engine = nullptr;   // standalone stepping
     hipStream_t stream = nullptr;
 };
@@ -84,6 +96,10 @@ struct Engine {
     // between force and update); LULESH_NO_FUSED_UPDATE=1 keeps them apart
     bool fuse_update = true;
     bool store_forces = true;   // fx..fz written (exchange or keep_forces)
+    // halo groups on the device_local backend: CommSBN summed inside the
+    // node update and CommMonoQ written by the kinematics kernel instead of
+    // pack/unpack launches (LULESH_NO_DIRECT=1 keeps the exchanges)
+    bool direct = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_cycles = 0;
@@ -100,6 +116,9 @@ int engine_enqueue_cycle(Engine *e, hipStream_t s);
 int engine_profile(Engine *e, int ncycles, hipStream_t s, double ms[7]);
 
 int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **out);
+// tables of the direct halo steps (device_local backend only)
+int group_direct(lulesh_group *g, DirectTables **out);
+bool group_device_local(const lulesh_group *g);
 void group_free(lulesh_group *g);
 int group_reset_ctl(lulesh_group *g);
 int group_set_param(lulesh_group *g);

@@ -18,6 +18,8 @@ constexpr int kForcePatch = 320;
 hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, bool advance,
                               bool store_f, hipStream_t s);
 hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+// CommSBN summed directly from the neighbours' partial forces + node update
+hipError_t launch_node_update_direct(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_kinematics(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 // te elements per tile (<= bs threads per workgroup)
 hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int te, int bs, hipStream_t s);

@@ -560,9 +560,9 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             if (c >= k0) {
                 const int nd = c * s2 + nj * s1 + ni;
                 if (store_f) {
-                    D.fx[nd] = f[0];
-                    D.fy[nd] = f[1];
-                    D.fz[nd] = f[2];
+                    D.fpx[nd] = f[0];
+                    D.fpy[nd] = f[1];
+                    D.fpz[nd] = f[2];
                 }
                 if (advance) {
                     // the node is complete: advance it from its old state
@@ -654,9 +654,9 @@ __global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const T
     if (FUSE_UPDATE) {
         node_advance(D, ns_old(D), ns_new(D), nd, i, j, k, fx, fy, fz, D.ctl->deltatime);
     } else {
-        D.fx[nd] = fx;
-        D.fy[nd] = fy;
-        D.fz[nd] = fz;
+        D.fpx[nd] = fx;
+        D.fpy[nd] = fy;
+        D.fpz[nd] = fz;
     }
 }
 
@@ -675,6 +675,39 @@ __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const T
     node_advance(D, ns_old(D), ns_new(D), nd, i, j, k, D.fx[nd], D.fy[nd], D.fz[nd], D.ctl->deltatime);
 }
 
+// CommSBN + the node update in one pass (device_local backend): a shared
+// node adds the neighbours' own partial forces -- read where their force
+// kernel left them, untouched during this launch -- in this subdomain's
+// receive-slot order onto its own partial (lulesh-comm.cc:848-1257), as
+// the reference's pack-then-unpack does, stores the sums as fx..fz and
+// advances.
+__global__ __launch_bounds__(256) void k_node_update_direct(const DomDev *doms, const Tile *tiles, int ntiles)
+{
+    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const DomDev &D = doms[t.dom];
+    if (group_stopped(*D.grp)) return;
+    const int nd = t.start + threadIdx.x;
+    if (nd >= D.numNode) return;
+    const int s1 = D.nx + 1;
+    const int k = nd / (s1 * s1);
+    const int rem = nd - k * s1 * s1;
+    const int j = rem / s1;
+    const int i = rem - j * s1;
+    double fx = D.fpx[nd], fy = D.fpy[nd], fz = D.fpz[nd];
+    const IdxSpan r = D.dir_node ? D.dir_node[nd] : IdxSpan{0, 0};
+    for (int m = 0; m < r.count; ++m) {
+        const DirRef s = D.dir_sbn[r.first + m];
+        const DomDev &S = doms[D.dom_base + s.dom];
+        fx += S.fpx[s.idx];
+        fy += S.fpy[s.idx];
+        fz += S.fpz[s.idx];
+    }
+    D.fx[nd] = fx;
+    D.fy[nd] = fy;
+    D.fz[nd] = fz;
+    node_advance(D, ns_old(D), ns_new(D), nd, i, j, k, fx, fy, fz, D.ctl->deltatime);
+}
+
 // ---------------------------------------------------------------------------
 // CalcKinematicsForElems + CalcLagrangeElements + CalcMonotonicQGradients-
 // ForElems (lulesh.cc:1861-2054, 2059-2244): one gather of the element's
@@ -682,7 +715,7 @@ __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const T
 // MonoQ gradients.  The deviatoric strains of the reference are not
 // stored: nothing reads them.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void kin_element(const DomDev &D, GroupCtl *G, int el, double dt,
+__device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D, GroupCtl *G, int el, double dt,
                                             const double x[8], const double y[8], const double z[8],
                                             const double xv[8], const double yv[8], const double zv[8])
 {
@@ -738,7 +771,8 @@ __device__ __forceinline__ void kin_element(const DomDev &D, GroupCtl *G, int el
     dxv = 0.25 * ((xv[4] + xv[5] + xv[6] + xv[7]) - (xv[0] + xv[1] + xv[2] + xv[3]));
     dyv = 0.25 * ((yv[4] + yv[5] + yv[6] + yv[7]) - (yv[0] + yv[1] + yv[2] + yv[3]));
     dzv = 0.25 * ((zv[4] + zv[5] + zv[6] + zv[7]) - (zv[0] + zv[1] + zv[2] + zv[3]));
-    D.delv_zeta[el] = ax * dxv + ay * dyv + az * dzv;
+    const double dv_zeta = ax * dxv + ay * dyv + az * dzv;
+    D.delv_zeta[el] = dv_zeta;
 
     ax = dyj * dzk - dzj * dyk; ay = dzj * dxk - dxj * dzk; az = dxj * dyk - dyj * dxk;
     D.delx_xi[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
@@ -746,7 +780,8 @@ __device__ __forceinline__ void kin_element(const DomDev &D, GroupCtl *G, int el
     dxv = 0.25 * ((xv[1] + xv[2] + xv[6] + xv[5]) - (xv[0] + xv[3] + xv[7] + xv[4]));
     dyv = 0.25 * ((yv[1] + yv[2] + yv[6] + yv[5]) - (yv[0] + yv[3] + yv[7] + yv[4]));
     dzv = 0.25 * ((zv[1] + zv[2] + zv[6] + zv[5]) - (zv[0] + zv[3] + zv[7] + zv[4]));
-    D.delv_xi[el] = ax * dxv + ay * dyv + az * dzv;
+    const double dv_xi = ax * dxv + ay * dyv + az * dzv;
+    D.delv_xi[el] = dv_xi;
 
     ax = dyk * dzi - dzk * dyi; ay = dzk * dxi - dxk * dzi; az = dxk * dyi - dyk * dxi;
     D.delx_eta[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
@@ -754,7 +789,25 @@ __device__ __forceinline__ void kin_element(const DomDev &D, GroupCtl *G, int el
     dxv = -0.25 * ((xv[0] + xv[1] + xv[5] + xv[4]) - (xv[3] + xv[2] + xv[6] + xv[7]));
     dyv = -0.25 * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
     dzv = -0.25 * ((zv[0] + zv[1] + zv[5] + zv[4]) - (zv[3] + zv[2] + zv[6] + zv[7]));
-    D.delv_eta[el] = ax * dxv + ay * dyv + az * dzv;
+    const double dv_eta = ax * dxv + ay * dyv + az * dzv;
+    D.delv_eta[el] = dv_eta;
+
+    // CommMonoQ done in place (device_local backend): an element on a
+    // shared face fills its neighbours' ghost slots with its three
+    // gradients, exactly what the face message would carry
+    // (lulesh-comm.cc:1684-1837); qeos reads them after this launch
+    if (D.dir_elem) {
+        const IdxSpan r = D.dir_elem[el];
+        if (r.count > 0) {
+            for (int m = 0; m < r.count; ++m) {
+                const DirRef g = D.dir_ghost[r.first + m];
+                const DomDev &T = doms[D.dom_base + g.dom];
+                T.delv_xi[g.idx] = dv_xi;
+                T.delv_eta[g.idx] = dv_eta;
+                T.delv_zeta[g.idx] = dv_zeta;
+            }
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Tile *tiles, int ntiles)
@@ -776,7 +829,7 @@ __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Ti
         x[c] = S.x[n[c]]; y[c] = S.y[n[c]]; z[c] = S.z[n[c]];
         xv[c] = S.xd[n[c]]; yv[c] = S.yd[n[c]]; zv[c] = S.zd[n[c]];
     }
-    kin_element(D, G, el, dt, x, y, z, xv, yv, zv);
+    kin_element(doms, D, G, el, dt, x, y, z, xv, yv, zv);
 }
 
 // one direction of CalcMonotonicQRegionForElems (lulesh.cc:2307-2340):
@@ -1144,6 +1197,13 @@ hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hi
 {
     if (nt == 0) return hipSuccess;
     hipLaunchKernelGGL(k_node_update, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_update_direct(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+{
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_node_update_direct, dim3(nt), dim3(bs), 0, s, d, t, nt);
     return hipGetLastError();
 }
 

@@ -208,9 +208,11 @@ size_t domain_device_bytes(int nx, bool corner_array, long long evtotal)
 {
     const size_t ne = (size_t)nx * nx * nx, nn = (size_t)(nx + 1) * (nx + 1) * (nx + 1);
     const size_t all = ne + 6 * (size_t)nx * nx;
-    // 10 node fields, 12 element fields, 3 ghosted gradients, 3 plain
-    // gradients, [48 corner components], elemRep; 256-byte alignment slack
-    return 8 * (10 * nn + 12 * ne + 3 * all + 3 * ne + (corner_array ? 48 * ne : 0)) + 4 * ne +
+    // a one-subdomain problem: 16 node arrays (two node-state buffers, f,
+    // nodalMass), 12 element fields, 3 ghosted gradients, 3 plain
+    // gradients, [48 corner components], elemRep, the EOS work table;
+    // 256-byte alignment slack
+    return 8 * (16 * nn + 12 * ne + 3 * all + 3 * ne + (corner_array ? 48 * ne : 0)) + 4 * ne +
            2 * (size_t)evtotal + 64 * 256;
 }
 

@@ -79,3 +79,27 @@ def test_forces_not_kept_rejected(amd):
     assert g.run() == 0
     with pytest.raises(amd.LuleshError):
         g.field("fx")
+
+
+@pytest.mark.parametrize("env", [{}, {"LULESH_NO_DIRECT": "1"}, {"LULESH_FORCE_PATH": "twopass"}])
+def test_decomposed_paths_bitwise(amd, oracle_mod, monkeypatch, env):
+    """2x2x2 subdomains: the direct halo steps (CommSBN summed in the node
+    update, CommMonoQ written by the kinematics kernel), the pack/unpack
+    exchanges, and the two-pass force path all give the oracle's state."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    o = oracle_mod.Oracle(5, tp=2, cbrt_mode=oracle_mod.CBRT_EXACT)
+    assert o.run(30) == 0
+    g = amd.Group(5, tp=2)
+    g.set_max_cycles(30)
+    assert g.run() == 0
+    for r in range(8):
+        for f in NODE + ["e", "p", "q", "v", "ss", "delv_xi", "delv_eta", "delv_zeta"]:
+            assert_bitwise(g.field(f, r), o.field(f, r), f"{env} rank {r} {f}")
+    g.set_keep_forces(True)
+    g.set_max_cycles(31)
+    o.run(1)
+    assert g.run() == 0
+    for r in range(8):
+        for f in ["fx", "fy", "fz"]:
+            assert_bitwise(g.field(f, r), o.field(f, r), f"{env} rank {r} {f}")
