@@ -139,3 +139,15 @@ def test_s45_8_subdomains_200_cycles(amd):
     want = float.fromhex(rec["origin_energy"])
     assert abs(e0 - want) / want <= 1e-10
     assert "%12.6e" % e0 == "2.188600e+07"   # ucx_lulesh_experiments_46508.csv:11
+
+
+def test_tp4_short_run_bitwise(amd, oracle_mod):
+    # 64 subdomains, the largest decomposition the library takes
+    o = oracle_mod.Oracle(3, tp=4, cbrt_mode=oracle_mod.CBRT_EXACT)
+    o.run(6)
+    g = amd.Group(3, tp=4)
+    g.set_max_cycles(6)
+    assert g.run() == 0
+    for r in range(64):
+        for f in ["e", "p", "x", "xd", "delv_zeta"]:
+            assert_bitwise(g.field(f, r), o.field(f, r), f"tp4 rank {r} {f}")
