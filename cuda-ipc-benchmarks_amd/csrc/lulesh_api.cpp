@@ -19,6 +19,7 @@
 #include "lulesh_dev.h"
 #include "lulesh_engine.h"
 #include "lulesh_kernels.h"
+#include "lulesh_mesh.h"
 #include "lulesh_setup.h"
 
 using namespace lulesh;
@@ -68,11 +69,21 @@ int domain_alloc(lulesh_domain *d)
         {&D.vdov, ne}, {&D.arealg, ne}, {&D.ss, ne}, {&D.elemMass, ne}, {&D.vnew, ne},
         {&D.delv_xi, all}, {&D.delv_eta, all}, {&D.delv_zeta, all},
         {&D.delx_xi, ne}, {&D.delx_eta, ne}, {&D.delx_zeta, ne}, {&D.hgcoef, ne},
-        {&D.corner, corner_path_enabled() ? (size_t)kCornerComps * ne : 0}};
+        {&D.corner, corner_path_enabled() || h.umesh ? (size_t)kCornerComps * ne : 0}};
+    // integer tables: elemRep, then the unstructured connectivity
+    struct ISlot { const void **p; size_t bytes; };
+    const size_t um = h.umesh ? 1 : 0;
+    std::vector<ISlot> islots = {
+        {(const void **)&D.elemRep, ne * sizeof(int)},
+        {(const void **)&D.u_nodelist, um * 8 * ne * sizeof(int)},
+        {(const void **)&D.u_face, um * 8 * ne * sizeof(int)},
+        {(const void **)&D.u_nstart, um * (nn + 1) * sizeof(int)},
+        {(const void **)&D.u_ncorner, um * 8 * ne * sizeof(int)},
+        {(const void **)&D.u_nsymm, um * nn},
+        {(const void **)&D.evtab, (size_t)h.evtotal * sizeof(unsigned short)}};
     size_t bytes = 0;
     for (auto &s : slots) bytes += align256(s.n * sizeof(double));
-    bytes += align256(ne * sizeof(int));
-    bytes += align256((size_t)h.evtotal * sizeof(unsigned short));
+    for (auto &s : islots) bytes += align256(s.bytes);
     void *arena = nullptr;
     if (hipMalloc(&arena, bytes) != hipSuccess) return LULESH_ENOMEM;
     d->arena = arena;
@@ -82,9 +93,10 @@ int domain_alloc(lulesh_domain *d)
         *s.p = (double *)p;
         p += align256(s.n * sizeof(double));
     }
-    D.elemRep = (const int *)p;
-    p += align256(ne * sizeof(int));
-    D.evtab = (const unsigned short *)p;
+    for (auto &s : islots) {
+        *s.p = s.bytes ? (const void *)p : nullptr;
+        p += align256(s.bytes);
+    }
     if (!two) b = a;
     D.cur = a;   // cycle 0: the set-up state is in buffer 0
     D.old = b;
@@ -105,14 +117,32 @@ int domain_upload(lulesh_domain *d)
     D.nx = h.nx; D.numElem = h.numElem; D.numNode = h.numNode; D.allElem = h.allElem;
     D.col = h.col; D.row = h.row; D.plane = h.plane; D.tp = h.tp;
     for (int f = 0; f < 6; ++f) D.ghost[f] = h.ghost[f];
+    if (h.umesh) {
+        // coordinates and connectivity tables before the set-up kernels
+        // that read them (blocking copies on the null stream)
+        const size_t nn = h.numNode, ne = h.numElem;
+        HIPCHK(hipMemcpy(D.ns[0].x, h.ux.data(), nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(D.ns[0].y, h.uy.data(), nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(D.ns[0].z, h.uz.data(), nn * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void *)D.u_nodelist, h.u_nodelist.data(), 8 * ne * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void *)D.u_face, h.u_face.data(), 8 * ne * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void *)D.u_nstart, h.u_nstart.data(), (nn + 1) * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void *)D.u_ncorner, h.u_ncorner.data(), 8 * ne * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy((void *)D.u_nsymm, h.u_nsymm.data(), nn, hipMemcpyHostToDevice));
+    }
     // the arena was cleared on the null stream; the set-up kernels follow
     // it there, and the blocking copies below wait for them
-    HIPCHK(launch_setup(D, nullptr));
-    HIPCHK(hipMemcpy(D.e, &h.e0, 8, hipMemcpyHostToDevice));
+    HIPCHK(launch_setup(D, h.umesh, nullptr));
+    HIPCHK(hipMemcpy(D.e + h.e0_index, &h.e0, 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy((void *)D.elemRep, h.elemRep.data(), (size_t)h.numElem * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy((void *)D.evtab, h.evtab.data(), (size_t)h.evtotal * 2, hipMemcpyHostToDevice));
     HIPCHK(hipStreamSynchronize(nullptr));
-    d->host.evtab = std::vector<unsigned short>();   // the device copy is the one used
+    // the device copies are the ones used
+    HostDomain &hm = d->host;
+    hm.evtab = std::vector<unsigned short>();
+    hm.ux = hm.uy = hm.uz = std::vector<double>();
+    hm.u_nodelist = hm.u_face = hm.u_nstart = hm.u_ncorner = std::vector<int>();
+    hm.u_nsymm = std::vector<unsigned char>();
     return LULESH_OK;
 }
 
@@ -419,10 +449,19 @@ int group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, hipS
 // ---------------------------------------------------------------------------
 // groups
 // ---------------------------------------------------------------------------
+// a group under construction: released with everything it holds if set-up
+// fails part way
+struct GroupDeleter {
+    void operator()(lulesh_group *g) const { group_free(g); }
+};
+using GroupPtr = std::unique_ptr<lulesh_group, GroupDeleter>;
+
+static int group_finish(GroupPtr &g, lulesh_group **out);
+
 int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **out)
 {
     if (!out || !problem_ok(nx, tp, nr, balance, cost)) return LULESH_EINVAL;
-    std::unique_ptr<lulesh_group> g(new lulesh_group());
+    GroupPtr g(new lulesh_group());
     g->nx = nx;
     g->tp = tp;
     g->ndom = tp * tp * tp;
@@ -447,6 +486,13 @@ int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **o
         g->phase_setup_ms = std::max(
             g->phase_setup_ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     }
+    return group_finish(g, out);
+}
+
+// control blocks, hourglass coefficients and the initial nodalMass
+// exchange of a group whose subdomains are set up
+static int group_finish(GroupPtr &g, lulesh_group **out)
+{
     // control blocks: GroupCtl then ndom DomCtl in one allocation
     const size_t cbytes = sizeof(GroupCtl) + (size_t)g->ndom * sizeof(DomCtl);
     HIPCHK(hipMalloc(&g->ctl_mem, cbytes));
@@ -474,6 +520,30 @@ int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **o
     HIPCHK(hipStreamSynchronize(g->stream));
     *out = g.release();
     return LULESH_OK;
+}
+
+int group_create_mesh(const lulesh_mesh *m, int nr, int balance, int cost, lulesh_group **out)
+{
+    if (!out || !m || !mesh_problem_ok(m->numElem, nr, balance, cost)) return LULESH_EINVAL;
+    if ((int)m->elemBC.size() != m->numElem) return LULESH_EINVAL;
+    GroupPtr g(new lulesh_group());
+    g->nx = 0;
+    g->tp = 1;
+    g->ndom = 1;
+    g->umesh = true;
+    g->backend = &kDeviceLocal;
+    HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    lulesh_domain *d = new lulesh_domain();
+    d->g = g.get();
+    d->rank = 0;
+    g->doms.push_back(d);
+    build_host_domain_mesh(d->host, *m, nr, balance, cost);
+    int rc = domain_alloc(d);
+    if (rc) return rc;
+    rc = domain_upload(d);
+    if (rc) return rc;
+    // no comm buffers: the unstructured path is single-domain
+    return group_finish(g, out);
 }
 
 int group_reset_ctl(lulesh_group *g)
@@ -597,6 +667,11 @@ int lulesh_group_create(int nx, int tp, int nr, int balance, int cost, lulesh_gr
     return LULESH_OK;
 }
 
+int lulesh_group_create_mesh(const lulesh_mesh *m, int nr, int balance, int cost, lulesh_group **out)
+{
+    return group_create_mesh(m, nr, balance, cost, out);
+}
+
 int lulesh_group_destroy(lulesh_group *g)
 {
     group_free(g);
@@ -698,6 +773,15 @@ int lulesh_group_verify(lulesh_group *g, int rank, double out[4])
 {
     if (!g || rank < 0 || rank >= g->ndom || !out) return LULESH_EINVAL;
     lulesh_domain *d = g->doms[rank];
+    if (g->umesh) {
+        // the unstructured path has no plane-0 symmetry test in the
+        // reference (VerifyAndWriteFinalOutput's structured branch only,
+        // LULESH/cuda/src/lulesh.cu:4700-4737): the octant corner's energy
+        HIPCHK(hipStreamSynchronize(g->stream));
+        HIPCHK(hipMemcpy(&out[0], d->dev.e + d->host.e0_index, 8, hipMemcpyDeviceToHost));
+        out[1] = out[2] = out[3] = 0.0;
+        return LULESH_OK;
+    }
     const int nx = g->nx;
     std::vector<double> e((size_t)nx * nx);
     HIPCHK(hipStreamSynchronize(g->stream));
@@ -787,6 +871,7 @@ int lulesh_domain_get_region(lulesh_domain *d, int *host, size_t n)
 // ---------------------------------------------------------------------------
 static int check_dims(const lulesh_domain *d, int msgType, int dx, int dy, int dz)
 {
+    if (d->g->umesh) return LULESH_EINVAL;   // no comm on the unstructured path
     const int n = d->host.nx + (msgType == LULESH_MSG_MONOQ ? 0 : 1);
     if (dx != n || dy != n || dz != n) return LULESH_EINVAL;
     if (msgType != LULESH_MSG_COMM_SBN && msgType != LULESH_MSG_SYNC_POS_VEL &&
@@ -832,6 +917,7 @@ int CommSend(lulesh_domain *d, int msgType, int xferFields, const int *fields, i
 int CommSBN(lulesh_domain *d, int xferFields, const int *fields, lulesh_stream_t stream)
 {
     if (!d || !fields || xferFields < 1 || xferFields > 6) return LULESH_EINVAL;
+    if (d->g->umesh) return LULESH_EINVAL;
     lulesh_group *g = d->g;
     if (g->ndom == 1) return LULESH_OK;
     DevPlan *p = nullptr;
@@ -845,6 +931,7 @@ int CommSBN(lulesh_domain *d, int xferFields, const int *fields, lulesh_stream_t
 int CommSyncPosVel(lulesh_domain *d, lulesh_stream_t stream)
 {
     if (!d) return LULESH_EINVAL;
+    if (d->g->umesh) return LULESH_EINVAL;
     lulesh_group *g = d->g;
     if (g->ndom == 1) return LULESH_OK;
     DevPlan *p = nullptr;
@@ -857,6 +944,7 @@ int CommSyncPosVel(lulesh_domain *d, lulesh_stream_t stream)
 int CommMonoQ(lulesh_domain *d, lulesh_stream_t stream)
 {
     if (!d) return LULESH_EINVAL;
+    if (d->g->umesh) return LULESH_EINVAL;
     lulesh_group *g = d->g;
     if (g->ndom == 1) return LULESH_OK;
     DevPlan *p = nullptr;
@@ -870,13 +958,13 @@ int lulesh_comm_buf_size(const lulesh_domain *d) { return d ? d->host.comBufSize
 int lulesh_comm_recv_offset(const lulesh_domain *d, int dc, int dr, int dp, int xfer, int doRecv,
                             int planeOnly)
 {
-    if (!d) return LULESH_EINVAL;
+    if (!d || d->g->umesh) return LULESH_EINVAL;
     return recv_offset(d->host, dc, dr, dp, xfer, doRecv != 0, planeOnly != 0);
 }
 
 int lulesh_comm_get_recv_buffer(lulesh_domain *d, double *host, size_t n)
 {
-    if (!d || !host || n < (size_t)d->host.comBufSize) return LULESH_EINVAL;
+    if (!d || d->g->umesh || !host || n < (size_t)d->host.comBufSize) return LULESH_EINVAL;
     lulesh_group *g = d->g;
     const double *base = g->backend->recv_base(d, g->backend_ctx);
     HIPCHK(hipStreamSynchronize(g->stream));
@@ -888,8 +976,8 @@ const lulesh_comm_backend_ops *lulesh_device_local_backend(void) { return &kDevi
 
 int lulesh_group_set_comm_backend(lulesh_group *g, const lulesh_comm_backend_ops *ops, void *ctx)
 {
-    if (!g || !ops || !ops->alloc_recv || !ops->teardown || !ops->pack_dest || !ops->send_finish ||
-        !ops->recv_base)
+    if (!g || g->umesh || !ops || !ops->alloc_recv || !ops->teardown || !ops->pack_dest ||
+        !ops->send_finish || !ops->recv_base)
         return LULESH_EINVAL;
     HIPCHK(hipStreamSynchronize(g->stream));
     // receive buffers belong to the backend: hand them over
@@ -971,7 +1059,7 @@ int lulesh_domain_write_solution(lulesh_domain *d, const char *path)
 
 int lulesh_group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, lulesh_stream_t stream)
 {
-    if (!g || !fields) return LULESH_EINVAL;
+    if (!g || g->umesh || !fields) return LULESH_EINVAL;
     return group_exchange(g, msgType, nf, fields, stream_of(g, stream));
 }
 

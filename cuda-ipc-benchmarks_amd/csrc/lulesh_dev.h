@@ -68,6 +68,18 @@ struct DomDev {
     // the coming force computation (CalcFBHourglassForceForElems,
     // lulesh.cc:1012-1052), produced where ss and v are produced
     double *hgcoef;
+    // unstructured mesh (the reference's .lmesh input): explicit
+    // connectivity in place of the (i,j,k) formulas; null for the block.
+    //   u_nodelist  [numElem][8] node ids
+    //   u_face      [numElem][8] lxim lxip letam letap lzetam lzetap elemBC 0
+    //   u_nstart    [numNode+1]  start of each node's corners in u_ncorner
+    //   u_ncorner   8*element + corner, elements ascending per node
+    //   u_nsymm     per node: 1 symmX, 2 symmY, 4 symmZ
+    const int *u_nodelist;
+    const int *u_face;
+    const int *u_nstart;
+    const int *u_ncorner;
+    const unsigned char *u_nsymm;
     const int *elemRep;
     // one entry per EOS evaluation, elements in ascending order, each
     // repeated elemRep times (entry format: kEvLast / kEvMask)
@@ -125,6 +137,14 @@ struct ColTile {
     int ex, ey;
     int pad[2];
 };
+
+// elemBC bits (LULESH/omp_4.0/lulesh.h:60-88)
+constexpr int kXiM = 0x00007, kXiMSymm = 0x00001, kXiMFree = 0x00002, kXiMComm = 0x00004;
+constexpr int kXiP = 0x00038, kXiPSymm = 0x00008, kXiPFree = 0x00010, kXiPComm = 0x00020;
+constexpr int kEtaM = 0x001c0, kEtaMSymm = 0x00040, kEtaMFree = 0x00080, kEtaMComm = 0x00100;
+constexpr int kEtaP = 0x00e00, kEtaPSymm = 0x00200, kEtaPFree = 0x00400, kEtaPComm = 0x00800;
+constexpr int kZetaM = 0x07000, kZetaMSymm = 0x01000, kZetaMFree = 0x02000, kZetaMComm = 0x04000;
+constexpr int kZetaP = 0x38000, kZetaPSymm = 0x08000, kZetaPFree = 0x10000, kZetaPComm = 0x20000;
 
 // physics constants of Domain::Domain (LULESH/omp_4.0/lulesh-init.cc:20-38)
 struct Phys {

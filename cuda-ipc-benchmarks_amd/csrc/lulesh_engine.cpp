@@ -119,20 +119,17 @@ static int auto_kchunk(const Engine *e)
 
 static int build_tiles(Engine *e)
 {
-    for (auto *g : e->groups)
-        for (auto *d : g->doms) {
-            int ex, ey;
-            tile_shape(d->host.nx, ex, ey);
-            if (ex * ey > 256 || (ex + 1) * (ey + 1) > kForcePatch) return LULESH_EINVAL;
-        }
     std::vector<Tile> et, nt;
     std::vector<ColTile> ct;
-    const int kc = e->kchunk > 0 ? e->kchunk : auto_kchunk(e);
-    for (int di = 0, gi = 0; gi < (int)e->groups.size(); ++gi)
+    // column tiles of the force march (the block's (i,j,k) structure; an
+    // unstructured engine uses the element/node tiles only)
+    const int kc = e->umesh ? 1 : e->kchunk > 0 ? e->kchunk : auto_kchunk(e);
+    for (int di = 0, gi = 0; gi < (int)e->groups.size() && !e->umesh; ++gi)
         for (auto *d : e->groups[gi]->doms) {
             const int s1 = d->host.nx + 1;
             int ex, ey;
             tile_shape(d->host.nx, ex, ey);
+            if (ex * ey > 256 || (ex + 1) * (ey + 1) > kForcePatch) return LULESH_EINVAL;
             for (int k0 = 0; k0 < s1; k0 += kc)
                 for (int j0 = 0; j0 < s1; j0 += ey - 1)
                     for (int i0 = 0; i0 < s1; i0 += ex - 1)
@@ -224,6 +221,15 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
     std::vector<DomDev> doms;
     std::vector<GroupCtl *> gptr;
     std::vector<int> dom0;
+    // an engine steps block groups or unstructured groups, not both: the
+    // kernels' connectivity is chosen per launch
+    e->umesh = !groups.empty() && groups[0]->umesh;
+    for (auto *g : groups)
+        if (g->umesh != e->umesh) {
+            engine_destroy(e);
+            return LULESH_EINVAL;
+        }
+    if (e->umesh) e->march = false;
     for (auto *g : groups) {
         if (g->ndom > 1) e->fused = false;
         // node-state buffers as of the group's last counted cycle
@@ -391,9 +397,9 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
                                           e->fuse_update, e->store_forces, s)));
         updated = e->fuse_update;
     } else {
-        STEP(1, HIPCHK(launch_elem_force(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
+        STEP(1, HIPCHK(launch_elem_force(e->umesh, e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
         const bool gather_update = e->fused && !e->store_forces;
-        STEP(2, HIPCHK(launch_node_gather(gather_update, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+        STEP(2, HIPCHK(launch_node_gather(e->umesh, gather_update, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
         updated = gather_update;
     }
     if (e->direct) {
@@ -405,18 +411,18 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
             if (rc) return rc;
         }
         if (!updated)
-            STEP(3, HIPCHK(launch_node_update(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+            STEP(3, HIPCHK(launch_node_update(e->umesh, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
     }
     if (!e->fused) {
         STEP(6, rc = exchange_all(e, LULESH_MSG_SYNC_POS_VEL, 6, kPosVel, s));
         if (rc) return rc;
     }
-    STEP(4, HIPCHK(launch_kinematics(e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
+    STEP(4, HIPCHK(launch_kinematics(e->umesh, e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));
     if (!e->fused && !e->direct) {   // direct: the kinematics kernel filled the ghosts
         STEP(6, rc = exchange_all(e, LULESH_MSG_MONOQ, 3, kMonoQ, s));
         if (rc) return rc;
     }
-    STEP(5, HIPCHK(launch_qeos(e->d_doms, e->d_qtiles, e->n_qtiles, e->qeos_te, 256, s)));
+    STEP(5, HIPCHK(launch_qeos(e->umesh, e->d_doms, e->d_qtiles, e->n_qtiles, e->qeos_te, 256, s)));
     return LULESH_OK;
 }
 

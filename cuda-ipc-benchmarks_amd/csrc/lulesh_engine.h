@@ -50,6 +50,7 @@ struct lulesh_domain {
 
 struct lulesh_group {
     int nx = 0, tp = 1, ndom = 1;
+    bool umesh = false;         // one unstructured domain (nx = 0)
     std::vector<lulesh_domain *> doms;
     void *ctl_mem = nullptr;
     lulesh::GroupCtl *gctl = nullptr;
@@ -100,6 +101,7 @@ struct Engine {
     // node update and CommMonoQ written by the kinematics kernel instead of
     // pack/unpack launches (LULESH_NO_DIRECT=1 keeps the exchanges)
     bool direct = false;
+    bool umesh = false;         // members are unstructured meshes
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int graph_cycles = 0;
@@ -116,6 +118,7 @@ int engine_enqueue_cycle(Engine *e, hipStream_t s);
 int engine_profile(Engine *e, int ncycles, hipStream_t s, double ms[7]);
 
 int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **out);
+int group_create_mesh(const lulesh_mesh *m, int nr, int balance, int cost, lulesh_group **out);
 // tables of the direct halo steps (device_local backend only)
 int group_direct(lulesh_group *g, DirectTables **out);
 bool group_device_local(const lulesh_group *g);

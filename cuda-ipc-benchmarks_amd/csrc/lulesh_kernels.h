@@ -8,8 +8,9 @@ namespace lulesh {
 
 hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, DomDev *doms, const int *dom0,
                                  hipStream_t s);
-hipError_t launch_elem_force(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
-hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+hipError_t launch_elem_force(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+hipError_t launch_node_gather(bool umesh, bool fuse, const DomDev *d, const Tile *t, int nt, int bs,
+                              hipStream_t s);
 // fused force + ordered gather (16x16-thread column tiles, kchunk node planes)
 // node slots per plane of the force kernel's LDS patch: (ex+1)*(ey+1) must fit
 constexpr int kForcePatch = 320;
@@ -17,20 +18,21 @@ constexpr int kForcePatch = 320;
 // node plane completes; store_f: write fx..fz
 hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, bool advance,
                               bool store_f, hipStream_t s);
-hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+hipError_t launch_node_update(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 // CommSBN summed directly from the neighbours' partial forces + node update
 hipError_t launch_node_update_direct(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
-hipError_t launch_kinematics(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+hipError_t launch_kinematics(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+// umesh: the unstructured-mesh variants (DomDev::u_*)
 // te elements per tile (<= bs threads per workgroup)
-hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int te, int bs, hipStream_t s);
+hipError_t launch_qeos(bool umesh, const DomDev *d, const Tile *t, int nt, int te, int bs, hipStream_t s);
 // gate: skip the exchange once the group has stopped (NULL: always run)
 hipError_t launch_pack(const PackEnt *e, int n, const double *const *fptr, int nfields,
                        const GroupCtl *gate, hipStream_t s);
 hipError_t launch_unpack(const UnpackTgt *t, int n, const UnpackSrc *src, double *const *fptr,
                          int nfields, int mode, const GroupCtl *gate, hipStream_t s);
-// Sedov set-up of one subdomain on the device: coordinates, volo,
-// elemMass, v = 1, nodal masses
-hipError_t launch_setup(const DomDev &D, hipStream_t s);
+// Sedov set-up of one subdomain on the device: coordinates (block only),
+// volo, elemMass, v = 1, nodal masses
+hipError_t launch_setup(const DomDev &D, bool umesh, hipStream_t s);
 hipError_t launch_hg_coef(double *coef, const double *ss, const double *mass, const double *volo,
                           const double *v, int n, hipStream_t s);
 hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s);

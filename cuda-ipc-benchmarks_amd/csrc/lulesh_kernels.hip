@@ -145,6 +145,26 @@ __device__ __forceinline__ void elem_nodes(int nx, int el, int n[8], int &i, int
     n[7] = n0 + s2 + s1;
 }
 
+// node ids of element el of an unstructured mesh (two 16-byte loads)
+__device__ __forceinline__ void elem_nodes_u(const DomDev &D, int el, int n[8])
+{
+    const int4 *p = reinterpret_cast<const int4 *>(D.u_nodelist) + 2 * (size_t)el;
+    const int4 a = p[0], b = p[1];
+    n[0] = a.x; n[1] = a.y; n[2] = a.z; n[3] = a.w;
+    n[4] = b.x; n[5] = b.y; n[6] = b.z; n[7] = b.w;
+}
+
+template <bool U>
+__device__ __forceinline__ void elem_nodes_of(const DomDev &D, int el, int n[8])
+{
+    if (U) {
+        elem_nodes_u(D, el, n);
+    } else {
+        int i, j, k;
+        elem_nodes(D.nx, el, n, i, j, k);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // IntegrateStressForElems + CalcHourglassControlForElems +
 // CalcFBHourglassForceForElems (lulesh.cc:546-712, 837-1297): per element,
@@ -152,6 +172,7 @@ __device__ __forceinline__ void elem_nodes(int nx, int el, int n[8], int &i, int
 // Flanagan-Belytschko hourglass corner forces.  Both sets are stored
 // separately because the reference sums them separately at each node.
 // ---------------------------------------------------------------------------
+template <bool U>
 __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -160,8 +181,8 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
     if (group_stopped(*G)) return;
     const int el = t.start + threadIdx.x;
     if (el >= D.numElem) return;
-    int n[8], i, j, k;
-    elem_nodes(D.nx, el, n, i, j, k);
+    int n[8];
+    elem_nodes_of<U>(D, el, n);
     const size_t E = (size_t)D.numElem;
     const NodeState S = ns_old(D);
 
@@ -226,20 +247,40 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
     }
 }
 
+// symmetry-plane membership of node nd (bit 0: x = 0 plane, 1: y, 2: z).
+// On the block the planes exist on the subdomains at the mesh origin
+// (SetupSymmetryPlanes, lulesh-init.cc:487-508); an unstructured mesh lists
+// its symmetry node sets (lulesh.cu:939-965).
+template <bool U>
+__device__ __forceinline__ unsigned node_symm(const DomDev &D, int nd, int i, int j, int k)
+{
+    if (U) return D.u_nsymm[nd];
+    return (D.col == 0 && i == 0 ? 1u : 0u) | (D.row == 0 && j == 0 ? 2u : 0u) |
+           (D.plane == 0 && k == 0 ? 4u : 0u);
+}
+
+// node (i,j,k) of the block, from its index
+__device__ __forceinline__ void node_ijk(int nx, int nd, int &i, int &j, int &k)
+{
+    const int s1 = nx + 1;
+    k = nd / (s1 * s1);
+    const int rem = nd - k * s1 * s1;
+    j = rem / s1;
+    i = rem - j * s1;
+}
+
 // CalcAccelerationForNodes + ApplyAccelerationBoundaryConditionsForNodes +
 // CalcVelocityForNodes + CalcPositionForNodes (lulesh.cc:1415-1550) for
-// node (i,j,k) of subdomain D
+// node nd of subdomain D, symm from node_symm
 __device__ __forceinline__ void node_advance(const DomDev &D, const NodeState &src, const NodeState &dst,
-                                             int nd, int i, int j, int k, double fx, double fy, double fz,
+                                             int nd, unsigned symm, double fx, double fy, double fz,
                                              double dt)
 {
     const double one_over_nMass = 1. / D.nodalMass[nd];
     double ax = fx * one_over_nMass, ay = fy * one_over_nMass, az = fz * one_over_nMass;
-    // symmetry planes exist on the subdomains at the mesh origin
-    // (SetupSymmetryPlanes, lulesh-init.cc:487-508)
-    if (D.col == 0 && i == 0) ax = 0.0;
-    if (D.row == 0 && j == 0) ay = 0.0;
-    if (D.plane == 0 && k == 0) az = 0.0;
+    if (symm & 1u) ax = 0.0;
+    if (symm & 2u) ay = 0.0;
+    if (symm & 4u) az = 0.0;
     double vx = src.xd[nd] + ax * dt;
     if (fabs(vx) < Phys::u_cut) vx = 0.0;
     double vy = src.yd[nd] + ay * dt;
@@ -450,7 +491,6 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     double fs[3] = {0.0, 0.0, 0.0};                              // node plane c stress sums
     for (int c = k0 - 1; c < k1; ++c) {
         const bool el_ok = col_ok && c >= 0 && c < nx;
-        const int el = (c * nx + b) * nx + a;
         const double (*Lo)[kPatch] = sn[c & 1];
         const double (*Up)[kPatch] = sn[(c + 1) & 1];
         // loads for the next plane run while this one is computed
@@ -603,7 +643,43 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
 // ascending element order -- elements (i-1,j-1,k-1) ... (i,j,k) -- summed
 // from 0.0, stress first, then hourglass, f = stress + hourglass.
 // ---------------------------------------------------------------------------
+// An unstructured mesh lists each node's corners (8*element + corner) in
+// ascending element order (the reference's nodeElemCornerList,
+// lulesh-init.cc:243-313): the same two ordered sums over that list.
 template <bool FUSE_UPDATE>
+__device__ __forceinline__ void node_gather_u(const DomDev &D, int nd)
+{
+    const size_t E = (size_t)D.numElem;
+    const int q0 = D.u_nstart[nd], q1 = D.u_nstart[nd + 1];
+    double sx = 0.0, sy = 0.0, sz = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        const int ent = D.u_ncorner[q];
+        const size_t e = (size_t)(ent >> 3);
+        const int c = ent & 7;
+        sx += D.corner[(0 * 8 + c) * E + e];
+        sy += D.corner[(1 * 8 + c) * E + e];
+        sz += D.corner[(2 * 8 + c) * E + e];
+    }
+    for (int q = q0; q < q1; ++q) {
+        const int ent = D.u_ncorner[q];
+        const size_t e = (size_t)(ent >> 3);
+        const int c = ent & 7;
+        hx += D.corner[(3 * 8 + c) * E + e];
+        hy += D.corner[(4 * 8 + c) * E + e];
+        hz += D.corner[(5 * 8 + c) * E + e];
+    }
+    const double fx = sx + hx, fy = sy + hy, fz = sz + hz;
+    if (FUSE_UPDATE) {
+        node_advance(D, ns_old(D), ns_new(D), nd, node_symm<true>(D, nd, 0, 0, 0), fx, fy, fz,
+                     D.ctl->deltatime);
+    } else {
+        D.fpx[nd] = fx;
+        D.fpy[nd] = fy;
+        D.fpz[nd] = fz;
+    }
+}
+
+template <bool FUSE_UPDATE, bool U>
 __global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -611,11 +687,13 @@ __global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const T
     if (group_stopped(*D.grp)) return;
     const int nd = t.start + threadIdx.x;
     if (nd >= D.numNode) return;
-    const int nx = D.nx, s1 = nx + 1;
-    const int k = nd / (s1 * s1);
-    const int rem = nd - k * s1 * s1;
-    const int j = rem / s1;
-    const int i = rem - j * s1;
+    if (U) {
+        node_gather_u<FUSE_UPDATE>(D, nd);
+        return;
+    }
+    const int nx = D.nx;
+    int i, j, k;
+    node_ijk(nx, nd, i, j, k);
     const size_t E = (size_t)D.numElem;
 
     // (di, dj, dk, corner): element (i+di, j+dj, k+dk) holds this node as
@@ -652,7 +730,8 @@ __global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const T
     }
     const double fx = sx + hx, fy = sy + hy, fz = sz + hz;
     if (FUSE_UPDATE) {
-        node_advance(D, ns_old(D), ns_new(D), nd, i, j, k, fx, fy, fz, D.ctl->deltatime);
+        node_advance(D, ns_old(D), ns_new(D), nd, node_symm<false>(D, nd, i, j, k), fx, fy, fz,
+                     D.ctl->deltatime);
     } else {
         D.fpx[nd] = fx;
         D.fpy[nd] = fy;
@@ -660,6 +739,7 @@ __global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const T
     }
 }
 
+template <bool U>
 __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -667,12 +747,10 @@ __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const T
     if (group_stopped(*D.grp)) return;
     const int nd = t.start + threadIdx.x;
     if (nd >= D.numNode) return;
-    const int s1 = D.nx + 1;
-    const int k = nd / (s1 * s1);
-    const int rem = nd - k * s1 * s1;
-    const int j = rem / s1;
-    const int i = rem - j * s1;
-    node_advance(D, ns_old(D), ns_new(D), nd, i, j, k, D.fx[nd], D.fy[nd], D.fz[nd], D.ctl->deltatime);
+    int i = 0, j = 0, k = 0;
+    if (!U) node_ijk(D.nx, nd, i, j, k);
+    node_advance(D, ns_old(D), ns_new(D), nd, node_symm<U>(D, nd, i, j, k), D.fx[nd], D.fy[nd], D.fz[nd],
+                 D.ctl->deltatime);
 }
 
 // CommSBN + the node update in one pass (device_local backend): a shared
@@ -688,11 +766,8 @@ __global__ __launch_bounds__(256) void k_node_update_direct(const DomDev *doms, 
     if (group_stopped(*D.grp)) return;
     const int nd = t.start + threadIdx.x;
     if (nd >= D.numNode) return;
-    const int s1 = D.nx + 1;
-    const int k = nd / (s1 * s1);
-    const int rem = nd - k * s1 * s1;
-    const int j = rem / s1;
-    const int i = rem - j * s1;
+    int i, j, k;
+    node_ijk(D.nx, nd, i, j, k);
     double fx = D.fpx[nd], fy = D.fpy[nd], fz = D.fpz[nd];
     const IdxSpan r = D.dir_node ? D.dir_node[nd] : IdxSpan{0, 0};
     for (int m = 0; m < r.count; ++m) {
@@ -705,7 +780,8 @@ __global__ __launch_bounds__(256) void k_node_update_direct(const DomDev *doms, 
     D.fx[nd] = fx;
     D.fy[nd] = fy;
     D.fz[nd] = fz;
-    node_advance(D, ns_old(D), ns_new(D), nd, i, j, k, fx, fy, fz, D.ctl->deltatime);
+    node_advance(D, ns_old(D), ns_new(D), nd, node_symm<false>(D, nd, i, j, k), fx, fy, fz,
+                 D.ctl->deltatime);
 }
 
 // ---------------------------------------------------------------------------
@@ -810,6 +886,7 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
     }
 }
 
+template <bool U>
 __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -818,8 +895,8 @@ __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Ti
     if (group_stopped(*G)) return;
     const int el = t.start + threadIdx.x;
     if (el >= D.numElem) return;
-    int n[8], i, j, k;
-    elem_nodes(D.nx, el, n, i, j, k);
+    int n[8];
+    elem_nodes_of<U>(D, el, n);
     const double dt = D.ctl->deltatime;
 
     const NodeState S = ns_new(D);
@@ -859,6 +936,17 @@ __device__ __forceinline__ double face_nbr(const double *g, int el, bool at_edge
     if (!at_edge) return g[nbr];
     if (ghost_base >= 0) return g[ghost_base + ghost_off];
     return symm_side ? g[el] : 0.0;
+}
+
+// the same for an unstructured mesh, from the element's elemBC bits and
+// face-neighbour list (CalcMonotonicQRegionForElems' switch,
+// lulesh.cc:2307-2400): neighbour for COMM or no bit, own value for SYMM,
+// 0 for FREE and any other combination
+__device__ __forceinline__ double face_nbr_u(const double *g, int el, int nbr, int bits, int symm, int comm)
+{
+    if (bits == 0 || bits == comm) return g[nbr];
+    if (bits == symm) return g[el];
+    return 0.0;
 }
 
 __device__ __forceinline__ double ss_of(double pbvc, double e_new, double vv, double bvc,
@@ -964,6 +1052,7 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 // A tile holds te <= blockDim.x elements: the first te threads do the
 // per-element work, all threads share the EOS repetitions (small meshes
 // use te = 64 so a tile's repetitions spread over four times the lanes).
+template <bool U>
 __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *tiles, int ntiles, int te)
 {
     __shared__ double s_in[7][256];
@@ -977,22 +1066,32 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     const bool live = tid < te && el < D.numElem;
     double vdov = 0.0;
     if (live) {
-        const int nx = D.nx;
-        const int k = el / (nx * nx);
-        const int rem = el - k * nx * nx;
-        const int j = rem / nx;
-        const int i = rem - j * nx;
-
         // --- MonoQ region ---
         const double gxi = D.delv_xi[el], geta = D.delv_eta[el], gzeta = D.delv_zeta[el];
-        const double xim = face_nbr(D.delv_xi, el, i == 0, el - 1, D.ghost[4], k * nx + j, D.col == 0);
-        const double xip = face_nbr(D.delv_xi, el, i == nx - 1, el + 1, D.ghost[5], k * nx + j, false);
-        const double etam = face_nbr(D.delv_eta, el, j == 0, el - nx, D.ghost[2], k * nx + i, D.row == 0);
-        const double etap = face_nbr(D.delv_eta, el, j == nx - 1, el + nx, D.ghost[3], k * nx + i, false);
-        const double zetam = face_nbr(D.delv_zeta, el, k == 0, el - nx * nx, D.ghost[0], j * nx + i,
-                                      D.plane == 0);
-        const double zetap = face_nbr(D.delv_zeta, el, k == nx - 1, el + nx * nx, D.ghost[1], j * nx + i,
-                                      false);
+        double xim, xip, etam, etap, zetam, zetap;
+        if (U) {
+            const int4 *fp = reinterpret_cast<const int4 *>(D.u_face) + 2 * (size_t)el;
+            const int4 a = fp[0], b = fp[1];
+            const int bc = b.z;
+            xim = face_nbr_u(D.delv_xi, el, a.x, bc & kXiM, kXiMSymm, kXiMComm);
+            xip = face_nbr_u(D.delv_xi, el, a.y, bc & kXiP, kXiPSymm, kXiPComm);
+            etam = face_nbr_u(D.delv_eta, el, a.z, bc & kEtaM, kEtaMSymm, kEtaMComm);
+            etap = face_nbr_u(D.delv_eta, el, a.w, bc & kEtaP, kEtaPSymm, kEtaPComm);
+            zetam = face_nbr_u(D.delv_zeta, el, b.x, bc & kZetaM, kZetaMSymm, kZetaMComm);
+            zetap = face_nbr_u(D.delv_zeta, el, b.y, bc & kZetaP, kZetaPSymm, kZetaPComm);
+        } else {
+            const int nx = D.nx;
+            const int k = el / (nx * nx);
+            const int rem = el - k * nx * nx;
+            const int j = rem / nx;
+            const int i = rem - j * nx;
+            xim = face_nbr(D.delv_xi, el, i == 0, el - 1, D.ghost[4], k * nx + j, D.col == 0);
+            xip = face_nbr(D.delv_xi, el, i == nx - 1, el + 1, D.ghost[5], k * nx + j, false);
+            etam = face_nbr(D.delv_eta, el, j == 0, el - nx, D.ghost[2], k * nx + i, D.row == 0);
+            etap = face_nbr(D.delv_eta, el, j == nx - 1, el + nx, D.ghost[3], k * nx + i, false);
+            zetam = face_nbr(D.delv_zeta, el, k == 0, el - nx * nx, D.ghost[0], j * nx + i, D.plane == 0);
+            zetap = face_nbr(D.delv_zeta, el, k == nx - 1, el + nx * nx, D.ghost[1], j * nx + i, false);
+        }
         const double phixi = monoq_phi(gxi, xim, xip);
         const double phieta = monoq_phi(geta, etam, etap);
         const double phizeta = monoq_phi(gzeta, zetam, zetap);
@@ -1169,18 +1268,25 @@ hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, DomDev *d
     return hipGetLastError();
 }
 
-hipError_t launch_elem_force(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+hipError_t launch_elem_force(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_elem_force, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    if (umesh) hipLaunchKernelGGL(k_elem_force<true>, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    else hipLaunchKernelGGL(k_elem_force<false>, dim3(nt), dim3(bs), 0, s, d, t, nt);
     return hipGetLastError();
 }
 
-hipError_t launch_node_gather(bool fuse, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+hipError_t launch_node_gather(bool umesh, bool fuse, const DomDev *d, const Tile *t, int nt, int bs,
+                              hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
-    if (fuse) hipLaunchKernelGGL(k_node_gather<true>, dim3(nt), dim3(bs), 0, s, d, t, nt);
-    else hipLaunchKernelGGL(k_node_gather<false>, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    if (umesh) {
+        if (fuse) hipLaunchKernelGGL((k_node_gather<true, true>), dim3(nt), dim3(bs), 0, s, d, t, nt);
+        else hipLaunchKernelGGL((k_node_gather<false, true>), dim3(nt), dim3(bs), 0, s, d, t, nt);
+    } else {
+        if (fuse) hipLaunchKernelGGL((k_node_gather<true, false>), dim3(nt), dim3(bs), 0, s, d, t, nt);
+        else hipLaunchKernelGGL((k_node_gather<false, false>), dim3(nt), dim3(bs), 0, s, d, t, nt);
+    }
     return hipGetLastError();
 }
 
@@ -1193,10 +1299,11 @@ hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kch
     return hipGetLastError();
 }
 
-hipError_t launch_node_update(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+hipError_t launch_node_update(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_node_update, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    if (umesh) hipLaunchKernelGGL(k_node_update<true>, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    else hipLaunchKernelGGL(k_node_update<false>, dim3(nt), dim3(bs), 0, s, d, t, nt);
     return hipGetLastError();
 }
 
@@ -1207,18 +1314,20 @@ hipError_t launch_node_update_direct(const DomDev *d, const Tile *t, int nt, int
     return hipGetLastError();
 }
 
-hipError_t launch_kinematics(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+hipError_t launch_kinematics(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_kinematics, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    if (umesh) hipLaunchKernelGGL(k_kinematics<true>, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    else hipLaunchKernelGGL(k_kinematics<false>, dim3(nt), dim3(bs), 0, s, d, t, nt);
     return hipGetLastError();
 }
 
-hipError_t launch_qeos(const DomDev *d, const Tile *t, int nt, int te, int bs, hipStream_t s)
+hipError_t launch_qeos(bool umesh, const DomDev *d, const Tile *t, int nt, int te, int bs, hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
     if (te < 1 || te > bs || bs > 256) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_qeos, dim3(nt), dim3(bs), 0, s, d, t, nt, te);
+    if (umesh) hipLaunchKernelGGL(k_qeos<true>, dim3(nt), dim3(bs), 0, s, d, t, nt, te);
+    else hipLaunchKernelGGL(k_qeos<false>, dim3(nt), dim3(bs), 0, s, d, t, nt, te);
     return hipGetLastError();
 }
 
@@ -1257,12 +1366,13 @@ __global__ __launch_bounds__(256) void k_setup_coords(DomDev D)
 }
 
 // element volumes (lulesh-init.cc:151-162): volo = elemMass = volume, v = 1
+template <bool U>
 __global__ __launch_bounds__(256) void k_setup_volumes(DomDev D)
 {
     const int el = blockIdx.x * blockDim.x + threadIdx.x;
     if (el >= D.numElem) return;
-    int n[8], i, j, k;
-    elem_nodes(D.nx, el, n, i, j, k);
+    int n[8];
+    elem_nodes_of<U>(D, el, n);
     double x[8], y[8], z[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) { x[c] = D.ns[0].x[n[c]]; y[c] = D.ns[0].y[n[c]]; z[c] = D.ns[0].z[n[c]]; }
@@ -1275,10 +1385,20 @@ __global__ __launch_bounds__(256) void k_setup_volumes(DomDev D)
 // nodal masses (lulesh-init.cc:163-167): the reference adds volume/8 of
 // each element, elements in ascending order, onto a node mass that starts
 // at 0.0; per node that is the ordered sum over its elements
+template <bool U>
 __global__ __launch_bounds__(256) void k_setup_nodal_mass(DomDev D)
 {
     const int nd = blockIdx.x * blockDim.x + threadIdx.x;
     if (nd >= D.numNode) return;
+    if (U) {
+        // an unstructured mesh: the node's elements, ascending, from its
+        // corner list (a node twice in one element counts twice, as the
+        // reference's per-corner loop adds it twice)
+        double m = 0.0;
+        for (int q = D.u_nstart[nd]; q < D.u_nstart[nd + 1]; ++q) m += D.volo[D.u_ncorner[q] >> 3] / 8.0;
+        D.nodalMass[nd] = m;
+        return;
+    }
     const int nx = D.nx, s1 = nx + 1;
     const int k = nd / (s1 * s1), rem = nd - k * s1 * s1, j = rem / s1, i = rem - j * s1;
     double m = 0.0;
@@ -1291,11 +1411,17 @@ __global__ __launch_bounds__(256) void k_setup_nodal_mass(DomDev D)
     D.nodalMass[nd] = m;
 }
 
-hipError_t launch_setup(const DomDev &D, hipStream_t s)
+hipError_t launch_setup(const DomDev &D, bool umesh, hipStream_t s)
 {
+    if (umesh) {
+        // coordinates come from the mesh file
+        hipLaunchKernelGGL(k_setup_volumes<true>, dim3(grid_of(D.numElem, 256)), dim3(256), 0, s, D);
+        hipLaunchKernelGGL(k_setup_nodal_mass<true>, dim3(grid_of(D.numNode, 256)), dim3(256), 0, s, D);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_setup_coords, dim3(grid_of(D.numNode, 256)), dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_setup_volumes, dim3(grid_of(D.numElem, 256)), dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_setup_nodal_mass, dim3(grid_of(D.numNode, 256)), dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_setup_volumes<false>, dim3(grid_of(D.numElem, 256)), dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_setup_nodal_mass<false>, dim3(grid_of(D.numNode, 256)), dim3(256), 0, s, D);
     return hipGetLastError();
 }
 

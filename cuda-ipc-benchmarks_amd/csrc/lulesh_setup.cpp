@@ -8,7 +8,10 @@
 #include "lulesh_setup.h"
 
 #include <math.h>
+#include <limits.h>
 #include <stdlib.h>
+
+#include "lulesh_mesh.h"
 
 namespace lulesh {
 
@@ -110,6 +113,13 @@ bool problem_ok(int nx, int tp, int nr, int balance, int cost)
     return ne * max_elem_rep(nr, cost) <= 2147483647LL;
 }
 
+bool mesh_problem_ok(int numElem, int nr, int balance, int cost)
+{
+    if (numElem < 1 || nr < 1 || balance < 0 || cost < 0) return false;
+    if ((long long)numElem * 8 > INT_MAX) return false;
+    return (long long)numElem * max_elem_rep(nr, cost) <= INT_MAX;
+}
+
 // The EOS work list: element i appears elemRep[i] times, in element order,
 // its last appearance flagged.  A workgroup of the EOS kernel deals the
 // entries of its tile round-robin over its lanes, so every lane does the
@@ -202,6 +212,59 @@ void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int bala
     const double einit = ebase * scale * scale * scale;
     h.e0 = (h.row + h.col + h.plane == 0) ? einit : 0.0;
     h.deltatime0 = (.5 * cbrt(h.volo0)) / sqrt(2.0 * einit);
+}
+
+void build_host_domain_mesh(HostDomain &h, const lulesh_mesh &m, int nr, int balance, int cost)
+{
+    h.umesh = true;
+    h.nx = 0;
+    h.tp = 1;
+    h.rank = h.col = h.row = h.plane = 0;
+    h.numElem = m.numElem;
+    h.numNode = m.numNode;
+    h.allElem = m.numElem;   // one domain: no ghost planes, no comm buffers
+    create_regions(h, nr, balance, cost);
+    build_eval_table(h);
+
+    const int ne = m.numElem, nn = m.numNode;
+    h.ux = m.x;
+    h.uy = m.y;
+    h.uz = m.z;
+    h.u_nodelist = m.nodelist;
+    h.u_face.assign(8 * (size_t)ne, 0);
+    for (int i = 0; i < ne; ++i) {
+        for (int f = 0; f < 6; ++f) h.u_face[8 * (size_t)i + f] = m.nbr[6 * (size_t)i + f];
+        h.u_face[8 * (size_t)i + 6] = m.elemBC[i];
+    }
+    // node -> corner lists, elements ascending (the order the reference's
+    // corner list has, lulesh-init.cc:243-313)
+    h.u_nstart.assign((size_t)nn + 1, 0);
+    for (int v : m.nodelist) ++h.u_nstart[(size_t)v + 1];
+    for (int n = 0; n < nn; ++n) h.u_nstart[n + 1] += h.u_nstart[n];
+    h.u_ncorner.assign(8 * (size_t)ne, 0);
+    std::vector<int> fill(h.u_nstart.begin(), h.u_nstart.end() - 1);
+    for (int i = 0; i < ne; ++i)
+        for (int j = 0; j < 8; ++j) {
+            const int n = m.nodelist[8 * (size_t)i + j];
+            h.u_ncorner[fill[n]++] = 8 * i + j;
+        }
+    h.u_nsymm.assign((size_t)nn, 0);
+    for (int a = 0; a < 3; ++a)
+        for (int n : m.symm[a]) h.u_nsymm[n] |= (unsigned char)(1u << a);
+
+    // element 0's volume, the deposit at the octant corner, and dt0 by the
+    // analytic CFL estimate of lulesh.cu:1205 with the deposited energy
+    double xl[8], yl[8], zl[8];
+    for (int c = 0; c < 8; ++c) {
+        const int n = m.nodelist[c];
+        xl[c] = m.x[n];
+        yl[c] = m.y[n];
+        zl[c] = m.z[n];
+    }
+    h.volo0 = host_elem_volume(xl, yl, zl);
+    h.e0 = m.e_deposit;
+    h.e0_index = m.octantCorner;
+    h.deltatime0 = (.5 * cbrt(h.volo0)) / sqrt(2.0 * m.e_deposit);
 }
 
 size_t domain_device_bytes(int nx, bool corner_array, long long evtotal)

@@ -4,6 +4,8 @@
 
 #include <vector>
 
+struct lulesh_mesh;
+
 namespace lulesh {
 
 struct HostDomain {
@@ -20,16 +22,27 @@ struct HostDomain {
     std::vector<long long> evprefix64;
     long long evtotal = 0;
     double e0 = 0.0, deltatime0 = 0.0, volo0 = 0.0;
+    int e0_index = 0;   // the element that starts with energy e0
+    // unstructured mesh: coordinates and connectivity tables in the layout
+    // of DomDev::u_* (uploaded, then released)
+    bool umesh = false;
+    std::vector<double> ux, uy, uz;
+    std::vector<int> u_nodelist, u_face, u_nstart, u_ncorner;
+    std::vector<unsigned char> u_nsymm;
 };
 
 // decomposition, comm-buffer sizes, region layout (glibc rand()), e0 and
 // dt0; the mesh, volumes and nodal masses are built on the device
 void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost);
+// one unstructured domain: regions, EOS table, connectivity tables, e0 at
+// the octant corner and dt0
+void build_host_domain_mesh(HostDomain &h, const lulesh_mesh &m, int nr, int balance, int cost);
 size_t domain_device_bytes(int nx, bool corner_array, long long evtotal);
 // largest elemRep the region layout of (nr, cost) can assign
 long long max_elem_rep(int nr, int cost);
 // problem parameters the library accepts: 32-bit element and node
 // indexing, at most 4x4x4 subdomains, an EOS table under 2^31 entries
 bool problem_ok(int nx, int tp, int nr, int balance, int cost);
+bool mesh_problem_ok(int numElem, int nr, int balance, int cost);
 
 }  // namespace lulesh

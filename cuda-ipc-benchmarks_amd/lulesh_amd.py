@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liblulesh_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "lulesh_amd.h")
 
 OK, VOLUME_ERROR, QSTOP_ERROR, EINVAL, EHIP, ENOMEM, EDIRECTION, EBUSY = 0, -1, -2, -3, -4, -5, -6, -7
+EIO = -8
 MSG_COMM_SBN, MSG_SYNC_POS_VEL, MSG_MONOQ = 1024, 2048, 3072
 
 FIELDS = {
@@ -34,6 +35,15 @@ SIGNATURES = {
     "lulesh_status_string": (ctypes.c_char_p, [_i]),
     "lulesh_version": (ctypes.c_char_p, []),
     "lulesh_group_create": (_i, [_i, _i, _i, _i, _i, _pvp]),
+    "lulesh_mesh_read": (_i, [ctypes.c_char_p, _pvp]),
+    "lulesh_mesh_create": (_i, [_i, _i, _dp, _dp, _dp, _ip, _ip, _ip, _i, _ip, _i, _ip, _i, _ip, _i, _pvp]),
+    "lulesh_mesh_destroy": (_i, [_vp]),
+    "lulesh_mesh_sizes": (_i, [_vp, _ip, _ip]),
+    "lulesh_mesh_get_elem_bc": (_i, [_vp, _ip, _sz]),
+    "lulesh_mesh_set_elem_bc": (_i, [_vp, _ip, _sz]),
+    "lulesh_mesh_octant_corner": (_i, [_vp]),
+    "lulesh_mesh_set_energy": (_i, [_vp, _d]),
+    "lulesh_group_create_mesh": (_i, [_vp, _i, _i, _i, _pvp]),
     "lulesh_group_destroy": (_i, [_vp]),
     "lulesh_group_num_domains": (_i, [_vp]),
     "lulesh_group_domain": (_i, [_vp, _i, _pvp]),
@@ -210,8 +220,77 @@ class Domain:
         return out
 
 
+class Mesh:
+    """An unstructured hexahedral mesh on the host (the reference's .lmesh
+    input).  Needs no GPU: reading, checking and the boundary derivation
+    run on the host."""
+
+    def __init__(self, handle):
+        self.h = handle
+        ne, nn = ctypes.c_int(), ctypes.c_int()
+        _chk(load().lulesh_mesh_sizes(handle, ctypes.byref(ne), ctypes.byref(nn)), "mesh_sizes")
+        self.num_elem, self.num_node = ne.value, nn.value
+
+    @classmethod
+    def read(cls, path):
+        h = ctypes.c_void_p()
+        _chk(load().lulesh_mesh_read(str(path).encode(), ctypes.byref(h)), f"mesh_read {path}")
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, m):
+        """m: dict with num_elem, num_node, x, y, z, nodelist (ne x 8), nbr
+        (ne x 6), symm (three node lists) and free (node list)."""
+        a = {k: np.ascontiguousarray(m[k], dtype=np.float64) for k in ("x", "y", "z")}
+        nl = np.ascontiguousarray(m["nodelist"], dtype=np.int32)
+        nb = np.ascontiguousarray(m["nbr"], dtype=np.int32)
+        sx, sy, sz = (np.ascontiguousarray(v, dtype=np.int32) for v in m["symm"])
+        fr = np.ascontiguousarray(m["free"], dtype=np.int32)
+        ip = lambda v: _p(v, ctypes.c_int)
+        h = ctypes.c_void_p()
+        _chk(load().lulesh_mesh_create(int(m["num_elem"]), int(m["num_node"]), _p(a["x"]), _p(a["y"]),
+                                       _p(a["z"]), ip(nl), ip(nb), ip(sx), sx.size, ip(sy), sy.size,
+                                       ip(sz), sz.size, ip(fr), fr.size, ctypes.byref(h)), "mesh_create")
+        return cls(h.value)
+
+    def close(self):
+        if self.h:
+            load().lulesh_mesh_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def elem_bc(self):
+        out = np.empty(self.num_elem, dtype=np.int32)
+        _chk(load().lulesh_mesh_get_elem_bc(self.h, _p(out, ctypes.c_int), out.size), "elem_bc")
+        return out
+
+    def set_elem_bc(self, bc):
+        a = np.ascontiguousarray(bc, dtype=np.int32)
+        _chk(load().lulesh_mesh_set_elem_bc(self.h, _p(a, ctypes.c_int), a.size), "set_elem_bc")
+
+    @property
+    def octant_corner(self):
+        return load().lulesh_mesh_octant_corner(self.h)
+
+    def set_energy(self, e):
+        _chk(load().lulesh_mesh_set_energy(self.h, float(e)), "set_energy")
+
+
 class Group:
-    """tp^3 subdomains of nx^3 elements on the GPU (tp=1: the plain problem)."""
+    """tp^3 subdomains of nx^3 elements on the GPU (tp=1: the plain problem),
+    or one unstructured domain (Group.from_mesh)."""
+
+    @classmethod
+    def from_mesh(cls, mesh, nr=11, balance=1, cost=1):
+        init()
+        h = ctypes.c_void_p()
+        _chk(load().lulesh_group_create_mesh(mesh.h, nr, balance, cost, ctypes.byref(h)), "group_create_mesh")
+        return cls(0, 1, handle=h.value)
 
     def __init__(self, nx, tp=1, nr=11, balance=1, cost=1, handle=None, owned=True):
         init()

@@ -33,6 +33,7 @@ typedef struct lulesh_group  lulesh_group;   /* one decomposed problem        */
 typedef struct lulesh_domain lulesh_domain;  /* one subdomain of a group      */
 typedef struct lulesh_batch  lulesh_batch;   /* ensemble of independent groups */
 typedef struct lulesh_server lulesh_server;  /* continuous-batching server     */
+typedef struct lulesh_mesh   lulesh_mesh;    /* host-side unstructured mesh   */
 
 /* Status codes.  VOLUME / QSTOP keep the reference's exit values
  * (LULESH/omp_4.0/lulesh.h:41, enum { VolumeError = -1, QStopError = -2 }). */
@@ -46,7 +47,8 @@ enum lulesh_status {
     LULESH_EDIRECTION   = -6,   /* no message expected from a direction
                                    (shmRecvOffset abort, cuda
                                    comm/comm_backend.h:134-137)      */
-    LULESH_EBUSY        = -7    /* server queue full                  */
+    LULESH_EBUSY        = -7,   /* server queue full                  */
+    LULESH_EIO          = -8    /* mesh file unreadable or truncated  */
 };
 
 /* Field selector; replaces the reference's Domain_member pointer-to-member
@@ -90,6 +92,52 @@ const char *lulesh_version(void);
 int lulesh_group_create(int nx, int tp, int nr, int balance, int cost,
                         lulesh_group **out);
 int lulesh_group_destroy(lulesh_group *g);
+
+/* Unstructured hexahedral meshes: the reference's -u <file.lmesh> input
+ * (LULESH/cuda/src/lulesh.cu:855-1052), one domain only, as there.
+ *
+ * lulesh_mesh_read parses the text format of lulesh.cu:855-976: numElem
+ * numNode; numNode coordinate triples; numElem x 8 node ids (corners in
+ * the reference's hexahedron order); numElem x 6 face neighbours (xi-,
+ * xi+, eta-, eta+, zeta-, zeta+; a boundary face names the element
+ * itself); then four counted node lists: symmetry X, symmetry Y, symmetry
+ * Z, free surface.  elemBC and the octant corner are derived as
+ * lulesh.cu:978-1045 derives them: a face whose four nodes all lie in the
+ * union of the symmetry sets gets its *_SYMM bit, the first element whose
+ * bits are then exactly XI_M|ETA_M|ZETA_M symmetric is the octant corner,
+ * and a face whose four nodes are all free-surface nodes also gets its
+ * *_SYMM bit (the reference's choice, kept).  LULESH_EIO: unreadable or
+ * truncated file; LULESH_EINVAL: an index out of range.
+ * lulesh_mesh_create takes the same content from arrays
+ * (nodelist [numElem][8], nbr [numElem][6]). */
+int lulesh_mesh_read(const char *path, lulesh_mesh **out);
+int lulesh_mesh_create(int numElem, int numNode, const double *x, const double *y, const double *z,
+                       const int *nodelist, const int *nbr, const int *symmX, int nSymmX,
+                       const int *symmY, int nSymmY, const int *symmZ, int nSymmZ,
+                       const int *freeSurf, int nFree, lulesh_mesh **out);
+int lulesh_mesh_destroy(lulesh_mesh *m);
+int lulesh_mesh_sizes(const lulesh_mesh *m, int *numElem, int *numNode);
+/* elemBC per element (bits of LULESH/omp_4.0/lulesh.h:60-88); the setter
+ * replaces the derived bits. */
+int lulesh_mesh_get_elem_bc(const lulesh_mesh *m, int *out, size_t n);
+int lulesh_mesh_set_elem_bc(lulesh_mesh *m, const int *bc, size_t n);
+int lulesh_mesh_octant_corner(const lulesh_mesh *m);
+/* Energy deposited in the octant-corner element: 3.948746e+7 as read
+ * (lulesh.cu:1049-1050). */
+int lulesh_mesh_set_energy(lulesh_mesh *m, double e);
+/* A one-domain group on a mesh (nr, balance, cost as lulesh_group_create;
+ * the mesh may be destroyed afterwards).  The deposit stays in the octant
+ * corner and dt0 = 0.5 cbrt(volo[0]) / sqrt(2 e_deposit), the analytic CFL
+ * estimate of lulesh.cu:1205.  (The reference's own start-up then
+ * overwrites e[0] with an energy scaled by the structured edge length,
+ * which is atoi(<file name>) = 0 on this path: no energy and an infinite
+ * dt0.)  On such a group the comm calls and exchanges fail with
+ * LULESH_EINVAL (no subdomains: SetupCommBuffers has no unstructured form,
+ * LULESH/cuda/README.md:111-113), lulesh_domain_sizes reports nx = 0 and
+ * allElem = numElem, and lulesh_group_verify reports the octant corner's
+ * energy with zero symmetry figures (the reference prints none). */
+int lulesh_group_create_mesh(const lulesh_mesh *m, int nr, int balance, int cost, lulesh_group **out);
+
 int lulesh_group_num_domains(const lulesh_group *g);
 int lulesh_group_domain(lulesh_group *g, int rank, lulesh_domain **out);
 

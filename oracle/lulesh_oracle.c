@@ -70,6 +70,9 @@ typedef struct {
 
     double *x, *y, *z, *xd, *yd, *zd, *xdd, *ydd, *zdd, *fx, *fy, *fz, *nodalMass;
     int *symmX, *symmY, *symmZ;            /* NULL when empty */
+    int numSymmX, numSymmY, numSymmZ;
+    int umesh;          /* unstructured mesh (explicit connectivity, no ghosts) */
+    int octantCorner;   /* element holding the deposited energy */
 
     int *nodelist, *lxim, *lxip, *letam, *letap, *lzetam, *lzetap, *elemBC;
     double *e, *p, *q, *ql, *qq, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass;
@@ -414,12 +417,12 @@ static void node_update(lo_group *g, lo_dom *d)
         d->ydd[i] = d->fy[i] * one_over_nMass;
         d->zdd[i] = d->fz[i] * one_over_nMass;
     }
-    int nbc = (d->nx + 1) * (d->nx + 1);
-    for (int i = 0; i < nbc; ++i) {
-        if (d->symmX) d->xdd[d->symmX[i]] = 0.0;
-        if (d->symmY) d->ydd[d->symmY[i]] = 0.0;
-        if (d->symmZ) d->zdd[d->symmZ[i]] = 0.0;
-    }
+    /* ApplyAccelerationBoundaryConditionsForNodes, lulesh.cc:1446-1480 (the
+     * structured sets hold (nx+1)^2 nodes; an unstructured mesh lists its own,
+     * LULESH/cuda/src/lulesh.cu:939-965, 2957-2996) */
+    for (int i = 0; i < d->numSymmX; ++i) d->xdd[d->symmX[i]] = 0.0;
+    for (int i = 0; i < d->numSymmY; ++i) d->ydd[d->symmY[i]] = 0.0;
+    for (int i = 0; i < d->numSymmZ; ++i) d->zdd[d->symmZ[i]] = 0.0;
     for (int i = 0; i < d->numNode; ++i) {
         double t;
         t = d->xd[i] + d->xdd[i] * dt; if (fabs(t) < u_cut) t = 0.0; d->xd[i] = t;
@@ -1069,17 +1072,10 @@ static void setup_bc(lo_dom *d)
     }
 }
 
-static void init_dom(lo_group *g, lo_dom *d, int rank, int nr, int balance, int cost)
+/* field storage of one subdomain: numElem, numNode and allElem set */
+static void alloc_fields(lo_dom *d)
 {
-    int nx = g->nx, tp = g->tp;
-    d->rank = rank; d->tp = tp; d->nx = nx;
-    /* InitMeshDecomp, lulesh-init.cc:635-712 (one domain per rank) */
-    d->col = rank % tp; d->row = (rank / tp) % tp; d->plane = rank / (tp * tp);
-    d->numElem = nx * nx * nx;
-    d->numNode = (nx + 1) * (nx + 1) * (nx + 1);
-    d->allElem = d->numElem + 6 * nx * nx;
     int ne = d->numElem, nn = d->numNode;
-
     d->x = dalloc(nn); d->y = dalloc(nn); d->z = dalloc(nn);
     d->xd = dalloc(nn); d->yd = dalloc(nn); d->zd = dalloc(nn);
     d->xdd = dalloc(nn); d->ydd = dalloc(nn); d->zdd = dalloc(nn);
@@ -1098,6 +1094,33 @@ static void init_dom(lo_group *g, lo_dom *d, int rank, int nr, int balance, int 
     d->fx_elem = dalloc(8 * (size_t)ne); d->fy_elem = dalloc(8 * (size_t)ne); d->fz_elem = dalloc(8 * (size_t)ne);
     d->dvdx = dalloc(8 * (size_t)ne); d->dvdy = dalloc(8 * (size_t)ne); d->dvdz = dalloc(8 * (size_t)ne);
     d->x8n = dalloc(8 * (size_t)ne); d->y8n = dalloc(8 * (size_t)ne); d->z8n = dalloc(8 * (size_t)ne);
+}
+
+/* initial volumes and masses, lulesh-init.cc:151-168 */
+static void init_volumes(lo_dom *d)
+{
+    for (int i = 0; i < d->numElem; ++i) {
+        double xl[8], yl[8], zl[8];
+        const int *nl = &d->nodelist[8 * i];
+        gather8(nl, d->x, xl); gather8(nl, d->y, yl); gather8(nl, d->z, zl);
+        double volume = elem_volume(xl, yl, zl);
+        d->volo[i] = volume;
+        d->elemMass[i] = volume;
+        for (int j = 0; j < 8; ++j) d->nodalMass[nl[j]] += volume / 8.0;
+    }
+}
+
+static void init_dom(lo_group *g, lo_dom *d, int rank, int nr, int balance, int cost)
+{
+    int nx = g->nx, tp = g->tp;
+    d->rank = rank; d->tp = tp; d->nx = nx;
+    /* InitMeshDecomp, lulesh-init.cc:635-712 (one domain per rank) */
+    d->col = rank % tp; d->row = (rank / tp) % tp; d->plane = rank / (tp * tp);
+    d->numElem = nx * nx * nx;
+    d->numNode = (nx + 1) * (nx + 1) * (nx + 1);
+    d->allElem = d->numElem + 6 * nx * nx;
+    int ne = d->numElem;
+    alloc_fields(d);
 
     /* Domain::SetupCommBuffers, lulesh-init.cc:317-372 */
     int maxEdge = nx + 1;
@@ -1120,9 +1143,9 @@ static void init_dom(lo_group *g, lo_dom *d, int rank, int nr, int balance, int 
                   PAD_REAL;
     d->comBufSize = comBufSize;
     d->commDataRecv = dalloc(comBufSize);
-    if (d->col == 0) d->symmX = ialloc((size_t)(nx + 1) * (nx + 1));
-    if (d->row == 0) d->symmY = ialloc((size_t)(nx + 1) * (nx + 1));
-    if (d->plane == 0) d->symmZ = ialloc((size_t)(nx + 1) * (nx + 1));
+    if (d->col == 0) { d->symmX = ialloc((size_t)(nx + 1) * (nx + 1)); d->numSymmX = (nx + 1) * (nx + 1); }
+    if (d->row == 0) { d->symmY = ialloc((size_t)(nx + 1) * (nx + 1)); d->numSymmY = (nx + 1) * (nx + 1); }
+    if (d->plane == 0) { d->symmZ = ialloc((size_t)(nx + 1) * (nx + 1)); d->numSymmZ = (nx + 1) * (nx + 1); }
 
     /* Domain::Domain field initialisation, lulesh-init.cc:71-101 */
     for (int i = 0; i < ne; ++i) d->v[i] = 1.0;
@@ -1132,16 +1155,7 @@ static void init_dom(lo_group *g, lo_dom *d, int rank, int nr, int balance, int 
     create_regions(d, nr, balance, cost);
     setup_bc(d);
 
-    /* initial volumes and masses, lulesh-init.cc:151-168 */
-    for (int i = 0; i < ne; ++i) {
-        double xl[8], yl[8], zl[8];
-        const int *nl = &d->nodelist[8 * i];
-        gather8(nl, d->x, xl); gather8(nl, d->y, yl); gather8(nl, d->z, zl);
-        double volume = elem_volume(xl, yl, zl);
-        d->volo[i] = volume;
-        d->elemMass[i] = volume;
-        for (int j = 0; j < 8; ++j) d->nodalMass[nl[j]] += volume / 8.0;
-    }
+    init_volumes(d);
     /* initial energy, lulesh-init.cc:170-180 */
     const double ebase = 3.948746e+7;
     double scale = (nx * tp) / 45.0;
@@ -1154,11 +1168,8 @@ static void init_dom(lo_group *g, lo_dom *d, int rank, int nr, int balance, int 
     d->dthydro = 1.0e+20;
 }
 
-lo_group *lo_create(int nx, int tp, int nr, int balance, int cost, int cbrt_mode)
+static void group_constants(lo_group *g)
 {
-    if (nx < 1 || tp < 1 || nr < 1) return NULL;
-    lo_group *g = (lo_group *)calloc(1, sizeof(lo_group));
-    g->nx = nx; g->tp = tp; g->ndom = tp * tp * tp; g->cbrt_mode = cbrt_mode;
     g->e_cut = 1.0e-7; g->p_cut = 1.0e-7; g->q_cut = 1.0e-7; g->v_cut = 1.0e-10; g->u_cut = 1.0e-7;
     g->hgcoef = 3.0; g->ss4o3 = 4.0 / 3.0; g->qstop = 1.0e+12; g->monoq_max_slope = 1.0;
     g->monoq_limiter_mult = 2.0; g->qlc_monoq = 0.5; g->qqc_monoq = 2.0 / 3.0; g->qqc = 2.0;
@@ -1167,6 +1178,14 @@ lo_group *lo_create(int nx, int tp, int nr, int balance, int cost, int cbrt_mode
     /* lulesh-init.cc:138-148 */
     g->dtfixed = -1.0e-6; g->stoptime = 1.0e-2; g->deltatimemultlb = 1.1;
     g->deltatimemultub = 1.2; g->dtmax = 1.0e-2;
+}
+
+lo_group *lo_create(int nx, int tp, int nr, int balance, int cost, int cbrt_mode)
+{
+    if (nx < 1 || tp < 1 || nr < 1) return NULL;
+    lo_group *g = (lo_group *)calloc(1, sizeof(lo_group));
+    g->nx = nx; g->tp = tp; g->ndom = tp * tp * tp; g->cbrt_mode = cbrt_mode;
+    group_constants(g);
     g->d = (lo_dom *)calloc(g->ndom, sizeof(lo_dom));
     for (int r = 0; r < g->ndom; ++r) init_dom(g, &g->d[r], r, nr, balance, cost);
     /* initial nodalMass exchange, lulesh.cc:3147-3161 */
@@ -1174,6 +1193,124 @@ lo_group *lo_create(int nx, int tp, int nr, int balance, int cost, int cbrt_mode
     lo_exchange(g, MSG_COMM_SBN, 1, &fm);
     return g;
 }
+
+/* ======================= unstructured mesh input ======================= */
+
+/* elemBC of an unstructured mesh, LULESH/cuda/src/lulesh.cu:978-1045: a
+ * face gets the *_SYMM bit when its four nodes are all marked -- first with
+ * the union of the three symmetry node sets (octantCorner = the first
+ * element whose mask is then exactly XI_M|ETA_M|ZETA_M symmetric), then
+ * with the free-surface set, whose faces the reference also marks *_SYMM. */
+static void mesh_face_bits(const int *nodelist, int ne, const char *mark, int *elemBC)
+{
+    for (int i = 0; i < ne; ++i) {
+        int mask = 0;
+        for (int j = 0; j < 8; ++j) mask |= (mark[nodelist[8 * i + j]] ? 1 : 0) << j;
+        if ((mask & 0x0f) == 0x0f) elemBC[i] |= ZETA_M_SYMM;
+        if ((mask & 0xf0) == 0xf0) elemBC[i] |= ZETA_P_SYMM;
+        if ((mask & 0x33) == 0x33) elemBC[i] |= ETA_M_SYMM;
+        if ((mask & 0xcc) == 0xcc) elemBC[i] |= ETA_P_SYMM;
+        if ((mask & 0x99) == 0x99) elemBC[i] |= XI_M_SYMM;
+        if ((mask & 0x66) == 0x66) elemBC[i] |= XI_P_SYMM;
+    }
+}
+
+static int in_range(const int *a, long n, int hi)
+{
+    for (long i = 0; i < n; ++i)
+        if (a[i] < 0 || a[i] >= hi) return 0;
+    return 1;
+}
+
+lo_group *lo_create_mesh(int numElem, int numNode, const double *x, const double *y, const double *z,
+                         const int *nodelist, const int *nbr, const int *symmX, int nSymmX,
+                         const int *symmY, int nSymmY, const int *symmZ, int nSymmZ,
+                         const int *freeSurf, int nFree, const int *elemBC, double e_deposit,
+                         int nr, int balance, int cost, int cbrt_mode)
+{
+    if (numElem < 1 || numNode < 1 || nr < 1 || nSymmX < 0 || nSymmY < 0 || nSymmZ < 0 || nFree < 0)
+        return NULL;
+    if (!in_range(nodelist, 8L * numElem, numNode) || !in_range(nbr, 6L * numElem, numElem) ||
+        !in_range(symmX, nSymmX, numNode) || !in_range(symmY, nSymmY, numNode) ||
+        !in_range(symmZ, nSymmZ, numNode) || !in_range(freeSurf, nFree, numNode))
+        return NULL;
+    lo_group *g = (lo_group *)calloc(1, sizeof(lo_group));
+    g->nx = 0; g->tp = 1; g->ndom = 1; g->cbrt_mode = cbrt_mode;
+    group_constants(g);
+    g->d = (lo_dom *)calloc(1, sizeof(lo_dom));
+    lo_dom *d = &g->d[0];
+    d->tp = 1;
+    d->umesh = 1;
+    d->numElem = numElem;
+    d->numNode = numNode;
+    d->allElem = numElem;          /* no ghost planes: one domain only */
+    alloc_fields(d);
+    d->commDataRecv = dalloc(0);
+    memcpy(d->x, x, sizeof(double) * (size_t)numNode);
+    memcpy(d->y, y, sizeof(double) * (size_t)numNode);
+    memcpy(d->z, z, sizeof(double) * (size_t)numNode);
+    memcpy(d->nodelist, nodelist, sizeof(int) * 8 * (size_t)numElem);
+    for (int i = 0; i < numElem; ++i) {
+        d->lxim[i] = nbr[6 * i + 0]; d->lxip[i] = nbr[6 * i + 1];
+        d->letam[i] = nbr[6 * i + 2]; d->letap[i] = nbr[6 * i + 3];
+        d->lzetam[i] = nbr[6 * i + 4]; d->lzetap[i] = nbr[6 * i + 5];
+    }
+    int *sets[3] = {NULL, NULL, NULL};
+    const int *src[3] = {symmX, symmY, symmZ};
+    const int cnt[3] = {nSymmX, nSymmY, nSymmZ};
+    for (int a = 0; a < 3; ++a)
+        if (cnt[a] > 0) {
+            sets[a] = ialloc(cnt[a]);
+            memcpy(sets[a], src[a], sizeof(int) * (size_t)cnt[a]);
+        }
+    d->symmX = sets[0]; d->symmY = sets[1]; d->symmZ = sets[2];
+    d->numSymmX = nSymmX; d->numSymmY = nSymmY; d->numSymmZ = nSymmZ;
+
+    /* boundary bits and the octant corner (lulesh.cu:978-1045) */
+    char *mark = (char *)calloc((size_t)numNode, 1);
+    for (int a = 0; a < 3; ++a)
+        for (int i = 0; i < cnt[a]; ++i) mark[src[a][i]] = 1;
+    int *bc = ialloc(numElem);
+    mesh_face_bits(nodelist, numElem, mark, bc);
+    d->octantCorner = 0;
+    for (int i = 0; i < numElem; ++i)
+        if (bc[i] == (XI_M_SYMM | ETA_M_SYMM | ZETA_M_SYMM)) { d->octantCorner = i; break; }
+    memset(mark, 0, (size_t)numNode);
+    for (int i = 0; i < nFree; ++i) mark[freeSurf[i]] = 1;
+    mesh_face_bits(nodelist, numElem, mark, bc);
+    free(mark);
+    /* a caller-given elemBC replaces the derived one (the octant corner
+     * stays the derived one) */
+    memcpy(d->elemBC, elemBC ? elemBC : bc, sizeof(int) * (size_t)numElem);
+    free(bc);
+
+    for (int i = 0; i < numElem; ++i) d->v[i] = 1.0;
+    setup_node_elem(d);
+    create_regions(d, nr, balance, cost);
+    init_volumes(d);
+    /* The deposit stays where the reader put it (lulesh.cu:1049-1050), and
+     * dt0 is the analytic CFL estimate of lulesh.cu:1205 with that energy.
+     * (The reference then overwrites e[0] with an einit derived from the
+     * structured edge length, which is atoi(<file name>) = 0 on this path,
+     * leaving no energy and an infinite dt0.) */
+    d->e[d->octantCorner] = e_deposit;
+    d->deltatime = (.5 * cbrt(d->volo[0])) / sqrt(2.0 * e_deposit);
+    d->time = 0.;
+    d->cycle = 0;
+    d->dtcourant = 1.0e+20;
+    d->dthydro = 1.0e+20;
+    return g;
+}
+
+int lo_get_elem_bc(const lo_group *g, int dom, int *out)
+{
+    if (dom < 0 || dom >= g->ndom) return -1;
+    memcpy(out, g->d[dom].elemBC, sizeof(int) * (size_t)g->d[dom].numElem);
+    return g->d[dom].numElem;
+}
+
+int lo_octant_corner(const lo_group *g) { return g->d[0].octantCorner; }
+int lo_all_elem(const lo_group *g) { return g->d[0].allElem; }
 
 void lo_destroy(lo_group *g)
 {
@@ -1343,7 +1480,7 @@ void lo_verify(const lo_group *g, int dom, double *origin_energy, double *max_ab
                double *total_abs_diff, double *max_rel_diff)
 {
     const lo_dom *d = &g->d[dom];
-    int nx = g->nx;
+    int nx = g->nx;   /* 0 for an unstructured mesh: no plane-0 test */
     double MaxAbsDiff = 0.0, TotalAbsDiff = 0.0, MaxRelDiff = 0.0;
     for (int j = 0; j < nx; ++j)
         for (int k = j + 1; k < nx; ++k) {
@@ -1353,7 +1490,7 @@ void lo_verify(const lo_group *g, int dom, double *origin_energy, double *max_ab
             double RelDiff = AbsDiff / d->e[k * nx + j];
             if (MaxRelDiff < RelDiff) MaxRelDiff = RelDiff;
         }
-    *origin_energy = d->e[0];
+    *origin_energy = d->e[d->octantCorner];
     *max_abs_diff = MaxAbsDiff;
     *total_abs_diff = TotalAbsDiff;
     *max_rel_diff = MaxRelDiff;

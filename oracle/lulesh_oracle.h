@@ -37,6 +37,23 @@ enum { LO_CBRT_LIBM = 0, LO_CBRT_EXACT = 1 };
 lo_group *lo_create(int nx, int tp, int nr, int balance, int cost, int cbrt_mode);
 void      lo_destroy(lo_group *g);
 
+/* One unstructured domain: the reference's -u <file.lmesh> input
+ * (LULESH/cuda/src/lulesh.cu:855-1052, single domain only).  Arrays are the
+ * file's contents: coordinates [numNode], nodelist [numElem][8], face
+ * neighbours [numElem][6] (lxim lxip letam letap lzetam lzetap), the X/Y/Z
+ * symmetry node sets and the free-surface node set.  elemBC is derived as
+ * the reference's reader derives it unless `elemBC` is given; the element
+ * found as the octant corner receives e_deposit (3.948746e+7 in the
+ * reference).  Returns NULL on an index out of range. */
+lo_group *lo_create_mesh(int numElem, int numNode, const double *x, const double *y, const double *z,
+                         const int *nodelist, const int *nbr, const int *symmX, int nSymmX,
+                         const int *symmY, int nSymmY, const int *symmZ, int nSymmZ,
+                         const int *freeSurf, int nFree, const int *elemBC, double e_deposit,
+                         int nr, int balance, int cost, int cbrt_mode);
+int       lo_get_elem_bc(const lo_group *g, int dom, int *out);
+int       lo_octant_corner(const lo_group *g);
+int       lo_all_elem(const lo_group *g);     /* per subdomain, with ghost planes */
+
 /* Advance until time >= stoptime or `max_cycles` more cycles were taken
  * (max_cycles < 0: no cap).  Returns the status code. */
 int    lo_run(lo_group *g, int max_cycles);
@@ -58,7 +75,9 @@ void   lo_set_qstop(lo_group *g, double qstop);
 void   lo_set_stoptime(lo_group *g, double t);
 
 /* Final-output figures of VerifyAndWriteFinalOutput
- * (LULESH/omp_4.0/lulesh-util.cc:177-224) for subdomain `dom`. */
+ * (LULESH/omp_4.0/lulesh-util.cc:177-224) for subdomain `dom`.  For an
+ * unstructured mesh: the energy of the octant-corner element, and zeros for
+ * the plane-0 symmetry figures, which the reference does not compute there. */
 void   lo_verify(const lo_group *g, int dom, double *origin_energy,
                  double *max_abs_diff, double *total_abs_diff,
                  double *max_rel_diff);

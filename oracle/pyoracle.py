@@ -57,6 +57,11 @@ def lib():
             "lo_recv_offset": (i, [vp, i, i, i, i, i, i, i]),
             "lo_get_recv_buffer": (i, [vp, i, dp]),
             "lo_cbrt_exact": (d, [d]),
+            "lo_create_mesh": (vp, [i, i, dp, dp, dp, ip, ip, ip, i, ip, i, ip, i, ip, i, ip, d,
+                                    i, i, i, i]),
+            "lo_get_elem_bc": (i, [vp, i, ip]),
+            "lo_octant_corner": (i, [vp]),
+            "lo_all_elem": (i, [vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -74,18 +79,88 @@ def _ip(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
-class Oracle:
-    """tp^3 subdomains of nx^3 elements (tp=1: the plain problem)."""
+E_DEPOSIT = 3.948746e+7   # energy the .lmesh reader deposits (lulesh.cu:1049-1050)
 
-    def __init__(self, nx, tp=1, nr=11, balance=1, cost=1, cbrt_mode=CBRT_LIBM):
+
+def read_lmesh(path):
+    """The reference's .lmesh reader (LULESH/cuda/src/lulesh.cu:855-970),
+    restated: whitespace-separated numbers in the order numElem numNode,
+    numNode coordinate triples, numElem x 8 node ids, numElem x 6 face
+    neighbours (xi-, xi+, eta-, eta+, zeta-, zeta+), then four counted node
+    sets: symmetry X, Y, Z and the free surface."""
+    with open(path) as f:
+        tok = f.read().split()
+    pos = 0
+
+    def take(n, conv):
+        nonlocal pos
+        if pos + n > len(tok):
+            raise ValueError(f"{path}: truncated")
+        out = tok[pos:pos + n]
+        pos += n
+        return np.array([conv(t) for t in out])
+
+    ne, nn = (int(v) for v in take(2, int))
+    xyz = take(3 * nn, float).reshape(nn, 3)
+    nodelist = take(8 * ne, int).astype(np.int32).reshape(ne, 8)
+    nbr = take(6 * ne, int).astype(np.int32).reshape(ne, 6)
+    sets = []
+    for _ in range(4):
+        cnt = int(take(1, int)[0])
+        sets.append(take(cnt, int).astype(np.int32))
+    return dict(num_elem=ne, num_node=nn, x=np.ascontiguousarray(xyz[:, 0]),
+                y=np.ascontiguousarray(xyz[:, 1]), z=np.ascontiguousarray(xyz[:, 2]),
+                nodelist=nodelist, nbr=nbr, symm=sets[:3], free=sets[3])
+
+
+class Oracle:
+    """tp^3 subdomains of nx^3 elements (tp=1: the plain problem), or one
+    unstructured domain (Oracle.from_mesh)."""
+
+    def __init__(self, nx, tp=1, nr=11, balance=1, cost=1, cbrt_mode=CBRT_LIBM, _handle=None):
         self._L = lib()
-        self.h = self._L.lo_create(nx, tp, nr, balance, cost, cbrt_mode)
+        self.h = _handle if _handle else self._L.lo_create(nx, tp, nr, balance, cost, cbrt_mode)
         if not self.h:
             raise ValueError("lo_create failed")
         self.nx, self.tp = nx, tp
         self.ndom = tp ** 3
         self.num_elem = self._L.lo_num_elem(self.h)
         self.num_node = self._L.lo_num_node(self.h)
+        self.all_elem = self._L.lo_all_elem(self.h)
+
+    @classmethod
+    def from_mesh(cls, m, nr=11, balance=1, cost=1, cbrt_mode=CBRT_LIBM, elem_bc=None,
+                  e_deposit=E_DEPOSIT):
+        """One unstructured domain from a mesh dict (read_lmesh's layout)."""
+        L = lib()
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        x, y, z = (arr(m[k], np.float64) for k in ("x", "y", "z"))
+        nl = arr(m["nodelist"], np.int32)
+        nb = arr(m["nbr"], np.int32)
+        sx, sy, sz = (arr(s, np.int32) for s in m["symm"])
+        fr = arr(m["free"], np.int32)
+        bc = None if elem_bc is None else _ip(arr(elem_bc, np.int32))
+        h = L.lo_create_mesh(int(m["num_elem"]), int(m["num_node"]), _dp(x), _dp(y), _dp(z), _ip(nl),
+                             _ip(nb), _ip(sx), sx.size, _ip(sy), sy.size, _ip(sz), sz.size, _ip(fr),
+                             fr.size, bc, float(e_deposit), nr, balance, cost, cbrt_mode)
+        if not h:
+            raise ValueError("lo_create_mesh rejected the mesh")
+        return cls(0, 1, _handle=h)
+
+    @property
+    def octant_corner(self):
+        return self._L.lo_octant_corner(self.h)
+
+    def elem_bc(self, dom=0):
+        out = np.empty(self.num_elem, dtype=np.int32)
+        self._L.lo_get_elem_bc(self.h, dom, _ip(out))
+        return out
 
     def close(self):
         if self.h:
@@ -119,7 +194,7 @@ class Oracle:
         if fid < 16:
             return self.num_node
         if fid in (28, 29, 30):
-            return self.num_elem + 6 * self.nx * self.nx
+            return self.all_elem
         return self.num_elem
 
     def field(self, name, dom=0):

@@ -1,0 +1,170 @@
+"""Unstructured-mesh (.lmesh) input on the CPU: the oracle's restatement of
+the reference's reader and boundary derivation (LULESH/cuda/src/lulesh.cu:
+855-1052), the library's host-side reader against it, and the properties
+that pin the unstructured step (no GPU needed).
+
+The reference's unstructured path runs only in its CUDA build, which cannot
+run here; the step itself is the oracle's structured step, pinned bit for bit
+against the compiled omp_4.0 reference, so the unstructured form is pinned by
+(1) hand-derived boundary bits of the reference's own mesh file, (2) the
+block written as an unstructured mesh reproducing the structured run bit for
+bit, and (3) invariance under node renumbering.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, REPO, assert_bitwise
+import umesh_util as U
+
+LMESH = os.path.join(GOLDEN, U.GOLDEN_LMESH)
+REF_LMESH21 = "/root/reference/LULESH/cuda/src/sedov21oct.lmesh"
+FIELDS = ["x", "y", "z", "xd", "yd", "zd", "e", "p", "q", "v", "ss", "volo", "elemMass",
+          "nodalMass", "delv", "vdov", "arealg", "delv_xi", "delv_eta", "delv_zeta"]
+
+
+def test_reader_fixture(oracle_mod):
+    m = oracle_mod.read_lmesh(LMESH)
+    assert (m["num_elem"], m["num_node"]) == (875, 1126)
+    assert [len(s) for s in m["symm"]] == [146, 146, 146] and len(m["free"]) == 91
+    assert m["nodelist"][0].tolist() == [0, 5, 6, 7, 1, 2, 3, 4]
+    assert m["nbr"][0].tolist() == [0, 5, 0, 7, 0, 1]
+    assert (m["x"][1], m["y"][1], m["z"][1]) == (0.0, 0.0, 0.08)
+    assert m["free"][-1] == 1125
+
+
+def test_boundary_bits_hand_derived(oracle_mod):
+    # element 0 = nodes (0,5,6,7,1,2,3,4); on a symmetry plane (x, y or z
+    # = 0) are all but node 3 (0.071149, 0.071149, 0.071149), its local
+    # corner 6.  Mask 0xbf: corners {0..3} (zeta-), {0,1,4,5} (eta-) and
+    # {0,3,4,7} (xi-) complete; {4..7}, {2,3,6,7}, {1,2,5,6} miss corner 6.
+    # None of its nodes is on the free surface.
+    m = oracle_mod.read_lmesh(LMESH)
+    assert not np.isin(m["nodelist"][0], m["free"]).any()
+    o = oracle_mod.Oracle.from_mesh(m)
+    bc = o.elem_bc()
+    assert bc[0] == U.XI_M_SYMM | U.ETA_M_SYMM | U.ZETA_M_SYMM
+    assert o.octant_corner == 0
+    # an element with a zeta+ face on the free surface gets ZETA_P_SYMM
+    onfree = np.isin(m["nodelist"], m["free"])
+    zp = np.nonzero(onfree[:, 4:].all(axis=1))[0]
+    assert zp.size > 0 and (bc[zp] & U.ZETA_P_SYMM).all()
+    # no FREE or COMM bits are ever derived
+    assert not (bc & (U.FREE_BITS | 0x24924)).any()
+
+
+def test_library_reader_matches_oracle(oracle_mod):
+    import lulesh_amd as la
+    la.load()
+    pm = oracle_mod.read_lmesh(LMESH)
+    o = oracle_mod.Oracle.from_mesh(pm)
+    m = la.Mesh.read(LMESH)
+    assert (m.num_elem, m.num_node) == (875, 1126)
+    assert m.octant_corner == o.octant_corner
+    assert np.array_equal(m.elem_bc(), o.elem_bc())
+    m2 = la.Mesh.from_arrays(pm)
+    assert np.array_equal(m2.elem_bc(), o.elem_bc())
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LMESH21), reason="reference tree not present")
+def test_library_reader_sedov21oct(oracle_mod):
+    import lulesh_amd as la
+    la.load()
+    pm = oracle_mod.read_lmesh(REF_LMESH21)
+    o = oracle_mod.Oracle.from_mesh(pm)
+    m = la.Mesh.read(REF_LMESH21)
+    assert (m.num_elem, m.num_node) == (31408, 34446)
+    assert m.octant_corner == o.octant_corner
+    assert np.array_equal(m.elem_bc(), o.elem_bc())
+
+
+def test_library_reader_errors(tmp_path):
+    import lulesh_amd as la
+    L = la.load()
+    import ctypes
+    h = ctypes.c_void_p()
+    assert L.lulesh_mesh_read(str(tmp_path / "missing.lmesh").encode(), ctypes.byref(h)) == la.EIO
+    text = open(LMESH).read()
+    short = tmp_path / "short.lmesh"
+    short.write_text(text[: len(text) // 2])
+    assert L.lulesh_mesh_read(str(short).encode(), ctypes.byref(h)) == la.EIO
+    bad = tmp_path / "bad.lmesh"
+    lines = text.split("\n")
+    lines[1 + 1126] = "0 5 6 7 1 2 3 99999"          # node id out of range
+    bad.write_text("\n".join(lines))
+    assert L.lulesh_mesh_read(str(bad).encode(), ctypes.byref(h)) == la.EINVAL
+
+
+def test_block_boundary_bits(oracle_mod):
+    # the block as an unstructured mesh: the reader's bits are the block's
+    # elemBC with every FREE face marked SYMM instead (the reference's
+    # reader marks free-surface faces with the *_SYMM bits)
+    nx = 6
+    m = U.structured_mesh(nx)
+    bc_block = oracle_mod.Oracle(nx).elem_bc()
+    o = oracle_mod.Oracle.from_mesh(m)
+    want = bc_block & ~U.FREE_BITS
+    for free, symm in [(U.XI_P_FREE, U.XI_P_SYMM), (U.ETA_P_FREE, U.ETA_P_SYMM),
+                       (U.ZETA_P_FREE, U.ZETA_P_SYMM)]:
+        want |= np.where(bc_block & free, symm, 0)
+    assert np.array_equal(o.elem_bc(), want)
+    assert o.octant_corner == 0
+
+
+@pytest.mark.parametrize("nx", [5, 8])
+def test_block_as_umesh_matches_structured(oracle_mod, nx, tmp_path):
+    # written to a file and read back, given the block's own elemBC and
+    # deposit: the unstructured set-up and step reproduce the structured
+    # run bit for bit (full run to stoptime)
+    m = U.structured_mesh(nx)
+    path = tmp_path / "block.lmesh"
+    U.write_lmesh(path, m)
+    m = oracle_mod.read_lmesh(str(path))
+    ref = oracle_mod.Oracle(nx)
+    o = oracle_mod.Oracle.from_mesh(m, elem_bc=ref.elem_bc(), e_deposit=U.einit(nx))
+    assert o.deltatime == ref.deltatime
+    assert ref.run() == 0 and o.run() == 0
+    assert o.cycle == ref.cycle and o.time == ref.time
+    ne = nx ** 3
+    for f in FIELDS:   # (the block's gradient arrays carry ghost planes after ne)
+        assert_bitwise(o.field(f)[:ne] if f.startswith("delv_") else o.field(f),
+                       ref.field(f)[:ne] if f.startswith("delv_") else ref.field(f), f"s={nx} {f}")
+
+
+def test_node_renumbering_invariance(oracle_mod):
+    # renumbering nodes changes no sum order (per-node sums run over
+    # elements, per-element work over the element's own corner order)
+    m = oracle_mod.read_lmesh(LMESH)
+    perm = np.random.default_rng(7).permutation(m["num_node"])
+    a = oracle_mod.Oracle.from_mesh(m)
+    b = oracle_mod.Oracle.from_mesh(U.renumber_nodes(m, perm))
+    assert a.run(150) == 0 and b.run(150) == 0
+    for f in ["x", "y", "z", "xd", "yd", "zd", "nodalMass"]:
+        assert_bitwise(b.field(f)[perm], a.field(f), f"node field {f}")
+    for f in ["e", "p", "q", "v", "ss"]:
+        assert_bitwise(b.field(f), a.field(f), f"elem field {f}")
+
+
+def test_regions_do_not_change_results(oracle_mod):
+    # the EOS repetitions recompute the same values: -r/-b/-c change the
+    # cost only (lulesh.cc:2677-2815)
+    m = oracle_mod.read_lmesh(LMESH)
+    a = oracle_mod.Oracle.from_mesh(m, nr=11, balance=1, cost=1)
+    b = oracle_mod.Oracle.from_mesh(m, nr=5, balance=2, cost=3)
+    assert not np.array_equal(a.elem_rep(), b.elem_rep())
+    assert a.run(200) == 0 and b.run(200) == 0
+    for f in ["e", "p", "q", "x", "xd"]:
+        assert_bitwise(b.field(f), a.field(f), f)
+
+
+def test_full_run_energy_is_finite(oracle_mod):
+    # the deposit stays in the octant corner and the run reaches stoptime
+    m = oracle_mod.read_lmesh(LMESH)
+    o = oracle_mod.Oracle.from_mesh(m)
+    assert o.field("e")[0] == U.E_BASE
+    assert o.run() == 0
+    assert abs(o.time - 1.0e-2) < 1e-15 and o.cycle > 100
+    e = o.field("e")
+    assert np.isfinite(e).all() and e.max() > 0
+    assert o.verify()["origin_energy"] == e[0]
