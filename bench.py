@@ -19,6 +19,9 @@ Other modes (extra JSON lines are not emitted by default):
   --mode batched   64-problem ensemble vs sequential plain runs
   --mode serve     Poisson-arrival serving loop, p50/p99 latency
   --mode multidomain  2x2x2 subdomains of -s 45 on one GPU
+  --mode umesh     the unstructured-mesh (.lmesh) path: the -s 100 block as
+                   an unstructured mesh, and the reference's sedov15oct mesh
+                   to stoptime
 """
 import argparse
 import json
@@ -368,6 +371,57 @@ def bench_multidomain(args):
     }
 
 
+def bench_umesh(args):
+    nx = args.size
+    E = nx ** 3
+    m = lulesh_amd.Mesh.from_arrays(lulesh_amd.block_mesh_arrays(nx))
+    scale = nx / 45.0
+    m.set_energy(3.948746e+7 * scale * scale * scale)   # the block's own deposit
+    g = lulesh_amd.Group.from_mesh(m)
+    m.close()
+    g.enqueue(args.warmup)
+    g.enqueue(args.chunk)
+    rc, c0, _, _ = g.sync()
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = min(args.chunk, args.steps - done)
+        g.enqueue(k)
+        done += k
+    rc, c1, _, _ = g.sync()
+    t1 = time.perf_counter()
+    if rc != 0 or c1 - c0 != args.steps:
+        raise SystemExit(f"umesh run failed: rc {rc}, {c1 - c0} cycles")
+    ms = g.profile(10) / 10
+    g.close()
+    # the reference's own unstructured mesh, 875 elements, to stoptime
+    small = {}
+    path = os.path.join(REPO, "tests", "golden", "sedov15oct.lmesh")
+    if os.path.exists(path):
+        gs = lulesh_amd.Group.from_mesh(lulesh_amd.Mesh.read(path))
+        gs.set_max_cycles(20)
+        gs.run()
+        gs.set_max_cycles(-1)
+        s0 = time.perf_counter()
+        rc2 = gs.run(100)
+        s1 = time.perf_counter()
+        _, cyc, _, _ = gs.sync()
+        small = {"mesh": "sedov15oct.lmesh (875 elements)", "status": rc2, "cycles": cyc - 20,
+                 "ms_per_cycle": (s1 - s0) * 1e3 / max(1, cyc - 20)}
+        gs.close()
+    return {
+        "metric": "unstructured-mesh path: zone-cycles/s on the -s 100 block given as an .lmesh-style mesh",
+        "value": E * (c1 - c0) / (t1 - t0), "unit": "zone-cycles/s", "n_gpus": 1,
+        "steps": c1 - c0, "warmup": args.warmup, "ms_per_step": (t1 - t0) * 1e3 / (c1 - c0),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: Sedov block written as an unstructured mesh (explicit connectivity)",
+        "config": {"workload": f"-s {nx} block as an unstructured mesh, -r 11 -b 1 -c 1",
+                   "elements": E},
+        "kernels_ms": {KERNEL_NAMES[k]: round(float(ms[k]), 4) for k in range(7) if ms[k] > 0},
+        "sedov15oct": small,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,7 +430,7 @@ def main():
     ap.add_argument("--size", type=int, default=100)
     ap.add_argument("--chunk", type=int, default=50, help="cycles per captured graph launch")
     ap.add_argument("--mode", default="plain",
-                    choices=["plain", "copy", "batched", "serve", "multidomain"])
+                    choices=["plain", "copy", "batched", "serve", "multidomain", "umesh"])
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=8.0, help="serve: Poisson arrivals per second")
@@ -402,6 +456,8 @@ def main():
         line = bench_serve(args)
     elif args.mode == "multidomain":
         line = bench_multidomain(args)
+    elif args.mode == "umesh":
+        line = bench_umesh(args)
     else:
         line = bench_plain(args)
     print(json.dumps(line), flush=True)

@@ -650,23 +650,39 @@ template <bool FUSE_UPDATE>
 __device__ __forceinline__ void node_gather_u(const DomDev &D, int nd)
 {
     const size_t E = (size_t)D.numElem;
-    const int q0 = D.u_nstart[nd], q1 = D.u_nstart[nd + 1];
+    const int q0 = D.u_nstart[nd], cnt = D.u_nstart[nd + 1] - q0;
     double sx = 0.0, sy = 0.0, sz = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;
-    for (int q = q0; q < q1; ++q) {
-        const int ent = D.u_ncorner[q];
-        const size_t e = (size_t)(ent >> 3);
-        const int c = ent & 7;
-        sx += D.corner[(0 * 8 + c) * E + e];
-        sy += D.corner[(1 * 8 + c) * E + e];
-        sz += D.corner[(2 * 8 + c) * E + e];
-    }
-    for (int q = q0; q < q1; ++q) {
-        const int ent = D.u_ncorner[q];
-        const size_t e = (size_t)(ent >> 3);
-        const int c = ent & 7;
-        hx += D.corner[(3 * 8 + c) * E + e];
-        hy += D.corner[(4 * 8 + c) * E + e];
-        hz += D.corner[(5 * 8 + c) * E + e];
+    // corners in batches of 8 (a hexahedral mesh node has at most 8 unless
+    // the mesh is irregular): every load of a batch is issued before the
+    // first add, then the adds run in list order
+    for (int b = 0; b < cnt; b += 8) {
+        size_t off[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int ent = b + t < cnt ? D.u_ncorner[q0 + b + t] : 0;
+            off[t] = (size_t)(ent & 7) * E + (size_t)(ent >> 3);
+        }
+        double v[6][8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const bool ok = b + t < cnt;
+#pragma unroll
+            for (int comp = 0; comp < 6; ++comp) v[comp][t] = ok ? D.corner[comp * 8 * E + off[t]] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            if (b + t >= cnt) break;
+            sx += v[0][t];
+            sy += v[1][t];
+            sz += v[2][t];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            if (b + t >= cnt) break;
+            hx += v[3][t];
+            hy += v[4][t];
+            hz += v[5][t];
+        }
     }
     const double fx = sx + hx, fy = sy + hy, fz = sz + hz;
     if (FUSE_UPDATE) {

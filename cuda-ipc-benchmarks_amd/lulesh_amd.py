@@ -281,6 +281,35 @@ class Mesh:
         _chk(load().lulesh_mesh_set_energy(self.h, float(e)), "set_energy")
 
 
+def block_mesh_arrays(nx):
+    """The Sedov block of -s nx (LULESH/omp_4.0/lulesh-init.cc:189-238) as
+    unstructured-mesh arrays for Mesh.from_arrays: the block's numbering
+    and coordinates, boundary faces naming their own element, symmetry sets
+    on the x, y, z = 0 planes, the far faces as free surface."""
+    en = nx + 1
+    idx = np.arange(en)
+    k, j, i = np.meshgrid(idx, idx, idx, indexing="ij")
+    coord = lambda a: np.ascontiguousarray((1.125 * a / float(nx)).ravel())
+    e = np.arange(nx)
+    ek, ej, ei = np.meshgrid(e, e, e, indexing="ij")
+    n0 = (ek * en * en + ej * en + ei).ravel()
+    corner = np.array([0, 1, en + 1, en, en * en, en * en + 1, en * en + en + 1, en * en + en])
+    el = np.arange(nx ** 3).reshape(nx, nx, nx)
+    nbr = np.empty((nx ** 3, 6), dtype=np.int32)
+    for f, (axis, d) in enumerate([(2, -1), (2, 1), (1, -1), (1, 1), (0, -1), (0, 1)]):
+        nb = np.roll(el, -d, axis=axis)
+        edge = [slice(None)] * 3
+        edge[axis] = -1 if d > 0 else 0
+        nb[tuple(edge)] = el[tuple(edge)]
+        nbr[:, f] = nb.ravel()
+    ni, nj, nk = i.ravel(), j.ravel(), k.ravel()
+    nid = np.arange(en ** 3, dtype=np.int32)
+    return dict(num_elem=nx ** 3, num_node=en ** 3, x=coord(i), y=coord(j), z=coord(k),
+                nodelist=(n0[:, None] + corner[None, :]).astype(np.int32), nbr=nbr,
+                symm=[nid[ni == 0], nid[nj == 0], nid[nk == 0]],
+                free=nid[(ni == nx) | (nj == nx) | (nk == nx)])
+
+
 class Group:
     """tp^3 subdomains of nx^3 elements on the GPU (tp=1: the plain problem),
     or one unstructured domain (Group.from_mesh)."""

@@ -56,9 +56,8 @@ def test_block_as_umesh_matches_structured(amd, nx):
     # corner lists) with the block's own elemBC and deposit gives the
     # structured kernels' result bit for bit
     print(f"case: block s={nx} as an unstructured mesh", flush=True)
-    ref = amd.Group(nx)
-    bc = None
     import pyoracle
+    ref = amd.Group(nx)
     bc = pyoracle.Oracle(nx).elem_bc()
     m = amd.Mesh.from_arrays(U.structured_mesh(nx))
     m.set_elem_bc(bc)
@@ -114,3 +113,23 @@ def test_umesh_group_rejects_comm(amd, tmp_path):
     assert len(vals) == 4 * 1126
     x = g.field("x")
     assert float(vals[4 * 7 + 1]) == pytest.approx(x[7], abs=1e-10)
+
+
+def test_driver_unstructured(oracle_mod, tmp_path):
+    import subprocess
+    from conftest import REPO
+    print("case: driver -u sedov15oct.lmesh", flush=True)
+    driver = os.path.join(REPO, "cuda-ipc-benchmarks_amd", "lulesh_amd_run")
+    out = subprocess.run([driver, "-u", LMESH], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    o = oracle_mod.Oracle.from_mesh(oracle_mod.read_lmesh(LMESH), cbrt_mode=oracle_mod.CBRT_EXACT)
+    assert o.run() == 0
+    lines = out.stdout.splitlines()
+    assert lines[0].startswith("Running until t=0.010000, Problem size=875")
+    assert f"   Iteration count     =  {o.cycle} " in lines
+    assert f"FullPrecision OriginEnergy = {o.field('e')[o.octant_corner]:.17e}" in lines
+    assert (tmp_path / "xyz.asc").stat().st_size > 0
+    bad = subprocess.run([driver, "-u", str(tmp_path / "none.lmesh")], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "could not read" in bad.stderr
