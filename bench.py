@@ -432,7 +432,7 @@ def main():
     ap.add_argument("--mode", default="plain",
                     choices=["plain", "copy", "batched", "serve", "multidomain", "umesh"])
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=8.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--slots", type=int, default=64, help="serve: max resident problems")
     ap.add_argument("--seq-sample", type=int, default=40)
