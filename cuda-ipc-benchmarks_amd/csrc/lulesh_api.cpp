@@ -105,6 +105,11 @@ int domain_alloc(lulesh_domain *d)
     D.fpz = D.fz;
     D.dir_node = D.dir_elem = nullptr;
     D.dir_sbn = D.dir_ghost = nullptr;
+    D.dir_pv = nullptr;
+    D.dir_pvnode = nullptr;
+    D.pv_snap = nullptr;
+    D.pv_nslot = 0;
+    D.dir_pvlist = nullptr;
     D.dom_base = 0;
     HIPCHK(hipMemset(arena, 0, bytes));
     return LULESH_OK;
@@ -343,6 +348,10 @@ DirectTables::~DirectTables()
     if (elem) (void)hipFree(elem);
     if (sbn) (void)hipFree(sbn);
     if (ghost) (void)hipFree(ghost);
+    if (pv) (void)hipFree(pv);
+    if (pvnode) (void)hipFree(pvnode);
+    if (snap) (void)hipFree(snap);
+    if (pvlist) (void)hipFree(pvlist);
 }
 
 // The direct halo steps are the device_local exchange plans read the other
@@ -356,12 +365,12 @@ int group_direct(lulesh_group *g, DirectTables **out)
 {
     if (g->direct) { *out = g->direct.get(); return LULESH_OK; }
     if (g->backend != &kDeviceLocal || g->comm_down || g->ndom < 2) return LULESH_EINVAL;
-    auto plan = [g](int nf, bool planeOnly, HostPlan &hp) {
+    auto plan = [g](int nf, bool planeOnly, HostPlan &hp, bool doFlag = true) {
         PlanInput in;
         in.tp = g->tp;
         for (auto *d : g->doms) in.doms.push_back(&d->host);
         in.nfields = nf;
-        in.doFlag = true;
+        in.doFlag = doFlag;
         in.planeOnly = planeOnly;
         in.pack_dest = [g](int from, int to, size_t off) {
             return g->backend->pack_dest(g->doms[from], g->doms[to], off, g->backend_ctx);
@@ -371,8 +380,8 @@ int group_direct(lulesh_group *g, DirectTables **out)
         };
         return build_plan(in, hp);
     };
-    HostPlan sbn, mq;
-    if (plan(3, false, sbn) || plan(3, true, mq)) return LULESH_EDIRECTION;
+    HostPlan sbn, mq, pvp;
+    if (plan(3, false, sbn) || plan(3, true, mq) || plan(6, false, pvp, false)) return LULESH_EDIRECTION;
     auto senders = [](const HostPlan &hp) {
         std::unordered_map<const double *, DirRef> m;
         for (const PackEnt &e : hp.pack) m[e.dst] = DirRef{e.src_dom, e.src_idx};
@@ -413,6 +422,59 @@ int group_direct(lulesh_group *g, DirectTables **out)
             ghost_list.insert(ghost_list.end(), kv.second.begin(), kv.second.end());
         }
     }
+    // CommSyncPosVel (lulesh-comm.cc:1261-1680): a receiving copy keeps the
+    // last message's value, the final writer's own update as packed before
+    // any unpack.  A final writer's own copy can itself be overwritten (a
+    // mixed-direction edge), so each copy computes its final writer's
+    // update from that writer's own force sum and its pre-update state,
+    // which the writer keeps in a snapshot by cycle parity.
+    std::vector<int> pv(t->node_off[nd], -1);
+    std::vector<PvNode> pvnode;
+    {
+        const auto from = senders(pvp);
+        std::map<std::pair<int, int>, int> slot_of;   // final-writer node -> its snapshot slot
+        t->nslot.assign(nd, 0);
+        std::vector<std::pair<DirRef, DirRef>> fw;    // (copy, its final writer)
+        for (const UnpackTgt &u : pvp.tgt) {
+            if (u.count < 1) continue;
+            const auto it = from.find(pvp.src[u.first + u.count - 1].src);
+            if (it == from.end()) return LULESH_EDIRECTION;
+            const DirRef w = it->second;
+            if (!slot_of.count({w.dom, w.idx})) slot_of[{w.dom, w.idx}] = t->nslot[w.dom]++;
+            fw.push_back({DirRef{u.dom, u.idx}, w});
+        }
+        auto entry = [&](int dom, int idx) -> PvNode & {
+            int &k = pv[t->node_off[dom] + idx];
+            if (k < 0) {
+                k = (int)pvnode.size();
+                pvnode.push_back(PvNode{-1, 0, 0, -1});
+            }
+            return pvnode[k];
+        };
+        for (auto &c : fw) {
+            PvNode &e = entry(c.first.dom, c.first.idx);
+            e.fw_dom = c.second.dom;
+            e.fw_idx = c.second.idx;
+            e.fw_slot = slot_of[{c.second.dom, c.second.idx}];
+        }
+        for (auto &kv : slot_of) entry(kv.first.first, kv.first.second).own_slot = kv.second;
+        t->snap_off.assign(nd + 1, 0);
+        for (int r = 0; r < nd; ++r) t->snap_off[r + 1] = t->snap_off[r] + 12 * (size_t)t->nslot[r];
+    }
+    std::vector<DirRef> pvl;
+    for (int r = 0; r < nd; ++r)
+        for (size_t n = 0; n < t->node_off[r + 1] - t->node_off[r]; ++n)
+            if (pv[t->node_off[r] + n] >= 0) pvl.push_back(DirRef{r, (int)n});
+    t->npv = (int)pvl.size();
+    HIPCHK(hipMalloc((void **)&t->pvlist, std::max<size_t>(1, pvl.size()) * sizeof(DirRef)));
+    if (!pvl.empty())
+        HIPCHK(hipMemcpy(t->pvlist, pvl.data(), pvl.size() * sizeof(DirRef), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc((void **)&t->pv, pv.size() * sizeof(int)));
+    HIPCHK(hipMalloc((void **)&t->pvnode, std::max<size_t>(1, pvnode.size()) * sizeof(PvNode)));
+    HIPCHK(hipMalloc((void **)&t->snap, std::max<size_t>(1, t->snap_off[nd]) * sizeof(double)));
+    HIPCHK(hipMemcpy(t->pv, pv.data(), pv.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!pvnode.empty())
+        HIPCHK(hipMemcpy(t->pvnode, pvnode.data(), pvnode.size() * sizeof(PvNode), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc((void **)&t->node, node.size() * sizeof(IdxSpan)));
     HIPCHK(hipMalloc((void **)&t->elem, elem.size() * sizeof(IdxSpan)));
     HIPCHK(hipMalloc((void **)&t->sbn, std::max<size_t>(1, sbn_list.size()) * sizeof(DirRef)));
@@ -602,6 +664,18 @@ void group_free(lulesh_group *g)
     if (g->ctl_mem) (void)hipFree(g->ctl_mem);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
+}
+
+// a host-side write or exchange changed the node state of a decomposed
+// group: its engine's PosVel snapshots are stale, so the next run builds a
+// new engine (which takes them afresh)
+void group_state_changed(lulesh_group *g)
+{
+    if (g->ndom > 1 && g->engine) {
+        (void)hipStreamSynchronize(g->stream);
+        engine_destroy(g->engine);
+        g->engine = nullptr;
+    }
 }
 
 int group_read_ctl(lulesh_group *g, GroupCtl *G, DomCtl *D0, hipStream_t s)
@@ -836,6 +910,7 @@ int lulesh_domain_set_field(lulesh_domain *d, int f, const double *host, size_t 
     if (!p || n < len) return LULESH_EINVAL;
     HIPCHK(hipStreamSynchronize(d->g->stream));
     HIPCHK(hipMemcpy(p, host, len * 8, hipMemcpyHostToDevice));
+    if (f >= LULESH_X && f <= LULESH_ZD) group_state_changed(d->g);
     if (f == LULESH_SS || f == LULESH_V || f == LULESH_VOLO || f == LULESH_ELEM_MASS) {
         const int rc = domain_refresh_hgcoef(d, d->g->stream);
         if (rc) return rc;
@@ -918,6 +993,7 @@ int CommSBN(lulesh_domain *d, int xferFields, const int *fields, lulesh_stream_t
 {
     if (!d || !fields || xferFields < 1 || xferFields > 6) return LULESH_EINVAL;
     if (d->g->umesh) return LULESH_EINVAL;
+    group_state_changed(d->g);
     lulesh_group *g = d->g;
     if (g->ndom == 1) return LULESH_OK;
     DevPlan *p = nullptr;
@@ -932,6 +1008,7 @@ int CommSyncPosVel(lulesh_domain *d, lulesh_stream_t stream)
 {
     if (!d) return LULESH_EINVAL;
     if (d->g->umesh) return LULESH_EINVAL;
+    group_state_changed(d->g);
     lulesh_group *g = d->g;
     if (g->ndom == 1) return LULESH_OK;
     DevPlan *p = nullptr;
@@ -1060,6 +1137,7 @@ int lulesh_domain_write_solution(lulesh_domain *d, const char *path)
 int lulesh_group_exchange(lulesh_group *g, int msgType, int nf, const int *fields, lulesh_stream_t stream)
 {
     if (!g || g->umesh || !fields) return LULESH_EINVAL;
+    group_state_changed(g);
     return group_exchange(g, msgType, nf, fields, stream_of(g, stream));
 }
 

@@ -32,6 +32,17 @@ struct DirRef {           // an index of another subdomain of the group
     int idx;
 };
 
+// a node copy touched by CommSyncPosVel: its final writer (the last
+// message's sender, -1 if none) with the writer's node index and snapshot
+// slot, and this copy's own snapshot slot (-1 unless it is some other
+// copy's final writer)
+struct PvNode {
+    int fw_dom;
+    int fw_idx;
+    int fw_slot;
+    int own_slot;
+};
+
 // node positions and velocities
 struct NodeState {
     double *x, *y, *z, *xd, *yd, *zd;
@@ -60,6 +71,18 @@ struct DomDev {
     const IdxSpan *dir_elem;
     const DirRef *dir_sbn;
     const DirRef *dir_ghost;
+    // CommSyncPosVel done in place: per node an index into dir_pvnode, or
+    // -1 for a node no PosVel message touches; null when the PosVel
+    // exchange runs.  pv_snap holds, by cycle parity, the state after the
+    // cycle of this subdomain's nodes that are another copy's final writer
+    // ([2][6][pv_nslot]).
+    const int *dir_pv;
+    const struct PvNode *dir_pvnode;
+    double *pv_snap;
+    int pv_nslot;
+    // the group's node copies with dir_pv >= 0, as (rank, node): the node
+    // update runs them in tiles of their own (Tile::pad = 1)
+    const DirRef *dir_pvlist;
     int dom_base;
     double *e, *p, *q, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass, *vnew;
     double *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
@@ -115,7 +138,9 @@ struct GroupCtl {
 
 // element or node tile: [start, start + blockDim) of subdomain dom; for
 // element tiles eoff/ecount locate the tile's EOS evaluations in the
-// subdomain's evaluation table
+// subdomain's evaluation table.  A node-update tile with pad = 1 covers
+// entries [start, start + ecount) of the group's PosVel copy list instead,
+// dom being the group's first subdomain.
 struct Tile {
     int dom;
     int start;

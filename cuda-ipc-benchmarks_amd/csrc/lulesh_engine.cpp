@@ -192,10 +192,40 @@ static int build_tiles(Engine *e)
             a += len;
         }
     }
+    // the direct update: the node tiles, then tiles of each decomposed
+    // group's PosVel copies (those nodes leave the node tiles at once)
+    std::vector<Tile> pvt;
+    if (e->direct_pv)
+        for (int di = 0, gi = 0; gi < (int)e->groups.size(); ++gi) {
+            lulesh_group *g = e->groups[gi];
+            if (g->ndom > 1 && g->direct)
+                for (int s = 0; s < g->direct->npv; s += e->tile_n)
+                    pvt.push_back(Tile{di, s, 0, std::min(e->tile_n, g->direct->npv - s), 1});
+            di += g->ndom;
+        }
+    // the PosVel tiles (long chains of dependent loads) lead each XCD's
+    // chunk, so they start first and overlap the plain node tiles
+    std::vector<Tile> ut;
+    {
+        const size_t n = nt.size() + pvt.size(), q = n / 8, r = n % 8;
+        size_t ri = 0, pi = 0;
+        for (size_t x = 0; x < 8; ++x) {
+            const size_t len = x < r ? q + 1 : q;
+            const size_t pend = pvt.size() * (x + 1) / 8;
+            size_t k = 0;
+            for (; pi < pend && k < len; ++pi, ++k) ut.push_back(pvt[pi]);
+            for (; k < len && ri < nt.size(); ++k) ut.push_back(nt[ri++]);
+            for (; k < len && pi < pvt.size(); ++k) ut.push_back(pvt[pi++]);
+        }
+    }
     if (e->d_etiles) (void)hipFree(e->d_etiles);
     if (e->d_ntiles) (void)hipFree(e->d_ntiles);
     if (e->d_qtiles) (void)hipFree(e->d_qtiles);
-    e->d_etiles = e->d_ntiles = e->d_qtiles = nullptr;
+    if (e->d_utiles) (void)hipFree(e->d_utiles);
+    e->d_etiles = e->d_ntiles = e->d_qtiles = e->d_utiles = nullptr;
+    e->n_utiles = (int)ut.size();
+    HIPCHK(hipMalloc((void **)&e->d_utiles, std::max<size_t>(1, ut.size()) * sizeof(Tile)));
+    if (!ut.empty()) HIPCHK(hipMemcpy(e->d_utiles, ut.data(), ut.size() * sizeof(Tile), hipMemcpyHostToDevice));
     e->n_etiles = (int)et.size();
     e->n_ntiles = (int)nt.size();
     e->n_qtiles = (int)qt.size();
@@ -261,6 +291,8 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
         e->direct = !e->fused && !(nd && nd[0] == '1');
         for (auto *g : groups)
             if (g->ndom > 1 && !group_device_local(g)) e->direct = false;
+        const char *npv = getenv("LULESH_NO_DIRECT_POSVEL");
+        e->direct_pv = e->direct && !(npv && npv[0] == '1');
         if (e->direct) {
             for (size_t gi = 0; gi < groups.size(); ++gi) {
                 lulesh_group *g = groups[gi];
@@ -281,6 +313,19 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
                     D.dir_elem = t->elem + t->elem_off[r];
                     D.dir_sbn = t->sbn;
                     D.dir_ghost = t->ghost;
+                }
+            }
+            for (size_t gi = 0; gi < groups.size() && e->direct_pv; ++gi) {
+                lulesh_group *g = groups[gi];
+                if (g->ndom < 2) continue;
+                DirectTables *t = g->direct.get();
+                for (int r = 0; r < g->ndom; ++r) {
+                    DomDev &D = doms[dom0[gi] + r];
+                    D.dir_pv = t->pv + t->node_off[r];
+                    D.dir_pvnode = t->pvnode;
+                    D.pv_snap = t->snap + t->snap_off[r];
+                    D.pv_nslot = t->nslot[r];
+                    D.dir_pvlist = t->pvlist;
                 }
             }
         }
@@ -308,8 +353,18 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
         (void)hipMemcpy(e->d_groups, gptr.data(), gptr.size() * sizeof(GroupCtl *), hipMemcpyHostToDevice);
     if (!dom0.empty())
         (void)hipMemcpy(e->d_dom0, dom0.data(), dom0.size() * sizeof(int), hipMemcpyHostToDevice);
-    const int rc = build_tiles(e);
+    int rc = build_tiles(e);
     if (rc) { engine_destroy(e); return rc; }
+    if (e->direct_pv) {
+        // the final writers' snapshots start from the current node state
+        bool ok = true;
+        for (auto *g : groups) ok = ok && hipStreamSynchronize(g->stream) == hipSuccess;
+        if (!ok || launch_pv_snapshot(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, nullptr) != hipSuccess ||
+            hipStreamSynchronize(nullptr) != hipSuccess) {
+            engine_destroy(e);
+            return LULESH_EHIP;
+        }
+    }
     *out = e;
     return LULESH_OK;
 }
@@ -325,6 +380,7 @@ void engine_destroy(Engine *e)
     if (e->d_qtiles) (void)hipFree(e->d_qtiles);
     if (e->d_ntiles) (void)hipFree(e->d_ntiles);
     if (e->d_ctiles) (void)hipFree(e->d_ctiles);
+    if (e->d_utiles) (void)hipFree(e->d_utiles);
     delete e;
 }
 
@@ -404,7 +460,7 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
     }
     if (e->direct) {
         // CommSBN summed straight from the neighbours' partial forces
-        STEP(3, HIPCHK(launch_node_update_direct(e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
+        STEP(3, HIPCHK(launch_node_update_direct(e->d_doms, e->d_utiles, e->n_utiles, e->tile_n, s)));
     } else {
         if (!e->fused) {
             STEP(6, rc = exchange_all(e, LULESH_MSG_COMM_SBN, 3, kSbn, s));
@@ -413,7 +469,7 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
         if (!updated)
             STEP(3, HIPCHK(launch_node_update(e->umesh, e->d_doms, e->d_ntiles, e->n_ntiles, e->tile_n, s)));
     }
-    if (!e->fused) {
+    if (!e->fused && !e->direct_pv) {   // direct_pv: the node update wrote every copy
         STEP(6, rc = exchange_all(e, LULESH_MSG_SYNC_POS_VEL, 6, kPosVel, s));
         if (rc) return rc;
     }

@@ -21,6 +21,15 @@ struct DirectTables {
     IdxSpan *node = nullptr;       // sum of numNode over the group's subdomains
     IdxSpan *elem = nullptr;       // sum of numElem
     DirRef *sbn = nullptr, *ghost = nullptr;
+    // CommSyncPosVel: per node (sum of numNode) an index into pvnode or
+    // -1; the snapshots of the final-writer nodes, per rank at snap_off
+    int *pv = nullptr;
+    PvNode *pvnode = nullptr;
+    double *snap = nullptr;
+    std::vector<size_t> snap_off;   // per rank, into snap ([2][6][nslot] each)
+    std::vector<int> nslot;         // per rank
+    DirRef *pvlist = nullptr;       // (rank, node) of every copy with pv >= 0
+    int npv = 0;
     std::vector<size_t> node_off, elem_off;   // per rank
     ~DirectTables();
 };
@@ -83,6 +92,8 @@ struct Engine {
     int *d_dom0 = nullptr;      // index in d_doms of each group's first subdomain
     Tile *d_etiles = nullptr, *d_ntiles = nullptr, *d_qtiles = nullptr;
     int n_etiles = 0, n_ntiles = 0, n_qtiles = 0;
+    Tile *d_utiles = nullptr;   // node tiles + PosVel copy tiles of the direct update
+    int n_utiles = 0;
     int qeos_te = 256;          // elements per EOS tile (256 threads per workgroup)
     int tile_e = 256, tile_n = 256;
     // fused force + gather (z-marching column tiles) or the two-pass
@@ -98,9 +109,12 @@ struct Engine {
     bool fuse_update = true;
     bool store_forces = true;   // fx..fz written (exchange or keep_forces)
     // halo groups on the device_local backend: CommSBN summed inside the
-    // node update and CommMonoQ written by the kinematics kernel instead of
-    // pack/unpack launches (LULESH_NO_DIRECT=1 keeps the exchanges)
+    // node update, CommSyncPosVel written by each shared node's final
+    // writer, CommMonoQ written by the kinematics kernel, instead of
+    // pack/unpack launches (LULESH_NO_DIRECT=1 keeps the exchanges,
+    // LULESH_NO_DIRECT_POSVEL=1 keeps the PosVel one)
     bool direct = false;
+    bool direct_pv = false;
     bool umesh = false;         // members are unstructured meshes
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -123,6 +137,7 @@ int group_create_mesh(const lulesh_mesh *m, int nr, int balance, int cost, lules
 int group_direct(lulesh_group *g, DirectTables **out);
 bool group_device_local(const lulesh_group *g);
 void group_free(lulesh_group *g);
+void group_state_changed(lulesh_group *g);
 int group_reset_ctl(lulesh_group *g);
 int group_set_param(lulesh_group *g);
 int group_read_ctl(lulesh_group *g, GroupCtl *G, DomCtl *D0, hipStream_t s);

@@ -21,6 +21,8 @@ hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kch
 hipError_t launch_node_update(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 // CommSBN summed directly from the neighbours' partial forces + node update
 hipError_t launch_node_update_direct(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
+// the CommSyncPosVel final writers' snapshots of the current node state
+hipError_t launch_pv_snapshot(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_kinematics(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 // umesh: the unstructured-mesh variants (DomDev::u_*)
 // te elements per tile (<= bs threads per workgroup)

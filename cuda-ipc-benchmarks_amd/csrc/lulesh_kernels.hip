@@ -272,9 +272,13 @@ __device__ __forceinline__ void node_ijk(int nx, int nd, int &i, int &j, int &k)
 // CalcAccelerationForNodes + ApplyAccelerationBoundaryConditionsForNodes +
 // CalcVelocityForNodes + CalcPositionForNodes (lulesh.cc:1415-1550) for
 // node nd of subdomain D, symm from node_symm
-__device__ __forceinline__ void node_advance(const DomDev &D, const NodeState &src, const NodeState &dst,
-                                             int nd, unsigned symm, double fx, double fy, double fz,
-                                             double dt)
+struct NodeNew {
+    double x, y, z, xd, yd, zd;
+};
+
+__device__ __forceinline__ NodeNew node_advance(const DomDev &D, const NodeState &src, const NodeState &dst,
+                                                int nd, unsigned symm, double fx, double fy, double fz,
+                                                double dt)
 {
     const double one_over_nMass = 1. / D.nodalMass[nd];
     double ax = fx * one_over_nMass, ay = fy * one_over_nMass, az = fz * one_over_nMass;
@@ -296,6 +300,7 @@ __device__ __forceinline__ void node_advance(const DomDev &D, const NodeState &s
     dst.x[nd] = px;
     dst.y[nd] = py;
     dst.z[nd] = pz;
+    return NodeNew{px, py, pz, vx, vy, vz};
 }
 
 // ---------------------------------------------------------------------------
@@ -775,16 +780,20 @@ __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const T
 // receive-slot order onto its own partial (lulesh-comm.cc:848-1257), as
 // the reference's pack-then-unpack does, stores the sums as fx..fz and
 // advances.
-__global__ __launch_bounds__(256) void k_node_update_direct(const DomDev *doms, const Tile *tiles, int ntiles)
+//
+// With D.dir_pv, CommSyncPosVel too (lulesh-comm.cc:1261-1680): a copy
+// that the exchange would overwrite takes its final writer's update, which
+// it computes itself from the writer's force sum (the writer's partial and
+// the writer's receive-slot order), the writer's nodal mass and the
+// writer's pre-update state from the writer's snapshot of the previous
+// cycle; a copy that is some other copy's final writer stores its new
+// state in its own snapshot for the next cycle.
+__device__ __forceinline__ void sbn_sum(const DomDev *doms, const DomDev &D, int nd, double &fx, double &fy,
+                                        double &fz)
 {
-    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const DomDev &D = doms[t.dom];
-    if (group_stopped(*D.grp)) return;
-    const int nd = t.start + threadIdx.x;
-    if (nd >= D.numNode) return;
-    int i, j, k;
-    node_ijk(D.nx, nd, i, j, k);
-    double fx = D.fpx[nd], fy = D.fpy[nd], fz = D.fpz[nd];
+    fx = D.fpx[nd];
+    fy = D.fpy[nd];
+    fz = D.fpz[nd];
     const IdxSpan r = D.dir_node ? D.dir_node[nd] : IdxSpan{0, 0};
     for (int m = 0; m < r.count; ++m) {
         const DirRef s = D.dir_sbn[r.first + m];
@@ -793,11 +802,115 @@ __global__ __launch_bounds__(256) void k_node_update_direct(const DomDev *doms, 
         fy += S.fpy[s.idx];
         fz += S.fpz[s.idx];
     }
+}
+
+__device__ __forceinline__ void node_update_pv(const DomDev *doms, const DomDev &D, int nd, int pi);
+
+__global__ __launch_bounds__(256) void k_node_update_direct(const DomDev *doms, const Tile *tiles, int ntiles)
+{
+    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    if (t.pad) {
+        // a tile of PosVel copies: (rank, node) entries of the group
+        const DomDev &G0 = doms[t.dom];
+        if (group_stopped(*G0.grp) || (int)threadIdx.x >= t.ecount) return;
+        const DirRef c = G0.dir_pvlist[t.start + threadIdx.x];
+        const DomDev &D = doms[G0.dom_base + c.dom];
+        node_update_pv(doms, D, c.idx, D.dir_pv[c.idx]);
+        return;
+    }
+    const DomDev &D = doms[t.dom];
+    if (group_stopped(*D.grp)) return;
+    const int nd = t.start + threadIdx.x;
+    if (nd >= D.numNode) return;
+    if (D.dir_pv && D.dir_pv[nd] >= 0) return;   // in a PosVel tile
+    int i, j, k;
+    node_ijk(D.nx, nd, i, j, k);
+    double fx, fy, fz;
+    sbn_sum(doms, D, nd, fx, fy, fz);
     D.fx[nd] = fx;
     D.fy[nd] = fy;
     D.fz[nd] = fz;
     node_advance(D, ns_old(D), ns_new(D), nd, node_symm<false>(D, nd, i, j, k), fx, fy, fz,
                  D.ctl->deltatime);
+}
+
+// the update of a node copy that CommSyncPosVel touches
+__device__ __forceinline__ void node_update_pv(const DomDev *doms, const DomDev &D, int nd, int pi)
+{
+    int i, j, k;
+    node_ijk(D.nx, nd, i, j, k);
+    double fx, fy, fz;
+    sbn_sum(doms, D, nd, fx, fy, fz);
+    D.fx[nd] = fx;
+    D.fy[nd] = fy;
+    D.fz[nd] = fz;
+    const unsigned symm = node_symm<false>(D, nd, i, j, k);
+    const double dt = D.ctl->deltatime;
+    const PvNode P = D.dir_pvnode[pi];
+    const int par = D.ctl->cycle & 1;   // snapshots: read the other parity, write this one
+    NodeNew v;
+    if (P.fw_dom >= 0) {
+        // the same physical node in the writer: same symmetry planes
+        const DomDev &W = doms[D.dom_base + P.fw_dom];
+        double gx, gy, gz;
+        sbn_sum(doms, W, P.fw_idx, gx, gy, gz);
+        const double *old = W.pv_snap + (size_t)(par ^ 1) * 6 * W.pv_nslot + P.fw_slot;
+        const size_t ns = W.pv_nslot;
+        const double one_over_nMass = 1. / W.nodalMass[P.fw_idx];
+        double ax = gx * one_over_nMass, ay = gy * one_over_nMass, az = gz * one_over_nMass;
+        if (symm & 1u) ax = 0.0;
+        if (symm & 2u) ay = 0.0;
+        if (symm & 4u) az = 0.0;
+        double vx = old[3 * ns] + ax * dt;
+        if (fabs(vx) < Phys::u_cut) vx = 0.0;
+        double vy = old[4 * ns] + ay * dt;
+        if (fabs(vy) < Phys::u_cut) vy = 0.0;
+        double vz = old[5 * ns] + az * dt;
+        if (fabs(vz) < Phys::u_cut) vz = 0.0;
+        v = NodeNew{old[0] + vx * dt, old[ns] + vy * dt, old[2 * ns] + vz * dt, vx, vy, vz};
+        const NodeState N = ns_new(D);
+        N.x[nd] = v.x;
+        N.y[nd] = v.y;
+        N.z[nd] = v.z;
+        N.xd[nd] = v.xd;
+        N.yd[nd] = v.yd;
+        N.zd[nd] = v.zd;
+    } else {
+        v = node_advance(D, ns_old(D), ns_new(D), nd, symm, fx, fy, fz, dt);
+    }
+    if (P.own_slot >= 0) {
+        double *snap = D.pv_snap + (size_t)par * 6 * D.pv_nslot + P.own_slot;
+        const size_t ns = D.pv_nslot;
+        snap[0] = v.x;
+        snap[ns] = v.y;
+        snap[2 * ns] = v.z;
+        snap[3 * ns] = v.xd;
+        snap[4 * ns] = v.yd;
+        snap[5 * ns] = v.zd;
+    }
+}
+
+// the final writers' snapshots of the current node state, into the parity
+// of the last counted cycle (the one the next cycle reads)
+__global__ __launch_bounds__(256) void k_pv_snapshot(const DomDev *doms, const Tile *tiles, int ntiles)
+{
+    const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    const DomDev &D = doms[t.dom];
+    const int nd = t.start + threadIdx.x;
+    if (nd >= D.numNode || !D.dir_pv) return;
+    const int pi = D.dir_pv[nd];
+    if (pi < 0) return;
+    const int slot = D.dir_pvnode[pi].own_slot;
+    if (slot < 0) return;
+    const NodeState S = D.cur;
+    const size_t ns = D.pv_nslot;
+    double *snap = D.pv_snap + (size_t)(D.ctl->cycle & 1) * 6 * ns + slot;
+    snap[0] = S.x[nd];
+    snap[ns] = S.y[nd];
+    snap[2 * ns] = S.z[nd];
+    snap[3 * ns] = S.xd[nd];
+    snap[4 * ns] = S.yd[nd];
+    snap[5 * ns] = S.zd[nd];
 }
 
 // ---------------------------------------------------------------------------
@@ -1327,6 +1440,13 @@ hipError_t launch_node_update_direct(const DomDev *d, const Tile *t, int nt, int
 {
     if (nt == 0) return hipSuccess;
     hipLaunchKernelGGL(k_node_update_direct, dim3(nt), dim3(bs), 0, s, d, t, nt);
+    return hipGetLastError();
+}
+
+hipError_t launch_pv_snapshot(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s)
+{
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pv_snapshot, dim3(nt), dim3(bs), 0, s, d, t, nt);
     return hipGetLastError();
 }
 

@@ -81,11 +81,13 @@ def test_forces_not_kept_rejected(amd):
         g.field("fx")
 
 
-@pytest.mark.parametrize("env", [{}, {"LULESH_NO_DIRECT": "1"}, {"LULESH_FORCE_PATH": "twopass"}])
+@pytest.mark.parametrize("env", [{}, {"LULESH_NO_DIRECT": "1"}, {"LULESH_NO_DIRECT_POSVEL": "1"},
+                                 {"LULESH_FORCE_PATH": "twopass"}])
 def test_decomposed_paths_bitwise(amd, oracle_mod, monkeypatch, env):
     """2x2x2 subdomains: the direct halo steps (CommSBN summed in the node
-    update, CommMonoQ written by the kinematics kernel), the pack/unpack
-    exchanges, and the two-pass force path all give the oracle's state."""
+    update, CommSyncPosVel written by each shared node's final writer,
+    CommMonoQ written by the kinematics kernel), the pack/unpack exchanges,
+    and the two-pass force path all give the oracle's state."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     o = oracle_mod.Oracle(5, tp=2, cbrt_mode=oracle_mod.CBRT_EXACT)
@@ -102,4 +104,30 @@ def test_decomposed_paths_bitwise(amd, oracle_mod, monkeypatch, env):
     assert g.run() == 0
     for r in range(8):
         for f in ["fx", "fy", "fz"]:
+            assert_bitwise(g.field(f, r), o.field(f, r), f"{env} rank {r} {f}")
+
+
+@pytest.mark.parametrize("env", [{}, {"LULESH_NO_DIRECT_POSVEL": "1"}])
+def test_decomposed_inconsistent_copies(amd, oracle_mod, monkeypatch, env):
+    """Host writes that leave the copies of shared nodes different (one
+    subdomain's velocities scaled): CommSyncPosVel's final writers then
+    carry values the other copies did not compute, and every copy must
+    still end as the exchange leaves it."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    o = oracle_mod.Oracle(5, tp=2, cbrt_mode=oracle_mod.CBRT_EXACT)
+    g = amd.Group(5, tp=2)
+    o.run(40)
+    g.set_max_cycles(40)
+    assert g.run() == 0
+    for r in (0, 5):
+        for f in ["xd", "y"]:
+            v = o.field(f, r) * (1.0 + 1e-9 * (r + 1))
+            o.set_field(f, v, r)
+            g.domains[r].set_field(f, v)
+    o.run(25)
+    g.set_max_cycles(65)
+    assert g.run() == 0
+    for r in range(8):
+        for f in NODE + ["e", "p", "q", "v"]:
             assert_bitwise(g.field(f, r), o.field(f, r), f"{env} rank {r} {f}")
