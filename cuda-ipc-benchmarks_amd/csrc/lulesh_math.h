@@ -14,12 +14,19 @@ namespace lulesh {
 // Correctly rounded cube root.
 //
 // |x| = M 2^E with M a 53-bit integer; t = floor((E-104)/3), k = E - 3t in
-// {104,105,106}, so cbrt(|x|) = cbrt(M 2^k) 2^t with cbrt(M 2^k) in
-// [2^52, 2^53].  A float seed refined by two Newton steps in double gives
-// the integer Y within a few units; the exact midpoint test
-// (2Y-1)^3 < 8 M 2^k < (2Y+1)^3 in 192-bit integer arithmetic then fixes
-// Y.  Ties cannot occur (odd cube vs. even right-hand side), so Y 2^t is
-// the correctly rounded cube root.
+// {104,105,106}, so cbrt(|x|) = cbrt(M 2^k) 2^t with cbrt(M 2^k) = 2^52
+// cbrt(g) in [2^52, 2^53], g = M 2^(k-156) in [1, 8).
+//
+// A float seed and one Newton step (float reciprocal of the derivative)
+// give y within 2^-44 of cbrt(g).  Rounded to an integer in units of
+// 2^-52, Yd = rint(y 2^52); the residual g - (Yd 2^-52)^3, formed with
+// FMA splits of the square and cube (g minus the leading part is exact by
+// Sterbenz), gives f = 2^52 (cbrt(g) - Yd 2^-52) to about 2^-40 (the
+// neglected second-order term is below 2^-34).  Unless f lies within
+// 2^-20 of a half-integer, Yd + rint(f) is the correctly rounded result;
+// otherwise the exact midpoint test (2Y-1)^3 < 8 M 2^k < (2Y+1)^3 in
+// 192-bit integer arithmetic fixes Y.  Ties cannot occur (odd cube vs.
+// even right-hand side), so Y 2^t is the correctly rounded cube root.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int cmp_cube192(uint64_t A, uint64_t n2, uint64_t n1)
 {
@@ -46,22 +53,30 @@ __device__ __forceinline__ double cbrt_cr(double x)
     const int E = e - 53;
     const int t = (E >= 104) ? (E - 104) / 3 : -((104 - E + 2) / 3);
     const int k = E - 3 * t;                              // 104..106
-    // seed: cbrt(M 2^k) = 2^52 cbrt(g), g = m 2^(k-103) in [1, 8)
-    const double g = ldexp(m, k - 103);
-    double y = (double)cbrtf((float)g);
-    y = y - (y * y * y - g) / (3.0 * y * y);
-    y = y - (y * y * y - g) / (3.0 * y * y);
-    uint64_t Y = (uint64_t)__double2ull_rn(ldexp(y, 52));
-    const int s = k + 3;                                  // 107..109
-    const uint64_t n1 = M << (s - 64);
-    const uint64_t n2 = M >> (128 - s);
-    for (int it = 0; it < 8; ++it) {
-        if (cmp_cube192(2 * Y + 1, n2, n1) > 0) { ++Y; continue; }
-        if (cmp_cube192(2 * Y - 1, n2, n1) < 0) { --Y; continue; }
-        break;
+    const double g = ldexp(m, k - 103);                   // [1, 8)
+    const float yf = cbrtf((float)g);
+    const double y0 = (double)yf;
+    const double y = y0 - (y0 * y0 * y0 - g) * (double)(1.0f / (3.0f * yf * yf));
+    const double Yd = rint(y * 0x1p52);
+    const double yr = Yd * 0x1p-52;
+    const double p = yr * yr, pe = fma(yr, yr, -p);       // yr^2 = p + pe
+    const double c = p * yr, ce = fma(p, yr, -c);         // p yr = c + ce
+    const double r = (g - c) - fma(pe, yr, ce);           // g - yr^3
+    const double f = r / (3.0 * p) * 0x1p52;
+    const double kf = rint(f);
+    uint64_t Y = (uint64_t)((int64_t)Yd + (int64_t)kf);
+    if (!(fabs(f - kf) < 0.5 - 0x1p-20)) {
+        const int s = k + 3;                              // 107..109
+        const uint64_t n1 = M << (s - 64);
+        const uint64_t n2 = M >> (128 - s);
+        for (int it = 0; it < 8; ++it) {
+            if (cmp_cube192(2 * Y + 1, n2, n1) > 0) { ++Y; continue; }
+            if (cmp_cube192(2 * Y - 1, n2, n1) < 0) { --Y; continue; }
+            break;
+        }
     }
-    const double r = ldexp((double)Y, t);
-    return neg ? -r : r;
+    const double res = ldexp((double)Y, t);
+    return neg ? -res : res;
 }
 
 // ---------------------------------------------------------------------------

@@ -30,6 +30,29 @@ def test_probe_cbrt_matches_exact(amd, oracle_mod):
     assert_bitwise(got, want, "cbrt")
 
 
+def test_probe_cbrt_hard_cases(amd, oracle_mod):
+    # inputs whose cube roots lie next to a rounding midpoint (the nearest
+    # doubles to ((2Y+1)/2^53)^3 and their neighbours, at many exponents)
+    # and exact cubes: the cases the fast rounding decision must hand to
+    # the exact midpoint test or get right by itself
+    from fractions import Fraction
+    rng = np.random.default_rng(3)
+    xs = []
+    for _ in range(4000):
+        y = int(rng.integers(1 << 52, 1 << 53))
+        mid = float(Fraction((2 * y + 1) ** 3, 2 ** 159))
+        sh = 3 * int(rng.integers(-300, 300))
+        for v in (mid, np.nextafter(mid, 0.0), np.nextafter(mid, 16.0)):
+            xs.append(np.ldexp(v, sh))
+    for _ in range(4000):
+        a = int(rng.integers(1, 1 << 17))
+        xs.append(np.ldexp(float(a) ** 3, 3 * int(rng.integers(-300, 300))))
+    x = np.array([v for v in xs if np.isfinite(v) and v > 0])
+    got = amd.probe_cbrt(x)
+    want = np.array([oracle_mod.cbrt_exact(v) for v in x])
+    assert_bitwise(got, want, "cbrt near midpoints / exact cubes")
+
+
 def test_probe_sqrt_div_ieee(amd):
     rng = np.random.default_rng(2)
     a = np.exp(rng.uniform(-300, 300, 50000))
