@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
 
     double x[8], y[8], z[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) { x[c] = S.x[n[c]]; y[c] = S.y[n[c]]; z[c] = S.z[n[c]]; }
+    for (int c = 0; c < 8; ++c) { x[c] = gp(S.x)[n[c]]; y[c] = gp(S.y)[n[c]]; z[c] = gp(S.z)[n[c]]; }
 
     // stress part
     {
@@ -196,19 +196,19 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
         if (J.volume <= 0.0) raise_status(G, -1);      // lulesh.cc:1340-1349
         double bx[8], by[8], bz[8];
         node_normals(x, y, z, bx, by, bz);
-        const double sig = -D.p[el] - D.q[el];         // InitStressTermsForElems
+        const double sig = -gp(D.p)[el] - gp(D.q)[el];         // InitStressTermsForElems
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            D.corner[(0 * 8 + c) * E + el] = -(sig * bx[c]);
-            D.corner[(1 * 8 + c) * E + el] = -(sig * by[c]);
-            D.corner[(2 * 8 + c) * E + el] = -(sig * bz[c]);
+            gp(D.corner)[(0 * 8 + c) * E + el] = -(sig * bx[c]);
+            gp(D.corner)[(1 * 8 + c) * E + el] = -(sig * by[c]);
+            gp(D.corner)[(2 * 8 + c) * E + el] = -(sig * bz[c]);
         }
     }
 
     // hourglass part
-    const double vv = D.v[el];
+    const double vv = gp(D.v)[el];
     if (vv <= 0.0) raise_status(G, -1);                // lulesh.cc:1263-1265
-    const double determ = D.volo[el] * vv;
+    const double determ = gp(D.volo)[el] * vv;
     double dvdx[8], dvdy[8], dvdz[8];
     volume_derivs(x, y, z, dvdx, dvdy, dvdz);
     const double volinv = 1.0 / determ;
@@ -226,8 +226,8 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
         for (int c = 0; c < 8; ++c)
             hg[c][m] = hg_gamma(m, c) - volinv * (dvdx[c] * hx + dvdy[c] * hy + dvdz[c] * hz);
     }
-    const double coefficient = D.hgcoef[el];
-    const double *vel[3] = {S.xd, S.yd, S.zd};
+    const double coefficient = gp(D.hgcoef)[el];
+    const __attribute__((address_space(1))) double *vel[3] = {gp(S.xd), gp(S.yd), gp(S.zd)};
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
         double vl[8], h[4];
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c)
-            D.corner[((3 + comp) * 8 + c) * E + el] =
+            gp(D.corner)[((3 + comp) * 8 + c) * E + el] =
                 coefficient * (hg[c][0] * h[0] + hg[c][1] * h[1] + hg[c][2] * h[2] + hg[c][3] * h[3]);
     }
 }
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
 template <bool U>
 __device__ __forceinline__ unsigned node_symm(const DomDev &D, int nd, int i, int j, int k)
 {
-    if (U) return D.u_nsymm[nd];
+    if (U) return gp(D.u_nsymm)[nd];
     return (D.col == 0 && i == 0 ? 1u : 0u) | (D.row == 0 && j == 0 ? 2u : 0u) |
            (D.plane == 0 && k == 0 ? 4u : 0u);
 }
@@ -280,26 +280,26 @@ __device__ __forceinline__ NodeNew node_advance(const DomDev &D, const NodeState
                                                 int nd, unsigned symm, double fx, double fy, double fz,
                                                 double dt)
 {
-    const double one_over_nMass = 1. / D.nodalMass[nd];
+    const double one_over_nMass = 1. / gp(D.nodalMass)[nd];
     double ax = fx * one_over_nMass, ay = fy * one_over_nMass, az = fz * one_over_nMass;
     if (symm & 1u) ax = 0.0;
     if (symm & 2u) ay = 0.0;
     if (symm & 4u) az = 0.0;
-    double vx = src.xd[nd] + ax * dt;
+    double vx = gp(src.xd)[nd] + ax * dt;
     if (fabs(vx) < Phys::u_cut) vx = 0.0;
-    double vy = src.yd[nd] + ay * dt;
+    double vy = gp(src.yd)[nd] + ay * dt;
     if (fabs(vy) < Phys::u_cut) vy = 0.0;
-    double vz = src.zd[nd] + az * dt;
+    double vz = gp(src.zd)[nd] + az * dt;
     if (fabs(vz) < Phys::u_cut) vz = 0.0;
-    const double px = src.x[nd] + vx * dt;
-    const double py = src.y[nd] + vy * dt;
-    const double pz = src.z[nd] + vz * dt;
-    dst.xd[nd] = vx;
-    dst.yd[nd] = vy;
-    dst.zd[nd] = vz;
-    dst.x[nd] = px;
-    dst.y[nd] = py;
-    dst.z[nd] = pz;
+    const double px = gp(src.x)[nd] + vx * dt;
+    const double py = gp(src.y)[nd] + vy * dt;
+    const double pz = gp(src.z)[nd] + vz * dt;
+    gp(dst.xd)[nd] = vx;
+    gp(dst.yd)[nd] = vy;
+    gp(dst.zd)[nd] = vz;
+    gp(dst.x)[nd] = px;
+    gp(dst.y)[nd] = py;
+    gp(dst.z)[nd] = pz;
     return NodeNew{px, py, pz, vx, vy, vz};
 }
 
@@ -329,7 +329,7 @@ __device__ __forceinline__ void load_elem_xyz(const NodeState &S, const int n[8]
                                               double z[8])
 {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) { x[c] = S.x[n[c]]; y[c] = S.y[n[c]]; z[c] = S.z[n[c]]; }
+    for (int c = 0; c < 8; ++c) { x[c] = gp(S.x)[n[c]]; y[c] = gp(S.y)[n[c]]; z[c] = gp(S.z)[n[c]]; }
 }
 
 // stress corner forces -(sigma B) from the face normals
@@ -458,13 +458,13 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         if (kn < 0 || kn > nx) return;
         if (own_in) {
             const int nd = kn * s2 + own_j * s1 + own_i;
-            pf_own[0] = S.x[nd]; pf_own[1] = S.y[nd]; pf_own[2] = S.z[nd];
-            pf_own[3] = S.xd[nd]; pf_own[4] = S.yd[nd]; pf_own[5] = S.zd[nd];
+            pf_own[0] = gp(S.x)[nd]; pf_own[1] = gp(S.y)[nd]; pf_own[2] = gp(S.z)[nd];
+            pf_own[3] = gp(S.xd)[nd]; pf_own[4] = gp(S.yd)[nd]; pf_own[5] = gp(S.zd)[nd];
         }
         if (xt_sl >= 0) {
             const int nd = kn * s2 + xt_j * s1 + xt_i;
-            pf_xt[0] = S.x[nd]; pf_xt[1] = S.y[nd]; pf_xt[2] = S.z[nd];
-            pf_xt[3] = S.xd[nd]; pf_xt[4] = S.yd[nd]; pf_xt[5] = S.zd[nd];
+            pf_xt[0] = gp(S.x)[nd]; pf_xt[1] = gp(S.y)[nd]; pf_xt[2] = gp(S.z)[nd];
+            pf_xt[3] = gp(S.xd)[nd]; pf_xt[4] = gp(S.yd)[nd]; pf_xt[5] = gp(S.zd)[nd];
         }
     };
     auto stage = [&](int kn) {           // write the fetched plane kn to LDS
@@ -482,7 +482,8 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     auto fetch_elem = [&](int c, double out[5]) {
         if (!(col_ok && c >= 0 && c < nx)) return;
         const int el = (c * nx + b) * nx + a;
-        out[0] = D.p[el]; out[1] = D.q[el]; out[2] = D.v[el]; out[3] = D.volo[el]; out[4] = D.hgcoef[el];
+        out[0] = gp(D.p)[el]; out[1] = gp(D.q)[el]; out[2] = gp(D.v)[el]; out[3] = gp(D.volo)[el];
+        out[4] = gp(D.hgcoef)[el];
     };
 
     fetch(k0 - 1);
@@ -502,7 +503,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         if (c + 2 <= k1) fetch(c + 2);
         fetch_elem(c + 1, ef_next);
         double nmass = 0.0;
-        if (advance && node_ok && c >= k0) nmass = D.nodalMass[c * s2 + nj * s1 + ni];
+        if (advance && node_ok && c >= k0) nmass = gp(D.nodalMass)[c * s2 + nj * s1 + ni];
 
         // ---- stress corners ----
         if (el_ok) {
@@ -605,9 +606,9 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             if (c >= k0) {
                 const int nd = c * s2 + nj * s1 + ni;
                 if (store_f) {
-                    D.fpx[nd] = f[0];
-                    D.fpy[nd] = f[1];
-                    D.fpz[nd] = f[2];
+                    gp(D.fpx)[nd] = f[0];
+                    gp(D.fpy)[nd] = f[1];
+                    gp(D.fpz)[nd] = f[2];
                 }
                 if (advance) {
                     // the node is complete: advance it from its old state
@@ -625,12 +626,12 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                     double vz = Lo[5][own_sl] + az * dt;
                     if (fabs(vz) < Phys::u_cut) vz = 0.0;
                     const NodeState N = ns_new(D);
-                    N.xd[nd] = vx;
-                    N.yd[nd] = vy;
-                    N.zd[nd] = vz;
-                    N.x[nd] = Lo[0][own_sl] + vx * dt;
-                    N.y[nd] = Lo[1][own_sl] + vy * dt;
-                    N.z[nd] = Lo[2][own_sl] + vz * dt;
+                    gp(N.xd)[nd] = vx;
+                    gp(N.yd)[nd] = vy;
+                    gp(N.zd)[nd] = vz;
+                    gp(N.x)[nd] = Lo[0][own_sl] + vx * dt;
+                    gp(N.y)[nd] = Lo[1][own_sl] + vy * dt;
+                    gp(N.z)[nd] = Lo[2][own_sl] + vz * dt;
                 }
             }
         }
@@ -655,7 +656,7 @@ template <bool FUSE_UPDATE>
 __device__ __forceinline__ void node_gather_u(const DomDev &D, int nd)
 {
     const size_t E = (size_t)D.numElem;
-    const int q0 = D.u_nstart[nd], cnt = D.u_nstart[nd + 1] - q0;
+    const int q0 = gp(D.u_nstart)[nd], cnt = gp(D.u_nstart)[nd + 1] - q0;
     double sx = 0.0, sy = 0.0, sz = 0.0, hx = 0.0, hy = 0.0, hz = 0.0;
     // corners in batches of 8 (a hexahedral mesh node has at most 8 unless
     // the mesh is irregular): every load of a batch is issued before the
@@ -664,7 +665,7 @@ __device__ __forceinline__ void node_gather_u(const DomDev &D, int nd)
         size_t off[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int ent = b + t < cnt ? D.u_ncorner[q0 + b + t] : 0;
+            const int ent = b + t < cnt ? gp(D.u_ncorner)[q0 + b + t] : 0;
             off[t] = (size_t)(ent & 7) * E + (size_t)(ent >> 3);
         }
         double v[6][8];
@@ -672,7 +673,7 @@ __device__ __forceinline__ void node_gather_u(const DomDev &D, int nd)
         for (int t = 0; t < 8; ++t) {
             const bool ok = b + t < cnt;
 #pragma unroll
-            for (int comp = 0; comp < 6; ++comp) v[comp][t] = ok ? D.corner[comp * 8 * E + off[t]] : 0.0;
+            for (int comp = 0; comp < 6; ++comp) v[comp][t] = ok ? gp(D.corner)[comp * 8 * E + off[t]] : 0.0;
         }
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -694,9 +695,9 @@ __device__ __forceinline__ void node_gather_u(const DomDev &D, int nd)
         node_advance(D, ns_old(D), ns_new(D), nd, node_symm<true>(D, nd, 0, 0, 0), fx, fy, fz,
                      D.ctl->deltatime);
     } else {
-        D.fpx[nd] = fx;
-        D.fpy[nd] = fy;
-        D.fpz[nd] = fz;
+        gp(D.fpx)[nd] = fx;
+        gp(D.fpy)[nd] = fy;
+        gp(D.fpz)[nd] = fz;
     }
 }
 
@@ -736,27 +737,27 @@ __global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const T
         if (eidx[q] < 0) continue;
         const size_t e = (size_t)eidx[q];
         const int c = ccorner[q];
-        sx += D.corner[(0 * 8 + c) * E + e];
-        sy += D.corner[(1 * 8 + c) * E + e];
-        sz += D.corner[(2 * 8 + c) * E + e];
+        sx += gp(D.corner)[(0 * 8 + c) * E + e];
+        sy += gp(D.corner)[(1 * 8 + c) * E + e];
+        sz += gp(D.corner)[(2 * 8 + c) * E + e];
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         if (eidx[q] < 0) continue;
         const size_t e = (size_t)eidx[q];
         const int c = ccorner[q];
-        hx += D.corner[(3 * 8 + c) * E + e];
-        hy += D.corner[(4 * 8 + c) * E + e];
-        hz += D.corner[(5 * 8 + c) * E + e];
+        hx += gp(D.corner)[(3 * 8 + c) * E + e];
+        hy += gp(D.corner)[(4 * 8 + c) * E + e];
+        hz += gp(D.corner)[(5 * 8 + c) * E + e];
     }
     const double fx = sx + hx, fy = sy + hy, fz = sz + hz;
     if (FUSE_UPDATE) {
         node_advance(D, ns_old(D), ns_new(D), nd, node_symm<false>(D, nd, i, j, k), fx, fy, fz,
                      D.ctl->deltatime);
     } else {
-        D.fpx[nd] = fx;
-        D.fpy[nd] = fy;
-        D.fpz[nd] = fz;
+        gp(D.fpx)[nd] = fx;
+        gp(D.fpy)[nd] = fy;
+        gp(D.fpz)[nd] = fz;
     }
 }
 
@@ -770,7 +771,7 @@ __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const T
     if (nd >= D.numNode) return;
     int i = 0, j = 0, k = 0;
     if (!U) node_ijk(D.nx, nd, i, j, k);
-    node_advance(D, ns_old(D), ns_new(D), nd, node_symm<U>(D, nd, i, j, k), D.fx[nd], D.fy[nd], D.fz[nd],
+    node_advance(D, ns_old(D), ns_new(D), nd, node_symm<U>(D, nd, i, j, k), gp(D.fx)[nd], gp(D.fy)[nd], gp(D.fz)[nd],
                  D.ctl->deltatime);
 }
 
@@ -791,16 +792,16 @@ __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const T
 __device__ __forceinline__ void sbn_sum(const DomDev *doms, const DomDev &D, int nd, double &fx, double &fy,
                                         double &fz)
 {
-    fx = D.fpx[nd];
-    fy = D.fpy[nd];
-    fz = D.fpz[nd];
+    fx = gp(D.fpx)[nd];
+    fy = gp(D.fpy)[nd];
+    fz = gp(D.fpz)[nd];
     const IdxSpan r = D.dir_node ? D.dir_node[nd] : IdxSpan{0, 0};
     for (int m = 0; m < r.count; ++m) {
         const DirRef s = D.dir_sbn[r.first + m];
         const DomDev &S = doms[D.dom_base + s.dom];
-        fx += S.fpx[s.idx];
-        fy += S.fpy[s.idx];
-        fz += S.fpz[s.idx];
+        fx += gp(S.fpx)[s.idx];
+        fy += gp(S.fpy)[s.idx];
+        fz += gp(S.fpz)[s.idx];
     }
 }
 
@@ -815,21 +816,21 @@ __global__ __launch_bounds__(256) void k_node_update_direct(const DomDev *doms, 
         if (group_stopped(*G0.grp) || (int)threadIdx.x >= t.ecount) return;
         const DirRef c = G0.dir_pvlist[t.start + threadIdx.x];
         const DomDev &D = doms[G0.dom_base + c.dom];
-        node_update_pv(doms, D, c.idx, D.dir_pv[c.idx]);
+        node_update_pv(doms, D, c.idx, gp(D.dir_pv)[c.idx]);
         return;
     }
     const DomDev &D = doms[t.dom];
     if (group_stopped(*D.grp)) return;
     const int nd = t.start + threadIdx.x;
     if (nd >= D.numNode) return;
-    if (D.dir_pv && D.dir_pv[nd] >= 0) return;   // in a PosVel tile
+    if (D.dir_pv && gp(D.dir_pv)[nd] >= 0) return;   // in a PosVel tile
     int i, j, k;
     node_ijk(D.nx, nd, i, j, k);
     double fx, fy, fz;
     sbn_sum(doms, D, nd, fx, fy, fz);
-    D.fx[nd] = fx;
-    D.fy[nd] = fy;
-    D.fz[nd] = fz;
+    gp(D.fx)[nd] = fx;
+    gp(D.fy)[nd] = fy;
+    gp(D.fz)[nd] = fz;
     node_advance(D, ns_old(D), ns_new(D), nd, node_symm<false>(D, nd, i, j, k), fx, fy, fz,
                  D.ctl->deltatime);
 }
@@ -841,9 +842,9 @@ __device__ __forceinline__ void node_update_pv(const DomDev *doms, const DomDev 
     node_ijk(D.nx, nd, i, j, k);
     double fx, fy, fz;
     sbn_sum(doms, D, nd, fx, fy, fz);
-    D.fx[nd] = fx;
-    D.fy[nd] = fy;
-    D.fz[nd] = fz;
+    gp(D.fx)[nd] = fx;
+    gp(D.fy)[nd] = fy;
+    gp(D.fz)[nd] = fz;
     const unsigned symm = node_symm<false>(D, nd, i, j, k);
     const double dt = D.ctl->deltatime;
     const PvNode P = D.dir_pvnode[pi];
@@ -854,9 +855,9 @@ __device__ __forceinline__ void node_update_pv(const DomDev *doms, const DomDev 
         const DomDev &W = doms[D.dom_base + P.fw_dom];
         double gx, gy, gz;
         sbn_sum(doms, W, P.fw_idx, gx, gy, gz);
-        const double *old = W.pv_snap + (size_t)(par ^ 1) * 6 * W.pv_nslot + P.fw_slot;
+        const auto old = gp(W.pv_snap) + (size_t)(par ^ 1) * 6 * W.pv_nslot + P.fw_slot;
         const size_t ns = W.pv_nslot;
-        const double one_over_nMass = 1. / W.nodalMass[P.fw_idx];
+        const double one_over_nMass = 1. / gp(W.nodalMass)[P.fw_idx];
         double ax = gx * one_over_nMass, ay = gy * one_over_nMass, az = gz * one_over_nMass;
         if (symm & 1u) ax = 0.0;
         if (symm & 2u) ay = 0.0;
@@ -869,17 +870,17 @@ __device__ __forceinline__ void node_update_pv(const DomDev *doms, const DomDev 
         if (fabs(vz) < Phys::u_cut) vz = 0.0;
         v = NodeNew{old[0] + vx * dt, old[ns] + vy * dt, old[2 * ns] + vz * dt, vx, vy, vz};
         const NodeState N = ns_new(D);
-        N.x[nd] = v.x;
-        N.y[nd] = v.y;
-        N.z[nd] = v.z;
-        N.xd[nd] = v.xd;
-        N.yd[nd] = v.yd;
-        N.zd[nd] = v.zd;
+        gp(N.x)[nd] = v.x;
+        gp(N.y)[nd] = v.y;
+        gp(N.z)[nd] = v.z;
+        gp(N.xd)[nd] = v.xd;
+        gp(N.yd)[nd] = v.yd;
+        gp(N.zd)[nd] = v.zd;
     } else {
         v = node_advance(D, ns_old(D), ns_new(D), nd, symm, fx, fy, fz, dt);
     }
     if (P.own_slot >= 0) {
-        double *snap = D.pv_snap + (size_t)par * 6 * D.pv_nslot + P.own_slot;
+        const auto snap = gp(D.pv_snap) + (size_t)par * 6 * D.pv_nslot + P.own_slot;
         const size_t ns = D.pv_nslot;
         snap[0] = v.x;
         snap[ns] = v.y;
@@ -898,19 +899,19 @@ __global__ __launch_bounds__(256) void k_pv_snapshot(const DomDev *doms, const T
     const DomDev &D = doms[t.dom];
     const int nd = t.start + threadIdx.x;
     if (nd >= D.numNode || !D.dir_pv) return;
-    const int pi = D.dir_pv[nd];
+    const int pi = gp(D.dir_pv)[nd];
     if (pi < 0) return;
     const int slot = D.dir_pvnode[pi].own_slot;
     if (slot < 0) return;
     const NodeState S = D.cur;
     const size_t ns = D.pv_nslot;
-    double *snap = D.pv_snap + (size_t)(D.ctl->cycle & 1) * 6 * ns + slot;
-    snap[0] = S.x[nd];
-    snap[ns] = S.y[nd];
-    snap[2 * ns] = S.z[nd];
-    snap[3 * ns] = S.xd[nd];
-    snap[4 * ns] = S.yd[nd];
-    snap[5 * ns] = S.zd[nd];
+    const auto snap = gp(D.pv_snap) + (size_t)(D.ctl->cycle & 1) * 6 * ns + slot;
+    snap[0] = gp(S.x)[nd];
+    snap[ns] = gp(S.y)[nd];
+    snap[2 * ns] = gp(S.z)[nd];
+    snap[3 * ns] = gp(S.xd)[nd];
+    snap[4 * ns] = gp(S.yd)[nd];
+    snap[5 * ns] = gp(S.zd)[nd];
 }
 
 // ---------------------------------------------------------------------------
@@ -924,12 +925,12 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
                                             const double x[8], const double y[8], const double z[8],
                                             const double xv[8], const double yv[8], const double zv[8])
 {
-    const double volo = D.volo[el];
+    const double volo = gp(D.volo)[el];
     const double volume = elem_volume(x, y, z);
     const double vnew = volume / volo;
-    D.vnew[el] = vnew;
-    D.delv[el] = vnew - D.v[el];
-    D.arealg[el] = char_length(x, y, z, volume);
+    gp(D.vnew)[el] = vnew;
+    gp(D.delv)[el] = vnew - gp(D.v)[el];
+    gp(D.arealg)[el] = char_length(x, y, z, volume);
     if (vnew <= 0.0) raise_status(G, -1);              // lulesh.cc:2037-2049
 
     {
@@ -952,7 +953,7 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
                                        b[1][2] * (yv[2] - yv[4]) + b[1][3] * (yv[3] - yv[5]));
         const double dzz = inv_detJ * (b[2][0] * (zv[0] - zv[6]) + b[2][1] * (zv[1] - zv[7]) +
                                        b[2][2] * (zv[2] - zv[4]) + b[2][3] * (zv[3] - zv[5]));
-        D.vdov[el] = dxx + dyy + dzz;
+        gp(D.vdov)[el] = dxx + dyy + dzz;
     }
 
     // MonoQ gradients at the end-of-step positions
@@ -971,31 +972,31 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
     double ax, ay, az, dxv, dyv, dzv;
 
     ax = dyi * dzj - dzi * dyj; ay = dzi * dxj - dxi * dzj; az = dxi * dyj - dyi * dxj;
-    D.delx_zeta[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    gp(D.delx_zeta)[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
     ax *= norm; ay *= norm; az *= norm;
     dxv = 0.25 * ((xv[4] + xv[5] + xv[6] + xv[7]) - (xv[0] + xv[1] + xv[2] + xv[3]));
     dyv = 0.25 * ((yv[4] + yv[5] + yv[6] + yv[7]) - (yv[0] + yv[1] + yv[2] + yv[3]));
     dzv = 0.25 * ((zv[4] + zv[5] + zv[6] + zv[7]) - (zv[0] + zv[1] + zv[2] + zv[3]));
     const double dv_zeta = ax * dxv + ay * dyv + az * dzv;
-    D.delv_zeta[el] = dv_zeta;
+    gp(D.delv_zeta)[el] = dv_zeta;
 
     ax = dyj * dzk - dzj * dyk; ay = dzj * dxk - dxj * dzk; az = dxj * dyk - dyj * dxk;
-    D.delx_xi[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    gp(D.delx_xi)[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
     ax *= norm; ay *= norm; az *= norm;
     dxv = 0.25 * ((xv[1] + xv[2] + xv[6] + xv[5]) - (xv[0] + xv[3] + xv[7] + xv[4]));
     dyv = 0.25 * ((yv[1] + yv[2] + yv[6] + yv[5]) - (yv[0] + yv[3] + yv[7] + yv[4]));
     dzv = 0.25 * ((zv[1] + zv[2] + zv[6] + zv[5]) - (zv[0] + zv[3] + zv[7] + zv[4]));
     const double dv_xi = ax * dxv + ay * dyv + az * dzv;
-    D.delv_xi[el] = dv_xi;
+    gp(D.delv_xi)[el] = dv_xi;
 
     ax = dyk * dzi - dzk * dyi; ay = dzk * dxi - dxk * dzi; az = dxk * dyi - dyk * dxi;
-    D.delx_eta[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    gp(D.delx_eta)[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
     ax *= norm; ay *= norm; az *= norm;
     dxv = -0.25 * ((xv[0] + xv[1] + xv[5] + xv[4]) - (xv[3] + xv[2] + xv[6] + xv[7]));
     dyv = -0.25 * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
     dzv = -0.25 * ((zv[0] + zv[1] + zv[5] + zv[4]) - (zv[3] + zv[2] + zv[6] + zv[7]));
     const double dv_eta = ax * dxv + ay * dyv + az * dzv;
-    D.delv_eta[el] = dv_eta;
+    gp(D.delv_eta)[el] = dv_eta;
 
     // CommMonoQ done in place (device_local backend): an element on a
     // shared face fills its neighbours' ghost slots with its three
@@ -1007,9 +1008,9 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
             for (int m = 0; m < r.count; ++m) {
                 const DirRef g = D.dir_ghost[r.first + m];
                 const DomDev &T = doms[D.dom_base + g.dom];
-                T.delv_xi[g.idx] = dv_xi;
-                T.delv_eta[g.idx] = dv_eta;
-                T.delv_zeta[g.idx] = dv_zeta;
+                gp(T.delv_xi)[g.idx] = dv_xi;
+                gp(T.delv_eta)[g.idx] = dv_eta;
+                gp(T.delv_zeta)[g.idx] = dv_zeta;
             }
         }
     }
@@ -1032,8 +1033,8 @@ __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Ti
     double x[8], y[8], z[8], xv[8], yv[8], zv[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-        x[c] = S.x[n[c]]; y[c] = S.y[n[c]]; z[c] = S.z[n[c]];
-        xv[c] = S.xd[n[c]]; yv[c] = S.yd[n[c]]; zv[c] = S.zd[n[c]];
+        x[c] = gp(S.x)[n[c]]; y[c] = gp(S.y)[n[c]]; z[c] = gp(S.z)[n[c]];
+        xv[c] = gp(S.xd)[n[c]]; yv[c] = gp(S.yd)[n[c]]; zv[c] = gp(S.zd)[n[c]];
     }
     kin_element(doms, D, G, el, dt, x, y, z, xv, yv, zv);
 }
@@ -1059,8 +1060,9 @@ __device__ __forceinline__ double monoq_phi(double self, double dm, double dp)
 // on a symmetry plane, 0 on a free surface, ghost plane on a subdomain face
 // (the elemBC / lxim.. logic of SetupBoundaryConditions,
 // lulesh-init.cc:576-630)
-__device__ __forceinline__ double face_nbr(const double *g, int el, bool at_edge, int nbr,
-                                           int ghost_base, int ghost_off, bool symm_side)
+template <typename P>
+__device__ __forceinline__ double face_nbr(P g, int el, bool at_edge, int nbr, int ghost_base, int ghost_off,
+                                           bool symm_side)
 {
     if (!at_edge) return g[nbr];
     if (ghost_base >= 0) return g[ghost_base + ghost_off];
@@ -1071,7 +1073,8 @@ __device__ __forceinline__ double face_nbr(const double *g, int el, bool at_edge
 // face-neighbour list (CalcMonotonicQRegionForElems' switch,
 // lulesh.cc:2307-2400): neighbour for COMM or no bit, own value for SYMM,
 // 0 for FREE and any other combination
-__device__ __forceinline__ double face_nbr_u(const double *g, int el, int nbr, int bits, int symm, int comm)
+template <typename P>
+__device__ __forceinline__ double face_nbr_u(P g, int el, int nbr, int bits, int symm, int comm)
 {
     if (bits == 0 || bits == comm) return g[nbr];
     if (bits == symm) return g[el];
@@ -1196,47 +1199,47 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     double vdov = 0.0;
     if (live) {
         // --- MonoQ region ---
-        const double gxi = D.delv_xi[el], geta = D.delv_eta[el], gzeta = D.delv_zeta[el];
+        const double gxi = gp(D.delv_xi)[el], geta = gp(D.delv_eta)[el], gzeta = gp(D.delv_zeta)[el];
         double xim, xip, etam, etap, zetam, zetap;
         if (U) {
             const int4 *fp = reinterpret_cast<const int4 *>(D.u_face) + 2 * (size_t)el;
             const int4 a = fp[0], b = fp[1];
             const int bc = b.z;
-            xim = face_nbr_u(D.delv_xi, el, a.x, bc & kXiM, kXiMSymm, kXiMComm);
-            xip = face_nbr_u(D.delv_xi, el, a.y, bc & kXiP, kXiPSymm, kXiPComm);
-            etam = face_nbr_u(D.delv_eta, el, a.z, bc & kEtaM, kEtaMSymm, kEtaMComm);
-            etap = face_nbr_u(D.delv_eta, el, a.w, bc & kEtaP, kEtaPSymm, kEtaPComm);
-            zetam = face_nbr_u(D.delv_zeta, el, b.x, bc & kZetaM, kZetaMSymm, kZetaMComm);
-            zetap = face_nbr_u(D.delv_zeta, el, b.y, bc & kZetaP, kZetaPSymm, kZetaPComm);
+            xim = face_nbr_u(gp(D.delv_xi), el, a.x, bc & kXiM, kXiMSymm, kXiMComm);
+            xip = face_nbr_u(gp(D.delv_xi), el, a.y, bc & kXiP, kXiPSymm, kXiPComm);
+            etam = face_nbr_u(gp(D.delv_eta), el, a.z, bc & kEtaM, kEtaMSymm, kEtaMComm);
+            etap = face_nbr_u(gp(D.delv_eta), el, a.w, bc & kEtaP, kEtaPSymm, kEtaPComm);
+            zetam = face_nbr_u(gp(D.delv_zeta), el, b.x, bc & kZetaM, kZetaMSymm, kZetaMComm);
+            zetap = face_nbr_u(gp(D.delv_zeta), el, b.y, bc & kZetaP, kZetaPSymm, kZetaPComm);
         } else {
             const int nx = D.nx;
             const int k = el / (nx * nx);
             const int rem = el - k * nx * nx;
             const int j = rem / nx;
             const int i = rem - j * nx;
-            xim = face_nbr(D.delv_xi, el, i == 0, el - 1, D.ghost[4], k * nx + j, D.col == 0);
-            xip = face_nbr(D.delv_xi, el, i == nx - 1, el + 1, D.ghost[5], k * nx + j, false);
-            etam = face_nbr(D.delv_eta, el, j == 0, el - nx, D.ghost[2], k * nx + i, D.row == 0);
-            etap = face_nbr(D.delv_eta, el, j == nx - 1, el + nx, D.ghost[3], k * nx + i, false);
-            zetam = face_nbr(D.delv_zeta, el, k == 0, el - nx * nx, D.ghost[0], j * nx + i, D.plane == 0);
-            zetap = face_nbr(D.delv_zeta, el, k == nx - 1, el + nx * nx, D.ghost[1], j * nx + i, false);
+            xim = face_nbr(gp(D.delv_xi), el, i == 0, el - 1, D.ghost[4], k * nx + j, D.col == 0);
+            xip = face_nbr(gp(D.delv_xi), el, i == nx - 1, el + 1, D.ghost[5], k * nx + j, false);
+            etam = face_nbr(gp(D.delv_eta), el, j == 0, el - nx, D.ghost[2], k * nx + i, D.row == 0);
+            etap = face_nbr(gp(D.delv_eta), el, j == nx - 1, el + nx, D.ghost[3], k * nx + i, false);
+            zetam = face_nbr(gp(D.delv_zeta), el, k == 0, el - nx * nx, D.ghost[0], j * nx + i, D.plane == 0);
+            zetap = face_nbr(gp(D.delv_zeta), el, k == nx - 1, el + nx * nx, D.ghost[1], j * nx + i, false);
         }
         const double phixi = monoq_phi(gxi, xim, xip);
         const double phieta = monoq_phi(geta, etam, etap);
         const double phizeta = monoq_phi(gzeta, zetam, zetap);
-        vdov = D.vdov[el];
-        const double vnew_raw = D.vnew[el];
+        vdov = gp(D.vdov)[el];
+        const double vnew_raw = gp(D.vnew)[el];
         double qlin, qquad;
         if (vdov > 0.) {
             qlin = 0.; qquad = 0.;
         } else {
-            double dxi = gxi * D.delx_xi[el];
-            double deta = geta * D.delx_eta[el];
-            double dzeta = gzeta * D.delx_zeta[el];
+            double dxi = gxi * gp(D.delx_xi)[el];
+            double deta = geta * gp(D.delx_eta)[el];
+            double dzeta = gzeta * gp(D.delx_zeta)[el];
             if (dxi > 0.) dxi = 0.;
             if (deta > 0.) deta = 0.;
             if (dzeta > 0.) dzeta = 0.;
-            const double rho = D.elemMass[el] / (D.volo[el] * vnew_raw);
+            const double rho = gp(D.elemMass)[el] / (gp(D.volo)[el] * vnew_raw);
             qlin = -Phys::qlc_monoq * rho *
                    (dxi * (1. - phixi) + deta * (1. - phieta) + dzeta * (1. - phizeta));
             qquad = Phys::qqc_monoq * rho *
@@ -1245,7 +1248,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         }
 
         // --- qstop on the q of the previous cycle (lulesh.cc:2531-2546) ---
-        const double q_old = D.q[el];
+        const double q_old = gp(D.q)[el];
         if (q_old > G->qstop) raise_status(G, -2);
 
         // --- EOS inputs (vnew clamped to [eosvmin, eosvmax], lulesh.cc:2642-2650) ---
@@ -1253,9 +1256,9 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         if (vnewc < Phys::eosvmin) vnewc = Phys::eosvmin;
         if (vnewc > Phys::eosvmax) vnewc = Phys::eosvmax;
         s_in[0][tid] = vnewc;
-        s_in[1][tid] = D.e[el];
-        s_in[2][tid] = D.delv[el];
-        s_in[3][tid] = D.p[el];
+        s_in[1][tid] = gp(D.e)[el];
+        s_in[2][tid] = gp(D.delv)[el];
+        s_in[3][tid] = gp(D.p)[el];
         s_in[4][tid] = q_old;
         s_in[5][tid] = qquad;
         s_in[6][tid] = qlin;
@@ -1264,7 +1267,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
 
     // --- EOS repetitions, dealt over the workgroup ---
     {
-        const unsigned short *tab = D.evtab + t.eoff;
+        const auto tab = gp(D.evtab) + t.eoff;
         const int total = t.ecount, bs = blockDim.x;
         const unsigned base = (unsigned)t.start & kEvMask;
         unsigned ent = tid < total ? tab[tid] : 0u;
@@ -1289,20 +1292,20 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     if (live) {
         const double p_new = s_out[0][tid], e_new = s_out[1][tid];
         const double vnewc = s_in[0][tid];
-        D.p[el] = p_new;
-        D.e[el] = e_new;
-        D.q[el] = s_out[2][tid];
+        gp(D.p)[el] = p_new;
+        gp(D.e)[el] = e_new;
+        gp(D.q)[el] = s_out[2][tid];
         const double ss = ss_of(s_out[4][tid], e_new, vnewc, s_out[3][tid], p_new);
-        D.ss[el] = ss;
+        gp(D.ss)[el] = ss;
         double vfin = vnewc;
         if (fabs(vfin - 1.0) < Phys::v_cut) vfin = 1.0;
-        D.v[el] = vfin;
+        gp(D.v)[el] = vfin;
         // the next cycle's hourglass coefficient needs exactly these ss, v
-        D.hgcoef[el] = hg_coefficient(ss, D.elemMass[el], D.volo[el], vfin);
+        gp(D.hgcoef)[el] = hg_coefficient(ss, gp(D.elemMass)[el], gp(D.volo)[el], vfin);
 
         // --- time constraints ---
         if (vdov != 0.) {
-            const double arealg = D.arealg[el];
+            const double arealg = gp(D.arealg)[el];
             const double qqc2 = 64.0 * Phys::qqc * Phys::qqc;
             double dtf = ss * ss;
             if (vdov < 0.) dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;

@@ -10,6 +10,24 @@
 
 namespace lulesh {
 
+// Global-address-space views of the device pointers held in descriptors.
+// A pointer loaded from a struct is generic to the compiler, which then
+// issues flat loads and stores; those count against the LDS counter as
+// well as the vector-memory one, so every wait for an LDS access also
+// drains the outstanding global loads.  Accessed through gp(), the same
+// memory is read and written with global instructions.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gp(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gp(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+
 // ---------------------------------------------------------------------------
 // Correctly rounded cube root.
 //
