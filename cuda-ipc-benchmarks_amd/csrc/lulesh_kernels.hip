@@ -176,7 +176,9 @@ template <bool U>
 __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const DomDev &D = doms[t.dom];
+    // the tile's subdomain is uniform over the workgroup: one scalar copy of
+    // its descriptor, which no store in the kernel can be taken to alias
+    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     GroupCtl *G = D.grp;
     if (group_stopped(*G)) return;
     const int el = t.start + threadIdx.x;
@@ -414,7 +416,9 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     __shared__ double lds[24][kMThreads];       // corner forces of one phase
     __shared__ double sn[2][6][kPatch];          // node planes (x y z xd yd zd), by plane parity
     const ColTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const DomDev &D = doms[t.dom];
+    // the tile's subdomain is uniform over the workgroup: one scalar copy of
+    // its descriptor, which no store in the kernel can be taken to alias
+    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     GroupCtl *G = D.grp;
     if (group_stopped(*G)) return;
     const int nx = D.nx, s1 = nx + 1, s2 = s1 * s1;
@@ -454,18 +458,19 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         if (xt_i < 0 || xt_i > nx || xt_j < 0 || xt_j > nx) xt_sl = -1;
     }
     double pf_own[6], pf_xt[6];
+    // The loads of a plane are issued on every path -- at a valid node
+    // (node 0) where the slot or the plane does not exist, the values then
+    // unused -- so the number of loads in flight is the same on all paths
+    // and the waits for the older ones can be counted exactly instead of
+    // draining the whole queue.
     auto fetch = [&](int kn) {           // issue the loads of node plane kn
-        if (kn < 0 || kn > nx) return;
-        if (own_in) {
-            const int nd = kn * s2 + own_j * s1 + own_i;
-            pf_own[0] = gp(S.x)[nd]; pf_own[1] = gp(S.y)[nd]; pf_own[2] = gp(S.z)[nd];
-            pf_own[3] = gp(S.xd)[nd]; pf_own[4] = gp(S.yd)[nd]; pf_own[5] = gp(S.zd)[nd];
-        }
-        if (xt_sl >= 0) {
-            const int nd = kn * s2 + xt_j * s1 + xt_i;
-            pf_xt[0] = gp(S.x)[nd]; pf_xt[1] = gp(S.y)[nd]; pf_xt[2] = gp(S.z)[nd];
-            pf_xt[3] = gp(S.xd)[nd]; pf_xt[4] = gp(S.yd)[nd]; pf_xt[5] = gp(S.zd)[nd];
-        }
+        const int kc = kn < 0 ? 0 : (kn > nx ? nx : kn);
+        const int nd = own_in ? kc * s2 + own_j * s1 + own_i : 0;
+        pf_own[0] = gp(S.x)[nd]; pf_own[1] = gp(S.y)[nd]; pf_own[2] = gp(S.z)[nd];
+        pf_own[3] = gp(S.xd)[nd]; pf_own[4] = gp(S.yd)[nd]; pf_own[5] = gp(S.zd)[nd];
+        const int nt = xt_sl >= 0 ? kc * s2 + xt_j * s1 + xt_i : 0;
+        pf_xt[0] = gp(S.x)[nt]; pf_xt[1] = gp(S.y)[nt]; pf_xt[2] = gp(S.z)[nt];
+        pf_xt[3] = gp(S.xd)[nt]; pf_xt[4] = gp(S.yd)[nt]; pf_xt[5] = gp(S.zd)[nt];
     };
     auto stage = [&](int kn) {           // write the fetched plane kn to LDS
         if (kn < 0 || kn > nx) return;
@@ -480,8 +485,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     // element fields of plane c (p, q, v, volo, hgcoef), fetched a plane ahead
     double ef[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ef_next[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     auto fetch_elem = [&](int c, double out[5]) {
-        if (!(col_ok && c >= 0 && c < nx)) return;
-        const int el = (c * nx + b) * nx + a;
+        const int el = (col_ok && c >= 0 && c < nx) ? (c * nx + b) * nx + a : 0;
         out[0] = gp(D.p)[el]; out[1] = gp(D.q)[el]; out[2] = gp(D.v)[el]; out[3] = gp(D.volo)[el];
         out[4] = gp(D.hgcoef)[el];
     };
@@ -493,17 +497,20 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     fetch_elem(k0 - 1, ef);
     __syncthreads();
 
+    // a non-positive volume (lulesh.cc:1263-1265, 1340-1349) is reported
+    // once, after the planes: no atomic sits on the loop's path
+    bool vol_err = false;
     double ps[3] = {0.0, 0.0, 0.0}, ph[3] = {0.0, 0.0, 0.0};   // node plane c+1 partials
     double fs[3] = {0.0, 0.0, 0.0};                              // node plane c stress sums
     for (int c = k0 - 1; c < k1; ++c) {
         const bool el_ok = col_ok && c >= 0 && c < nx;
         const double (*Lo)[kPatch] = sn[c & 1];
         const double (*Up)[kPatch] = sn[(c + 1) & 1];
-        // loads for the next plane run while this one is computed
-        if (c + 2 <= k1) fetch(c + 2);
+        // this plane's nodal masses (first: the update waits for them
+        // alone), then the next planes' loads, in flight meanwhile
+        const double nmass = gp(D.nodalMass)[(node_ok && c >= 0) ? c * s2 + nj * s1 + ni : 0];
+        fetch(c + 2);
         fetch_elem(c + 1, ef_next);
-        double nmass = 0.0;
-        if (advance && node_ok && c >= k0) nmass = gp(D.nodalMass)[c * s2 + nj * s1 + ni];
 
         // ---- stress corners ----
         if (el_ok) {
@@ -515,7 +522,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             }
             const double sig = -ef[0] - ef[1];
             const Jac J = elem_jacobian(x, y, z);
-            if (J.volume <= 0.0) raise_status(G, -1);
+            vol_err |= J.volume <= 0.0;
             double bx[8], by[8], bz[8];
             node_normals(x, y, z, bx, by, bz);
 #pragma unroll
@@ -554,7 +561,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         // ---- hourglass corners ----
         if (el_ok) {
             const double vv = ef[2];
-            if (vv <= 0.0) raise_status(G, -1);
+            vol_err |= vv <= 0.0;
             HgModes H;
             {
                 double x[8], y[8], z[8];
@@ -642,6 +649,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         for (int q = 0; q < 5; ++q) ef[q] = ef_next[q];
         __syncthreads();
     }
+    if (vol_err) raise_status(G, -1);
 }
 
 // ---------------------------------------------------------------------------
@@ -705,7 +713,9 @@ template <bool FUSE_UPDATE, bool U>
 __global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const DomDev &D = doms[t.dom];
+    // the tile's subdomain is uniform over the workgroup: one scalar copy of
+    // its descriptor, which no store in the kernel can be taken to alias
+    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     if (group_stopped(*D.grp)) return;
     const int nd = t.start + threadIdx.x;
     if (nd >= D.numNode) return;
@@ -765,7 +775,9 @@ template <bool U>
 __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const DomDev &D = doms[t.dom];
+    // the tile's subdomain is uniform over the workgroup: one scalar copy of
+    // its descriptor, which no store in the kernel can be taken to alias
+    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     if (group_stopped(*D.grp)) return;
     const int nd = t.start + threadIdx.x;
     if (nd >= D.numNode) return;
@@ -819,7 +831,9 @@ __global__ __launch_bounds__(256) void k_node_update_direct(const DomDev *doms, 
         node_update_pv(doms, D, c.idx, gp(D.dir_pv)[c.idx]);
         return;
     }
-    const DomDev &D = doms[t.dom];
+    // the tile's subdomain is uniform over the workgroup: one scalar copy of
+    // its descriptor, which no store in the kernel can be taken to alias
+    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     if (group_stopped(*D.grp)) return;
     const int nd = t.start + threadIdx.x;
     if (nd >= D.numNode) return;
@@ -896,7 +910,9 @@ __device__ __forceinline__ void node_update_pv(const DomDev *doms, const DomDev 
 __global__ __launch_bounds__(256) void k_pv_snapshot(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const DomDev &D = doms[t.dom];
+    // the tile's subdomain is uniform over the workgroup: one scalar copy of
+    // its descriptor, which no store in the kernel can be taken to alias
+    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     const int nd = t.start + threadIdx.x;
     if (nd >= D.numNode || !D.dir_pv) return;
     const int pi = gp(D.dir_pv)[nd];
@@ -1020,7 +1036,9 @@ template <bool U>
 __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Tile *tiles, int ntiles)
 {
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const DomDev &D = doms[t.dom];
+    // the tile's subdomain is uniform over the workgroup: one scalar copy of
+    // its descriptor, which no store in the kernel can be taken to alias
+    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     GroupCtl *G = D.grp;
     if (group_stopped(*G)) return;
     const int el = t.start + threadIdx.x;
@@ -1190,7 +1208,9 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     __shared__ double s_in[7][256];
     __shared__ double s_out[5][256];
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-    const DomDev &D = doms[t.dom];
+    // the tile's subdomain is uniform over the workgroup: one scalar copy of
+    // its descriptor, which no store in the kernel can be taken to alias
+    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     GroupCtl *G = D.grp;
     if (group_stopped(*G)) return;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1272,7 +1292,9 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         const unsigned base = (unsigned)t.start & kEvMask;
         unsigned ent = tid < total ? tab[tid] : 0u;
         for (int w = tid; w < total; w += bs) {
-            const unsigned next = w + bs < total ? tab[w + bs] : 0u;   // in flight meanwhile
+            // the next entry, in flight meanwhile; loaded unconditionally
+            // (clamped index) so the wait for it can sit at the loop's end
+            const unsigned next = tab[min(w + bs, total - 1)];
             const int o = (int)(((ent & kEvMask) - base) & kEvMask);
             const EosOut r = eos_rep(s_in[0][o], s_in[1][o], s_in[2][o], s_in[3][o], s_in[4][o],
                                      s_in[5][o], s_in[6][o]);
