@@ -1074,29 +1074,28 @@ __device__ __forceinline__ double monoq_phi(double self, double dm, double dp)
     return phi;
 }
 
-// gradient of the neighbour across one face: interior neighbour, own value
-// on a symmetry plane, 0 on a free surface, ghost plane on a subdomain face
-// (the elemBC / lxim.. logic of SetupBoundaryConditions,
-// lulesh-init.cc:576-630)
-template <typename P>
-__device__ __forceinline__ double face_nbr(P g, int el, bool at_edge, int nbr, int ghost_base, int ghost_off,
-                                           bool symm_side)
+// Neighbour gradient across one face: interior neighbour, own value on a
+// symmetry plane, 0 on a free surface, ghost plane on a subdomain face (the
+// elemBC / lxim.. logic of SetupBoundaryConditions, lulesh-init.cc:576-630).
+// Split into the index to load (the element's own where the face has no
+// neighbour value) and whether the loaded value is kept, so that all of an
+// element's loads can be issued before any of them is used.
+__device__ __forceinline__ int face_idx(int el, bool at_edge, int nbr, int ghost_base, int ghost_off,
+                                        bool symm_side, bool &keep)
 {
-    if (!at_edge) return g[nbr];
-    if (ghost_base >= 0) return g[ghost_base + ghost_off];
-    return symm_side ? g[el] : 0.0;
+    keep = !at_edge || ghost_base >= 0 || symm_side;
+    return !at_edge ? nbr : (ghost_base >= 0 ? ghost_base + ghost_off : el);
 }
 
 // the same for an unstructured mesh, from the element's elemBC bits and
 // face-neighbour list (CalcMonotonicQRegionForElems' switch,
 // lulesh.cc:2307-2400): neighbour for COMM or no bit, own value for SYMM,
 // 0 for FREE and any other combination
-template <typename P>
-__device__ __forceinline__ double face_nbr_u(P g, int el, int nbr, int bits, int symm, int comm)
+__device__ __forceinline__ int face_idx_u(int el, int nbr, int bits, int symm, int comm, bool &keep)
 {
-    if (bits == 0 || bits == comm) return g[nbr];
-    if (bits == symm) return g[el];
-    return 0.0;
+    const bool use_nbr = bits == 0 || bits == comm;
+    keep = use_nbr || bits == symm;
+    return use_nbr ? nbr : el;
 }
 
 __device__ __forceinline__ double ss_of(double pbvc, double e_new, double vv, double bvc,
@@ -1213,53 +1212,74 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     GroupCtl *G = D.grp;
     if (group_stopped(*G)) return;
+    const double qstop = *gp(&G->qstop);
+    // the tile's first EOS table entry, in flight during the element phase
+    const auto tab = gp(D.evtab) + t.eoff;
+    const int total = t.ecount;
+    unsigned ent = tab[min((int)threadIdx.x, total - 1)];
     const int tid = threadIdx.x, lane = tid & 63;
     const int el = t.start + tid;
     const bool live = tid < te && el < D.numElem;
+    bool q_err = false;   // raised after the element's work: no atomic on the load path
     double vdov = 0.0;
     if (live) {
-        // --- MonoQ region ---
+        // every load of the element first (none behind a branch), then the
+        // arithmetic
         const double gxi = gp(D.delv_xi)[el], geta = gp(D.delv_eta)[el], gzeta = gp(D.delv_zeta)[el];
-        double xim, xip, etam, etap, zetam, zetap;
+        vdov = gp(D.vdov)[el];
+        const double vnew_raw = gp(D.vnew)[el];
+        const double delx_xi = gp(D.delx_xi)[el], delx_eta = gp(D.delx_eta)[el];
+        const double delx_zeta = gp(D.delx_zeta)[el];
+        const double mass = gp(D.elemMass)[el], volo = gp(D.volo)[el];
+        const double q_old = gp(D.q)[el], e_old = gp(D.e)[el], delv = gp(D.delv)[el], p_old = gp(D.p)[el];
+        // --- MonoQ region ---
+        int ix[6];
+        bool keep[6];
         if (U) {
             const int4 *fp = reinterpret_cast<const int4 *>(D.u_face) + 2 * (size_t)el;
             const int4 a = fp[0], b = fp[1];
             const int bc = b.z;
-            xim = face_nbr_u(gp(D.delv_xi), el, a.x, bc & kXiM, kXiMSymm, kXiMComm);
-            xip = face_nbr_u(gp(D.delv_xi), el, a.y, bc & kXiP, kXiPSymm, kXiPComm);
-            etam = face_nbr_u(gp(D.delv_eta), el, a.z, bc & kEtaM, kEtaMSymm, kEtaMComm);
-            etap = face_nbr_u(gp(D.delv_eta), el, a.w, bc & kEtaP, kEtaPSymm, kEtaPComm);
-            zetam = face_nbr_u(gp(D.delv_zeta), el, b.x, bc & kZetaM, kZetaMSymm, kZetaMComm);
-            zetap = face_nbr_u(gp(D.delv_zeta), el, b.y, bc & kZetaP, kZetaPSymm, kZetaPComm);
+            ix[0] = face_idx_u(el, a.x, bc & kXiM, kXiMSymm, kXiMComm, keep[0]);
+            ix[1] = face_idx_u(el, a.y, bc & kXiP, kXiPSymm, kXiPComm, keep[1]);
+            ix[2] = face_idx_u(el, a.z, bc & kEtaM, kEtaMSymm, kEtaMComm, keep[2]);
+            ix[3] = face_idx_u(el, a.w, bc & kEtaP, kEtaPSymm, kEtaPComm, keep[3]);
+            ix[4] = face_idx_u(el, b.x, bc & kZetaM, kZetaMSymm, kZetaMComm, keep[4]);
+            ix[5] = face_idx_u(el, b.y, bc & kZetaP, kZetaPSymm, kZetaPComm, keep[5]);
         } else {
             const int nx = D.nx;
             const int k = el / (nx * nx);
             const int rem = el - k * nx * nx;
             const int j = rem / nx;
             const int i = rem - j * nx;
-            xim = face_nbr(gp(D.delv_xi), el, i == 0, el - 1, D.ghost[4], k * nx + j, D.col == 0);
-            xip = face_nbr(gp(D.delv_xi), el, i == nx - 1, el + 1, D.ghost[5], k * nx + j, false);
-            etam = face_nbr(gp(D.delv_eta), el, j == 0, el - nx, D.ghost[2], k * nx + i, D.row == 0);
-            etap = face_nbr(gp(D.delv_eta), el, j == nx - 1, el + nx, D.ghost[3], k * nx + i, false);
-            zetam = face_nbr(gp(D.delv_zeta), el, k == 0, el - nx * nx, D.ghost[0], j * nx + i, D.plane == 0);
-            zetap = face_nbr(gp(D.delv_zeta), el, k == nx - 1, el + nx * nx, D.ghost[1], j * nx + i, false);
+            ix[0] = face_idx(el, i == 0, el - 1, D.ghost[4], k * nx + j, D.col == 0, keep[0]);
+            ix[1] = face_idx(el, i == nx - 1, el + 1, D.ghost[5], k * nx + j, false, keep[1]);
+            ix[2] = face_idx(el, j == 0, el - nx, D.ghost[2], k * nx + i, D.row == 0, keep[2]);
+            ix[3] = face_idx(el, j == nx - 1, el + nx, D.ghost[3], k * nx + i, false, keep[3]);
+            ix[4] = face_idx(el, k == 0, el - nx * nx, D.ghost[0], j * nx + i, D.plane == 0, keep[4]);
+            ix[5] = face_idx(el, k == nx - 1, el + nx * nx, D.ghost[1], j * nx + i, false, keep[5]);
         }
+        const double r0 = gp(D.delv_xi)[ix[0]], r1 = gp(D.delv_xi)[ix[1]];
+        const double r2 = gp(D.delv_eta)[ix[2]], r3 = gp(D.delv_eta)[ix[3]];
+        const double r4 = gp(D.delv_zeta)[ix[4]], r5 = gp(D.delv_zeta)[ix[5]];
+        // all loads issued; nothing above uses a loaded value
+        __builtin_amdgcn_sched_barrier(0);
+        const double xim = keep[0] ? r0 : 0.0, xip = keep[1] ? r1 : 0.0;
+        const double etam = keep[2] ? r2 : 0.0, etap = keep[3] ? r3 : 0.0;
+        const double zetam = keep[4] ? r4 : 0.0, zetap = keep[5] ? r5 : 0.0;
         const double phixi = monoq_phi(gxi, xim, xip);
         const double phieta = monoq_phi(geta, etam, etap);
         const double phizeta = monoq_phi(gzeta, zetam, zetap);
-        vdov = gp(D.vdov)[el];
-        const double vnew_raw = gp(D.vnew)[el];
         double qlin, qquad;
         if (vdov > 0.) {
             qlin = 0.; qquad = 0.;
         } else {
-            double dxi = gxi * gp(D.delx_xi)[el];
-            double deta = geta * gp(D.delx_eta)[el];
-            double dzeta = gzeta * gp(D.delx_zeta)[el];
+            double dxi = gxi * delx_xi;
+            double deta = geta * delx_eta;
+            double dzeta = gzeta * delx_zeta;
             if (dxi > 0.) dxi = 0.;
             if (deta > 0.) deta = 0.;
             if (dzeta > 0.) dzeta = 0.;
-            const double rho = gp(D.elemMass)[el] / (gp(D.volo)[el] * vnew_raw);
+            const double rho = mass / (volo * vnew_raw);
             qlin = -Phys::qlc_monoq * rho *
                    (dxi * (1. - phixi) + deta * (1. - phieta) + dzeta * (1. - phizeta));
             qquad = Phys::qqc_monoq * rho *
@@ -1268,17 +1288,16 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         }
 
         // --- qstop on the q of the previous cycle (lulesh.cc:2531-2546) ---
-        const double q_old = gp(D.q)[el];
-        if (q_old > G->qstop) raise_status(G, -2);
+        q_err = q_old > qstop;
 
         // --- EOS inputs (vnew clamped to [eosvmin, eosvmax], lulesh.cc:2642-2650) ---
         double vnewc = vnew_raw;
         if (vnewc < Phys::eosvmin) vnewc = Phys::eosvmin;
         if (vnewc > Phys::eosvmax) vnewc = Phys::eosvmax;
         s_in[0][tid] = vnewc;
-        s_in[1][tid] = gp(D.e)[el];
-        s_in[2][tid] = gp(D.delv)[el];
-        s_in[3][tid] = gp(D.p)[el];
+        s_in[1][tid] = e_old;
+        s_in[2][tid] = delv;
+        s_in[3][tid] = p_old;
         s_in[4][tid] = q_old;
         s_in[5][tid] = qquad;
         s_in[6][tid] = qlin;
@@ -1287,10 +1306,8 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
 
     // --- EOS repetitions, dealt over the workgroup ---
     {
-        const auto tab = gp(D.evtab) + t.eoff;
-        const int total = t.ecount, bs = blockDim.x;
+        const int bs = blockDim.x;
         const unsigned base = (unsigned)t.start & kEvMask;
-        unsigned ent = tid < total ? tab[tid] : 0u;
         for (int w = tid; w < total; w += bs) {
             // the next entry, in flight meanwhile; loaded unconditionally
             // (clamped index) so the wait for it can sit at the loop's end
@@ -1362,6 +1379,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         if (cbits < *(volatile unsigned long long *)&C.dtcourant_bits) atomicMin(&C.dtcourant_bits, cbits);
         if (hbits < *(volatile unsigned long long *)&C.dthydro_bits) atomicMin(&C.dthydro_bits, hbits);
     }
+    if (q_err) raise_status(G, -2);
 }
 
 // ---------------------------------------------------------------------------
