@@ -194,16 +194,16 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
 
     // stress part
     {
-        const Jac J = elem_jacobian(x, y, z);
-        if (J.volume <= 0.0) raise_status(G, -1);      // lulesh.cc:1340-1349
+        if (jac_volume(x, y, z) <= 0.0) raise_status(G, -1);      // lulesh.cc:1340-1349
         double bx[8], by[8], bz[8];
-        node_normals(x, y, z, bx, by, bz);
-        const double sig = -gp(D.p)[el] - gp(D.q)[el];         // InitStressTermsForElems
+        node_normals16(x, y, z, bx, by, bz);
+        // InitStressTermsForElems' sigma, over 16 against the normals x 16
+        const double sig16 = (-gp(D.p)[el] - gp(D.q)[el]) * 0.0625;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            gp(D.corner)[(0 * 8 + c) * E + el] = -(sig * bx[c]);
-            gp(D.corner)[(1 * 8 + c) * E + el] = -(sig * by[c]);
-            gp(D.corner)[(2 * 8 + c) * E + el] = -(sig * bz[c]);
+            gp(D.corner)[(0 * 8 + c) * E + el] = -(sig16 * bx[c]);
+            gp(D.corner)[(1 * 8 + c) * E + el] = -(sig16 * by[c]);
+            gp(D.corner)[(2 * 8 + c) * E + el] = -(sig16 * bz[c]);
         }
     }
 
@@ -332,23 +332,6 @@ __device__ __forceinline__ void load_elem_xyz(const NodeState &S, const int n[8]
 {
 #pragma unroll
     for (int c = 0; c < 8; ++c) { x[c] = gp(S.x)[n[c]]; y[c] = gp(S.y)[n[c]]; z[c] = gp(S.z)[n[c]]; }
-}
-
-// stress corner forces -(sigma B) from the face normals
-// (IntegrateStressForElems, lulesh.cc:546-675); returns the Jacobian volume
-__device__ __forceinline__ double elem_stress(const double x[8], const double y[8], const double z[8],
-                                              double sig, double fx[8], double fy[8], double fz[8])
-{
-    const Jac J = elem_jacobian(x, y, z);
-    double bx[8], by[8], bz[8];
-    node_normals(x, y, z, bx, by, bz);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        fx[c] = -(sig * bx[c]);
-        fy[c] = -(sig * by[c]);
-        fz[c] = -(sig * bz[c]);
-    }
-    return J.volume;
 }
 
 // FB hourglass corner forces (CalcFBHourglassForceForElems,
@@ -520,16 +503,17 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                 x[q] = Lo[0][cs[q]]; y[q] = Lo[1][cs[q]]; z[q] = Lo[2][cs[q]];
                 x[q + 4] = Up[0][cs[q]]; y[q + 4] = Up[1][cs[q]]; z[q + 4] = Up[2][cs[q]];
             }
-            const double sig = -ef[0] - ef[1];
-            const Jac J = elem_jacobian(x, y, z);
-            vol_err |= J.volume <= 0.0;
+            // sigma / 16 against the normals x 16: the same products.  The
+            // division is exact: p and q are 0 or at least their cut-offs
+            const double sig16 = (-ef[0] - ef[1]) * 0.0625;
+            vol_err |= jac_volume(x, y, z) <= 0.0;
             double bx[8], by[8], bz[8];
-            node_normals(x, y, z, bx, by, bz);
+            node_normals16(x, y, z, bx, by, bz);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                lds[0 * 8 + q][tid] = -(sig * bx[q]);
-                lds[1 * 8 + q][tid] = -(sig * by[q]);
-                lds[2 * 8 + q][tid] = -(sig * bz[q]);
+                lds[0 * 8 + q][tid] = -(sig16 * bx[q]);
+                lds[1 * 8 + q][tid] = -(sig16 * by[q]);
+                lds[2 * 8 + q][tid] = -(sig16 * bz[q]);
             }
         } else {
 #pragma unroll
@@ -958,7 +942,9 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
             yh[c] = y[c] - dt2 * yv[c];
             zh[c] = z[c] - dt2 * zv[c];
         }
-        const Jac J = elem_jacobian(xh, yh, zh);
+        // B and the Jacobian determinant both 64x: inv_detJ * (B . v) is
+        // the reference's quotient exactly
+        const Jac64 J = elem_jacobian64(xh, yh, zh);
         double b[3][4];
         shape_b4(J, b);
         const double inv_detJ = 1.0 / J.volume;

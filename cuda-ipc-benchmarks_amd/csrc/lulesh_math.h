@@ -3,7 +3,9 @@
 // Each function evaluates the reference's expressions in the reference's
 // association order (LULESH/omp_4.0/lulesh.cc), and the library is built
 // with -ffp-contract=off, so the results are the same IEEE-754 doubles the
-// CPU reference produces.  Citations are path:line into the reference.
+// CPU reference produces (the force kernels' volume and normals gather the
+// reference's power-of-two factors: an exact rescaling, stated there).
+// Citations are path:line into the reference.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -129,24 +131,30 @@ __device__ __forceinline__ double elem_volume(const double x[8], const double y[
     return vol * twelveth;
 }
 
-// Jacobian terms of CalcElemShapeFunctionDerivatives, lulesh.cc:342-428
-struct Jac {
+// Jacobian terms of CalcElemShapeFunctionDerivatives (lulesh.cc:342-428)
+// times 64: the nine .125 factors are left out, so the cofactors (and the
+// B columns summed from them) carry 64x and the volume, without the
+// reference's final 8, 64x as well.  Scaling by a power of two commutes
+// with rounding while no intermediate is subnormal (coordinate differences
+// above ~1e-150), so each term is exactly 64 times the reference's; a
+// quotient B / volume is the reference's bit for bit.
+struct Jac64 {
     double cjxxi, cjxet, cjxze, cjyxi, cjyet, cjyze, cjzxi, cjzet, cjzze;
     double volume;
 };
 
-__device__ __forceinline__ Jac elem_jacobian(const double x[8], const double y[8], const double z[8])
+__device__ __forceinline__ Jac64 elem_jacobian64(const double x[8], const double y[8], const double z[8])
 {
-    const double fjxxi = .125 * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
-    const double fjxet = .125 * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
-    const double fjxze = .125 * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
-    const double fjyxi = .125 * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
-    const double fjyet = .125 * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
-    const double fjyze = .125 * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
-    const double fjzxi = .125 * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
-    const double fjzet = .125 * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
-    const double fjzze = .125 * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
-    Jac J;
+    const double fjxxi = (x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]);
+    const double fjxet = (x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]);
+    const double fjxze = (x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]);
+    const double fjyxi = (y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]);
+    const double fjyet = (y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]);
+    const double fjyze = (y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]);
+    const double fjzxi = (z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]);
+    const double fjzet = (z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]);
+    const double fjzze = (z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]);
+    Jac64 J;
     J.cjxxi = (fjyet * fjzze) - (fjzet * fjyze);
     J.cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
     J.cjxze = (fjyxi * fjzet) - (fjzxi * fjyet);
@@ -156,13 +164,13 @@ __device__ __forceinline__ Jac elem_jacobian(const double x[8], const double y[8
     J.cjzxi = (fjxet * fjyze) - (fjyet * fjxze);
     J.cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
     J.cjzze = (fjxxi * fjyet) - (fjyxi * fjxet);
-    J.volume = 8. * (fjxet * J.cjxet + fjyet * J.cjyet + fjzet * J.cjzet);
+    J.volume = fjxet * J.cjxet + fjyet * J.cjyet + fjzet * J.cjzet;
     return J;
 }
 
 // first four shape-function derivative columns (the rest are their
 // negatives, lulesh.cc:399-424)
-__device__ __forceinline__ void shape_b4(const Jac &J, double b[3][4])
+__device__ __forceinline__ void shape_b4(const Jac64 &J, double b[3][4])
 {
     b[0][0] = -J.cjxxi - J.cjxet - J.cjxze;
     b[0][1] = J.cjxxi - J.cjxet - J.cjxze;
@@ -178,40 +186,46 @@ __device__ __forceinline__ void shape_b4(const Jac &J, double b[3][4])
     b[2][3] = -J.cjzxi + J.cjzet - J.cjzze;
 }
 
-// SumElemFaceNormal for one face (a b c d), lulesh.cc:433-466
-__device__ __forceinline__ void face_area(const double x[8], const double y[8], const double z[8],
-                                          int a, int b, int c, int d,
-                                          double &ax, double &ay, double &az)
+// IntegrateStressForElems' Jacobian volume (lulesh.cc:560-572 ->
+// CalcElemShapeFunctionDerivatives), which the stress phase only tests for
+// sign: the 64x volume times 1/64.
+__device__ __forceinline__ double jac_volume(const double x[8], const double y[8], const double z[8])
 {
-    const double bx0 = 0.5 * (x[d] + x[c] - x[b] - x[a]);
-    const double by0 = 0.5 * (y[d] + y[c] - y[b] - y[a]);
-    const double bz0 = 0.5 * (z[d] + z[c] - z[b] - z[a]);
-    const double bx1 = 0.5 * (x[c] + x[b] - x[d] - x[a]);
-    const double by1 = 0.5 * (y[c] + y[b] - y[d] - y[a]);
-    const double bz1 = 0.5 * (z[c] + z[b] - z[d] - z[a]);
-    ax = 0.25 * (by0 * bz1 - bz0 * by1);
-    ay = 0.25 * (bz0 * bx1 - bx0 * bz1);
-    az = 0.25 * (bx0 * by1 - by0 * bx1);
+    return elem_jacobian64(x, y, z).volume * 0x1p-6;
 }
 
-// CalcElemNodeNormals, lulesh.cc:471-525: the six faces in reference order,
-// each adding its area vector to its four nodes (accumulated from 0.0)
-__device__ __forceinline__ void node_normals(const double x[8], const double y[8], const double z[8],
-                                             double nx_[8], double ny_[8], double nz_[8])
+// CalcElemNodeNormals (lulesh.cc:433-525) times 16: the bisectors without
+// their 0.5 and the areas without their 0.25.  Each node's sum starts at
+// its first face's term where the reference adds that term to 0.0; the two
+// differ only in the sign of a zero, which the node force sums (started
+// from +0.0) do not keep.  The caller folds the 1/16 into the stress.
+__device__ __forceinline__ void node_normals16(const double x[8], const double y[8], const double z[8],
+                                               double nx_[8], double ny_[8], double nz_[8])
 {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) { nx_[i] = 0.0; ny_[i] = 0.0; nz_[i] = 0.0; }
     constexpr int F[6][4] = {{0, 1, 2, 3}, {0, 4, 5, 1}, {1, 5, 6, 2},
                              {2, 6, 7, 3}, {3, 7, 4, 0}, {4, 7, 6, 5}};
+    // the face at which each corner first appears
+    constexpr bool FIRST[6][4] = {{true, true, true, true}, {false, true, true, false},
+                                  {false, false, true, false}, {false, false, true, false},
+                                  {false, false, false, false}, {false, false, false, false}};
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        double ax, ay, az;
-        face_area(x, y, z, F[f][0], F[f][1], F[f][2], F[f][3], ax, ay, az);
+        const int a = F[f][0], b = F[f][1], c = F[f][2], d = F[f][3];
+        const double bx0 = x[d] + x[c] - x[b] - x[a];
+        const double by0 = y[d] + y[c] - y[b] - y[a];
+        const double bz0 = z[d] + z[c] - z[b] - z[a];
+        const double bx1 = x[c] + x[b] - x[d] - x[a];
+        const double by1 = y[c] + y[b] - y[d] - y[a];
+        const double bz1 = z[c] + z[b] - z[d] - z[a];
+        const double ax = by0 * bz1 - bz0 * by1;
+        const double ay = bz0 * bx1 - bx0 * bz1;
+        const double az = bx0 * by1 - by0 * bx1;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            nx_[F[f][k]] += ax;
-            ny_[F[f][k]] += ay;
-            nz_[F[f][k]] += az;
+            const int n = F[f][k];
+            nx_[n] = FIRST[f][k] ? ax : nx_[n] + ax;
+            ny_[n] = FIRST[f][k] ? ay : ny_[n] + ay;
+            nz_[n] = FIRST[f][k] ? az : nz_[n] + az;
         }
     }
 }
