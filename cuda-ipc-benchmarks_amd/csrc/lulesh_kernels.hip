@@ -449,11 +449,11 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     auto fetch = [&](int kn) {           // issue the loads of node plane kn
         const int kc = kn < 0 ? 0 : (kn > nx ? nx : kn);
         const int nd = own_in ? kc * s2 + own_j * s1 + own_i : 0;
-        pf_own[0] = gp(S.x)[nd]; pf_own[1] = gp(S.y)[nd]; pf_own[2] = gp(S.z)[nd];
-        pf_own[3] = gp(S.xd)[nd]; pf_own[4] = gp(S.yd)[nd]; pf_own[5] = gp(S.zd)[nd];
+        pf_own[0] = gld(S.x, nd); pf_own[1] = gld(S.y, nd); pf_own[2] = gld(S.z, nd);
+        pf_own[3] = gld(S.xd, nd); pf_own[4] = gld(S.yd, nd); pf_own[5] = gld(S.zd, nd);
         const int nt = xt_sl >= 0 ? kc * s2 + xt_j * s1 + xt_i : 0;
-        pf_xt[0] = gp(S.x)[nt]; pf_xt[1] = gp(S.y)[nt]; pf_xt[2] = gp(S.z)[nt];
-        pf_xt[3] = gp(S.xd)[nt]; pf_xt[4] = gp(S.yd)[nt]; pf_xt[5] = gp(S.zd)[nt];
+        pf_xt[0] = gld(S.x, nt); pf_xt[1] = gld(S.y, nt); pf_xt[2] = gld(S.z, nt);
+        pf_xt[3] = gld(S.xd, nt); pf_xt[4] = gld(S.yd, nt); pf_xt[5] = gld(S.zd, nt);
     };
     auto stage = [&](int kn) {           // write the fetched plane kn to LDS
         if (kn < 0 || kn > nx) return;
@@ -469,8 +469,8 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     double ef[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ef_next[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     auto fetch_elem = [&](int c, double out[5]) {
         const int el = (col_ok && c >= 0 && c < nx) ? (c * nx + b) * nx + a : 0;
-        out[0] = gp(D.p)[el]; out[1] = gp(D.q)[el]; out[2] = gp(D.v)[el]; out[3] = gp(D.volo)[el];
-        out[4] = gp(D.hgcoef)[el];
+        out[0] = gld(D.p, el); out[1] = gld(D.q, el); out[2] = gld(D.v, el); out[3] = gld(D.volo, el);
+        out[4] = gld(D.hgcoef, el);
     };
 
     fetch(k0 - 1);
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         const double (*Up)[kPatch] = sn[(c + 1) & 1];
         // this plane's nodal masses (first: the update waits for them
         // alone), then the next planes' loads, in flight meanwhile
-        const double nmass = gp(D.nodalMass)[(node_ok && c >= 0) ? c * s2 + nj * s1 + ni : 0];
+        const double nmass = gld(D.nodalMass, (node_ok && c >= 0) ? c * s2 + nj * s1 + ni : 0);
         fetch(c + 2);
         fetch_elem(c + 1, ef_next);
 
@@ -597,9 +597,9 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             if (c >= k0) {
                 const int nd = c * s2 + nj * s1 + ni;
                 if (store_f) {
-                    gp(D.fpx)[nd] = f[0];
-                    gp(D.fpy)[nd] = f[1];
-                    gp(D.fpz)[nd] = f[2];
+                    gst(D.fpx, nd, f[0]);
+                    gst(D.fpy, nd, f[1]);
+                    gst(D.fpz, nd, f[2]);
                 }
                 if (advance) {
                     // the node is complete: advance it from its old state
@@ -617,12 +617,12 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                     double vz = Lo[5][own_sl] + az * dt;
                     if (fabs(vz) < Phys::u_cut) vz = 0.0;
                     const NodeState N = ns_new(D);
-                    gp(N.xd)[nd] = vx;
-                    gp(N.yd)[nd] = vy;
-                    gp(N.zd)[nd] = vz;
-                    gp(N.x)[nd] = Lo[0][own_sl] + vx * dt;
-                    gp(N.y)[nd] = Lo[1][own_sl] + vy * dt;
-                    gp(N.z)[nd] = Lo[2][own_sl] + vz * dt;
+                    gst(N.xd, nd, vx);
+                    gst(N.yd, nd, vy);
+                    gst(N.zd, nd, vz);
+                    gst(N.x, nd, Lo[0][own_sl] + vx * dt);
+                    gst(N.y, nd, Lo[1][own_sl] + vy * dt);
+                    gst(N.z, nd, Lo[2][own_sl] + vz * dt);
                 }
             }
         }
@@ -925,12 +925,12 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
                                             const double x[8], const double y[8], const double z[8],
                                             const double xv[8], const double yv[8], const double zv[8])
 {
-    const double volo = gp(D.volo)[el];
+    const double volo = gld(D.volo, el);
     const double volume = elem_volume(x, y, z);
     const double vnew = volume / volo;
-    gp(D.vnew)[el] = vnew;
-    gp(D.delv)[el] = vnew - gp(D.v)[el];
-    gp(D.arealg)[el] = char_length(x, y, z, volume);
+    gst(D.vnew, el, vnew);
+    gst(D.delv, el, vnew - gld(D.v, el));
+    gst(D.arealg, el, char_length(x, y, z, volume));
     if (vnew <= 0.0) raise_status(G, -1);              // lulesh.cc:2037-2049
 
     {
@@ -955,7 +955,7 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
                                        b[1][2] * (yv[2] - yv[4]) + b[1][3] * (yv[3] - yv[5]));
         const double dzz = inv_detJ * (b[2][0] * (zv[0] - zv[6]) + b[2][1] * (zv[1] - zv[7]) +
                                        b[2][2] * (zv[2] - zv[4]) + b[2][3] * (zv[3] - zv[5]));
-        gp(D.vdov)[el] = dxx + dyy + dzz;
+        gst(D.vdov, el, dxx + dyy + dzz);
     }
 
     // MonoQ gradients at the end-of-step positions
@@ -974,31 +974,31 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
     double ax, ay, az, dxv, dyv, dzv;
 
     ax = dyi * dzj - dzi * dyj; ay = dzi * dxj - dxi * dzj; az = dxi * dyj - dyi * dxj;
-    gp(D.delx_zeta)[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    gst(D.delx_zeta, el, vol / sqrt(ax * ax + ay * ay + az * az + ptiny));
     ax *= norm; ay *= norm; az *= norm;
     dxv = 0.25 * ((xv[4] + xv[5] + xv[6] + xv[7]) - (xv[0] + xv[1] + xv[2] + xv[3]));
     dyv = 0.25 * ((yv[4] + yv[5] + yv[6] + yv[7]) - (yv[0] + yv[1] + yv[2] + yv[3]));
     dzv = 0.25 * ((zv[4] + zv[5] + zv[6] + zv[7]) - (zv[0] + zv[1] + zv[2] + zv[3]));
     const double dv_zeta = ax * dxv + ay * dyv + az * dzv;
-    gp(D.delv_zeta)[el] = dv_zeta;
+    gst(D.delv_zeta, el, dv_zeta);
 
     ax = dyj * dzk - dzj * dyk; ay = dzj * dxk - dxj * dzk; az = dxj * dyk - dyj * dxk;
-    gp(D.delx_xi)[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    gst(D.delx_xi, el, vol / sqrt(ax * ax + ay * ay + az * az + ptiny));
     ax *= norm; ay *= norm; az *= norm;
     dxv = 0.25 * ((xv[1] + xv[2] + xv[6] + xv[5]) - (xv[0] + xv[3] + xv[7] + xv[4]));
     dyv = 0.25 * ((yv[1] + yv[2] + yv[6] + yv[5]) - (yv[0] + yv[3] + yv[7] + yv[4]));
     dzv = 0.25 * ((zv[1] + zv[2] + zv[6] + zv[5]) - (zv[0] + zv[3] + zv[7] + zv[4]));
     const double dv_xi = ax * dxv + ay * dyv + az * dzv;
-    gp(D.delv_xi)[el] = dv_xi;
+    gst(D.delv_xi, el, dv_xi);
 
     ax = dyk * dzi - dzk * dyi; ay = dzk * dxi - dxk * dzi; az = dxk * dyi - dyk * dxi;
-    gp(D.delx_eta)[el] = vol / sqrt(ax * ax + ay * ay + az * az + ptiny);
+    gst(D.delx_eta, el, vol / sqrt(ax * ax + ay * ay + az * az + ptiny));
     ax *= norm; ay *= norm; az *= norm;
     dxv = -0.25 * ((xv[0] + xv[1] + xv[5] + xv[4]) - (xv[3] + xv[2] + xv[6] + xv[7]));
     dyv = -0.25 * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
     dzv = -0.25 * ((zv[0] + zv[1] + zv[5] + zv[4]) - (zv[3] + zv[2] + zv[6] + zv[7]));
     const double dv_eta = ax * dxv + ay * dyv + az * dzv;
-    gp(D.delv_eta)[el] = dv_eta;
+    gst(D.delv_eta, el, dv_eta);
 
     // CommMonoQ done in place (device_local backend): an element on a
     // shared face fills its neighbours' ghost slots with its three
@@ -1035,10 +1035,24 @@ __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Ti
 
     const NodeState S = ns_new(D);
     double x[8], y[8], z[8], xv[8], yv[8], zv[8];
+    if (U) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        x[c] = gp(S.x)[n[c]]; y[c] = gp(S.y)[n[c]]; z[c] = gp(S.z)[n[c]];
-        xv[c] = gp(S.xd)[n[c]]; yv[c] = gp(S.yd)[n[c]]; zv[c] = gp(S.zd)[n[c]];
+        for (int c = 0; c < 8; ++c) {
+            x[c] = gld(S.x, n[c]); y[c] = gld(S.y, n[c]); z[c] = gld(S.z, n[c]);
+            xv[c] = gld(S.xd, n[c]); yv[c] = gld(S.yd, n[c]); zv[c] = gld(S.zd, n[c]);
+        }
+    } else {
+        // on the block, corners 1, 2, 5, 6 follow corners 0, 3, 4, 7 in
+        // node order: one 16-byte load per pair and array
+        const double *const A[6] = {S.x, S.y, S.z, S.xd, S.yd, S.zd};
+        double *const V[6] = {x, y, z, xv, yv, zv};
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            gld_pair(A[f], n[0], V[f][0], V[f][1]);
+            gld_pair(A[f], n[3], V[f][3], V[f][2]);
+            gld_pair(A[f], n[4], V[f][4], V[f][5]);
+            gld_pair(A[f], n[7], V[f][7], V[f][6]);
+        }
     }
     kin_element(doms, D, G, el, dt, x, y, z, xv, yv, zv);
 }
@@ -1053,10 +1067,13 @@ __device__ __forceinline__ double monoq_phi(double self, double dm, double dp)
     double phi = .5 * (delvm + delvp);
     delvm *= Phys::monoq_limiter_mult;
     delvp *= Phys::monoq_limiter_mult;
-    if (delvm < phi) phi = delvm;
-    if (delvp < phi) phi = delvp;
-    if (phi < 0.) phi = 0.;
-    if (phi > Phys::monoq_max_slope) phi = Phys::monoq_max_slope;
+    // the four clamps as min / max instructions: for non-NaN values a tie
+    // can only differ in the sign of a zero, and phi enters q only through
+    // 1 - phi and phi * phi
+    phi = fmin(phi, delvm);
+    phi = fmin(phi, delvp);
+    phi = fmax(phi, 0.);
+    phi = fmin(phi, Phys::monoq_max_slope);
     return phi;
 }
 
@@ -1093,6 +1110,14 @@ __device__ __forceinline__ double ss_of(double pbvc, double e_new, double vv, do
     return ssc;
 }
 
+// `if (x < c) x = c` as one maximum instruction: the same value for every
+// non-NaN x when c is not zero, or when x cannot be -0.0 (in the EOS each
+// pressure has just been snapped to +0.0 below its cut-off)
+__device__ __forceinline__ double raise_to(double x, double c)
+{
+    return fmax(x, c);
+}
+
 struct EosOut {
     double p, e, q, bvc, pbvc;
 };
@@ -1110,13 +1135,13 @@ __device__ __forceinline__ EosOut eos_rep(double vnewc, double e_old, double del
     if (vnewc >= Phys::eosvmax) { p_old = 0.; compression = 0.; compHalfStep = 0.; }
     const double work = 0.;
     double e_new = e_old - 0.5 * delvc * (p_old + q_old) + 0.5 * work;
-    if (e_new < Phys::emin) e_new = Phys::emin;
+    e_new = raise_to(e_new, Phys::emin);
     double bvc = c1s * (compHalfStep + 1.);
     double pbvc = c1s;
     double pHalfStep = bvc * e_new;
     if (fabs(pHalfStep) < Phys::p_cut) pHalfStep = 0.0;
     if (vnewc >= Phys::eosvmax) pHalfStep = 0.0;
-    if (pHalfStep < Phys::pmin) pHalfStep = Phys::pmin;
+    pHalfStep = raise_to(pHalfStep, Phys::pmin);
     const double vhalf = 1. / (1. + compHalfStep);
     double q_new;
     if (delvc > 0.) {
@@ -1130,25 +1155,25 @@ __device__ __forceinline__ EosOut eos_rep(double vnewc, double e_old, double del
     e_new = e_new + 0.5 * delvc * (3.0 * (p_old + q_old) - 4.0 * (pHalfStep + q_new));
     e_new += 0.5 * work;
     if (fabs(e_new) < Phys::e_cut) e_new = 0.;
-    if (e_new < Phys::emin) e_new = Phys::emin;
+    e_new = raise_to(e_new, Phys::emin);
     bvc = c1s * (compression + 1.);
     pbvc = c1s;
     double p_new = bvc * e_new;
     if (fabs(p_new) < Phys::p_cut) p_new = 0.0;
     if (vnewc >= Phys::eosvmax) p_new = 0.0;
-    if (p_new < Phys::pmin) p_new = Phys::pmin;
+    p_new = raise_to(p_new, Phys::pmin);
     double q_tilde;
     if (delvc > 0.) q_tilde = 0.;
     else q_tilde = ss_of(pbvc, e_new, vnewc, bvc, p_new) * ql_old + qq_old;
     e_new = e_new - (7.0 * (p_old + q_old) - 8.0 * (pHalfStep + q_new) + (p_new + q_tilde)) * delvc * sixth;
     if (fabs(e_new) < Phys::e_cut) e_new = 0.;
-    if (e_new < Phys::emin) e_new = Phys::emin;
+    e_new = raise_to(e_new, Phys::emin);
     bvc = c1s * (compression + 1.);
     pbvc = c1s;
     p_new = bvc * e_new;
     if (fabs(p_new) < Phys::p_cut) p_new = 0.0;
     if (vnewc >= Phys::eosvmax) p_new = 0.0;
-    if (p_new < Phys::pmin) p_new = Phys::pmin;
+    p_new = raise_to(p_new, Phys::pmin);
     if (delvc <= 0.) {
         q_new = ss_of(pbvc, e_new, vnewc, bvc, p_new) * ql_old + qq_old;
         if (fabs(q_new) < Phys::q_cut) q_new = 0.;
@@ -1211,13 +1236,13 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     if (live) {
         // every load of the element first (none behind a branch), then the
         // arithmetic
-        const double gxi = gp(D.delv_xi)[el], geta = gp(D.delv_eta)[el], gzeta = gp(D.delv_zeta)[el];
-        vdov = gp(D.vdov)[el];
-        const double vnew_raw = gp(D.vnew)[el];
-        const double delx_xi = gp(D.delx_xi)[el], delx_eta = gp(D.delx_eta)[el];
-        const double delx_zeta = gp(D.delx_zeta)[el];
-        const double mass = gp(D.elemMass)[el], volo = gp(D.volo)[el];
-        const double q_old = gp(D.q)[el], e_old = gp(D.e)[el], delv = gp(D.delv)[el], p_old = gp(D.p)[el];
+        const double gxi = gld(D.delv_xi, el), geta = gld(D.delv_eta, el), gzeta = gld(D.delv_zeta, el);
+        vdov = gld(D.vdov, el);
+        const double vnew_raw = gld(D.vnew, el);
+        const double delx_xi = gld(D.delx_xi, el), delx_eta = gld(D.delx_eta, el);
+        const double delx_zeta = gld(D.delx_zeta, el);
+        const double mass = gld(D.elemMass, el), volo = gld(D.volo, el);
+        const double q_old = gld(D.q, el), e_old = gld(D.e, el), delv = gld(D.delv, el), p_old = gld(D.p, el);
         // --- MonoQ region ---
         int ix[6];
         bool keep[6];
@@ -1244,9 +1269,9 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
             ix[4] = face_idx(el, k == 0, el - nx * nx, D.ghost[0], j * nx + i, D.plane == 0, keep[4]);
             ix[5] = face_idx(el, k == nx - 1, el + nx * nx, D.ghost[1], j * nx + i, false, keep[5]);
         }
-        const double r0 = gp(D.delv_xi)[ix[0]], r1 = gp(D.delv_xi)[ix[1]];
-        const double r2 = gp(D.delv_eta)[ix[2]], r3 = gp(D.delv_eta)[ix[3]];
-        const double r4 = gp(D.delv_zeta)[ix[4]], r5 = gp(D.delv_zeta)[ix[5]];
+        const double r0 = gld(D.delv_xi, ix[0]), r1 = gld(D.delv_xi, ix[1]);
+        const double r2 = gld(D.delv_eta, ix[2]), r3 = gld(D.delv_eta, ix[3]);
+        const double r4 = gld(D.delv_zeta, ix[4]), r5 = gld(D.delv_zeta, ix[5]);
         // all loads issued; nothing above uses a loaded value
         __builtin_amdgcn_sched_barrier(0);
         const double xim = keep[0] ? r0 : 0.0, xip = keep[1] ? r1 : 0.0;
@@ -1278,8 +1303,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
 
         // --- EOS inputs (vnew clamped to [eosvmin, eosvmax], lulesh.cc:2642-2650) ---
         double vnewc = vnew_raw;
-        if (vnewc < Phys::eosvmin) vnewc = Phys::eosvmin;
-        if (vnewc > Phys::eosvmax) vnewc = Phys::eosvmax;
+        vnewc = fmin(raise_to(vnewc, Phys::eosvmin), Phys::eosvmax);
         s_in[0][tid] = vnewc;
         s_in[1][tid] = e_old;
         s_in[2][tid] = delv;
@@ -1317,20 +1341,20 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     if (live) {
         const double p_new = s_out[0][tid], e_new = s_out[1][tid];
         const double vnewc = s_in[0][tid];
-        gp(D.p)[el] = p_new;
-        gp(D.e)[el] = e_new;
-        gp(D.q)[el] = s_out[2][tid];
+        gst(D.p, el, p_new);
+        gst(D.e, el, e_new);
+        gst(D.q, el, s_out[2][tid]);
         const double ss = ss_of(s_out[4][tid], e_new, vnewc, s_out[3][tid], p_new);
-        gp(D.ss)[el] = ss;
+        gst(D.ss, el, ss);
         double vfin = vnewc;
         if (fabs(vfin - 1.0) < Phys::v_cut) vfin = 1.0;
-        gp(D.v)[el] = vfin;
+        gst(D.v, el, vfin);
         // the next cycle's hourglass coefficient needs exactly these ss, v
-        gp(D.hgcoef)[el] = hg_coefficient(ss, gp(D.elemMass)[el], gp(D.volo)[el], vfin);
+        gst(D.hgcoef, el, hg_coefficient(ss, gld(D.elemMass, el), gld(D.volo, el), vfin));
 
         // --- time constraints ---
         if (vdov != 0.) {
-            const double arealg = gp(D.arealg)[el];
+            const double arealg = gld(D.arealg, el);
             const double qqc2 = 64.0 * Phys::qqc * Phys::qqc;
             double dtf = ss * ss;
             if (vdov < 0.) dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;

@@ -29,6 +29,35 @@ __device__ __forceinline__ __attribute__((address_space(1))) T *gp(T *p)
     return (__attribute__((address_space(1))) T *)p;
 }
 
+// Element idx of an array whose base is wave-uniform (a field of a
+// descriptor copied with a uniform index), addressed as that base plus a
+// 32-bit unsigned byte offset.  The compiler then uses the global
+// instruction form with the base in scalar registers, and accesses to
+// several arrays at one index share one offset register instead of each
+// forming a 64-bit address.  Requires idx * sizeof(T) < 2^32.
+template <typename T>
+__device__ __forceinline__ T gld(const T *base, unsigned idx)
+{
+    return *(const __attribute__((address_space(1))) T *)(
+        (const __attribute__((address_space(1))) char *)base + idx * (unsigned)sizeof(T));
+}
+// elements idx and idx+1 of a double array in one 16-byte load (8-byte
+// alignment is enough for global loads)
+__device__ __forceinline__ void gld_pair(const double *base, unsigned idx, double &a, double &b)
+{
+    typedef double d2a __attribute__((ext_vector_type(2), aligned(8)));
+    const d2a v = *(const __attribute__((address_space(1))) d2a *)(
+        (const __attribute__((address_space(1))) char *)base + idx * 8u);
+    a = v.x;
+    b = v.y;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T *base, unsigned idx, T v)
+{
+    *(__attribute__((address_space(1))) T *)((__attribute__((address_space(1))) char *)base +
+                                             idx * (unsigned)sizeof(T)) = v;
+}
+
 
 // ---------------------------------------------------------------------------
 // Correctly rounded cube root.
