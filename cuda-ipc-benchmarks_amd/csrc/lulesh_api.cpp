@@ -1181,6 +1181,21 @@ double lulesh_probe_copy(size_t n, int reps)
     return ms / reps;
 }
 
+int lulesh_probe_range_math(const double *a, double *out2, int n)
+{
+    if (n < 0 || (n > 0 && (!a || !out2))) return LULESH_EINVAL;
+    if (n == 0) return LULESH_OK;
+    double *da = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc((void **)&da, (size_t)n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&dout, (size_t)n * 16);
+    if (e == hipSuccess) e = hipMemcpy(da, a, (size_t)n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_range_math(da, dout, n, 0);
+    if (e == hipSuccess) e = hipMemcpy(out2, dout, (size_t)n * 16, hipMemcpyDeviceToHost);
+    if (da) (void)hipFree(da);
+    if (dout) (void)hipFree(dout);
+    return e == hipSuccess ? LULESH_OK : LULESH_EHIP;
+}
+
 int lulesh_probe_sqrt_div(const double *a, const double *b, double *out2, int n)
 {
     if (n < 0) return LULESH_EINVAL;

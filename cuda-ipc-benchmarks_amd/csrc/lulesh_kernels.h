@@ -40,5 +40,6 @@ hipError_t launch_hg_coef(double *coef, const double *ss, const double *mass, co
 hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s);
 hipError_t launch_probe_cbrt(const double *in, double *out, int n, hipStream_t s);
 hipError_t launch_probe_sqrt_div(const double *a, const double *b, double *out, int n, hipStream_t s);
+hipError_t launch_probe_range_math(const double *a, double *out, int n, hipStream_t s);
 
 }  // namespace lulesh

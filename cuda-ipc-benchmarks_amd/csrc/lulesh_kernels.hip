@@ -958,7 +958,9 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
         gst(D.vdov, el, dxx + dyy + dzz);
     }
 
-    // MonoQ gradients at the end-of-step positions
+    // MonoQ gradients at the end-of-step positions (each sqrt argument
+    // below is a sum of squares plus ptiny: at least ptiny, in sqrt_pos's
+    // range)
     const double ptiny = 1.e-36;
     const double vol = volo * vnew;
     const double norm = 1.0 / (vol + ptiny);
@@ -974,7 +976,7 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
     double ax, ay, az, dxv, dyv, dzv;
 
     ax = dyi * dzj - dzi * dyj; ay = dzi * dxj - dxi * dzj; az = dxi * dyj - dyi * dxj;
-    gst(D.delx_zeta, el, vol / sqrt(ax * ax + ay * ay + az * az + ptiny));
+    gst(D.delx_zeta, el, vol / sqrt_pos(ax * ax + ay * ay + az * az + ptiny));
     ax *= norm; ay *= norm; az *= norm;
     dxv = 0.25 * ((xv[4] + xv[5] + xv[6] + xv[7]) - (xv[0] + xv[1] + xv[2] + xv[3]));
     dyv = 0.25 * ((yv[4] + yv[5] + yv[6] + yv[7]) - (yv[0] + yv[1] + yv[2] + yv[3]));
@@ -983,7 +985,7 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
     gst(D.delv_zeta, el, dv_zeta);
 
     ax = dyj * dzk - dzj * dyk; ay = dzj * dxk - dxj * dzk; az = dxj * dyk - dyj * dxk;
-    gst(D.delx_xi, el, vol / sqrt(ax * ax + ay * ay + az * az + ptiny));
+    gst(D.delx_xi, el, vol / sqrt_pos(ax * ax + ay * ay + az * az + ptiny));
     ax *= norm; ay *= norm; az *= norm;
     dxv = 0.25 * ((xv[1] + xv[2] + xv[6] + xv[5]) - (xv[0] + xv[3] + xv[7] + xv[4]));
     dyv = 0.25 * ((yv[1] + yv[2] + yv[6] + yv[5]) - (yv[0] + yv[3] + yv[7] + yv[4]));
@@ -992,7 +994,7 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
     gst(D.delv_xi, el, dv_xi);
 
     ax = dyk * dzi - dzk * dyi; ay = dzk * dxi - dxk * dzi; az = dxk * dyi - dyk * dxi;
-    gst(D.delx_eta, el, vol / sqrt(ax * ax + ay * ay + az * az + ptiny));
+    gst(D.delx_eta, el, vol / sqrt_pos(ax * ax + ay * ay + az * az + ptiny));
     ax *= norm; ay *= norm; az *= norm;
     dxv = -0.25 * ((xv[0] + xv[1] + xv[5] + xv[4]) - (xv[3] + xv[2] + xv[6] + xv[7]));
     dyv = -0.25 * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
@@ -1106,7 +1108,7 @@ __device__ __forceinline__ double ss_of(double pbvc, double e_new, double vv, do
 {
     double ssc = (pbvc * e_new + vv * vv * bvc * p_new) / Phys::refdens;
     if (ssc <= .1111111e-36) ssc = .3333333e-18;
-    else ssc = sqrt(ssc);
+    else ssc = sqrt_pos(ssc);                      // ssc > 1.1e-37 here
     return ssc;
 }
 
@@ -1128,7 +1130,7 @@ __device__ __forceinline__ EosOut eos_rep(double vnewc, double e_old, double del
                                           double q_old, double qq_old, double ql_old)
 {
     const double c1s = 2.0 / 3.0, sixth = 1.0 / 6.0;
-    double compression = 1. / vnewc - 1.;
+    double compression = rcp_pos(vnewc) - 1.;     // vnewc in [eosvmin, eosvmax]
     const double vchalf = vnewc - delvc * .5;
     double compHalfStep = 1. / vchalf - 1.;
     if (vnewc <= Phys::eosvmin) compHalfStep = compression;
@@ -1149,7 +1151,7 @@ __device__ __forceinline__ EosOut eos_rep(double vnewc, double e_old, double del
     } else {
         double ssc = (pbvc * e_new + vhalf * vhalf * bvc * pHalfStep) / Phys::refdens;
         if (ssc <= .1111111e-36) ssc = .3333333e-18;
-        else ssc = sqrt(ssc);
+        else ssc = sqrt_pos(ssc);                  // ssc > 1.1e-37 here
         q_new = (ssc * ql_old + qq_old);
     }
     e_new = e_new + 0.5 * delvc * (3.0 * (p_old + q_old) - 4.0 * (pHalfStep + q_new));
@@ -1358,7 +1360,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
             const double qqc2 = 64.0 * Phys::qqc * Phys::qqc;
             double dtf = ss * ss;
             if (vdov < 0.) dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;
-            dtf = sqrt(dtf);
+            dtf = sqrt_pos(dtf);                   // dtf >= ss^2 >= 1.1e-37
             dtf = arealg / dtf;
             const unsigned long long bb = dbits(dtf);
             cbits = bb < cbits ? bb : cbits;
@@ -1645,6 +1647,15 @@ __global__ void k_probe_sqrt_div(const double *a, const double *b, double *out, 
     }
 }
 
+__global__ void k_probe_range_math(const double *a, double *out, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        out[2 * t] = sqrt_pos(a[t]);
+        out[2 * t + 1] = rcp_pos(a[t]);
+    }
+}
+
 // streaming copy, the bandwidth yardstick of the profiling runs
 __global__ __launch_bounds__(256) void k_probe_copy(const double2 *__restrict__ a, double2 *__restrict__ b,
                                                     size_t n2)
@@ -1671,6 +1682,13 @@ hipError_t launch_probe_sqrt_div(const double *a, const double *b, double *out, 
 {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_probe_sqrt_div, dim3(grid_of(n, 256)), dim3(256), 0, s, a, b, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_range_math(const double *a, double *out, int n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_probe_range_math, dim3(grid_of(n, 256)), dim3(256), 0, s, a, out, n);
     return hipGetLastError();
 }
 

@@ -129,6 +129,46 @@ __device__ __forceinline__ double cbrt_cr(double x)
 }
 
 // ---------------------------------------------------------------------------
+// sqrt and 1/x on a range known from the physics.
+//
+// The compiler's correctly rounded FP64 sqrt is an rsq seed refined by
+// Newton/Goldschmidt FMA steps, wrapped in a scaling for x < 2^-767 and a
+// fix-up for zero and infinity; its 1/x is an rcp seed, two Newton steps and
+// an FMA correction, wrapped in v_div_scale / v_div_fmas / v_div_fixup.  On
+// the ranges below the wrappers are the identity (no scaling, no special
+// case), so the bare FMA steps -- the same operations in the same order --
+// give the same bits with 7 and 4 fewer instructions.  Checked against the
+// correctly rounded results on random inputs across each range.
+// ---------------------------------------------------------------------------
+
+// sqrt(x) for 2^-767 <= x < +inf (not for zero, infinity or NaN)
+__device__ __forceinline__ double sqrt_pos(double x)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r;
+    double h = r * 0.5;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+}
+
+// 1.0 / x for 2^-767 <= |x| < 2^1021
+__device__ __forceinline__ double rcp_pos(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    const double rem = fma(-x, r, 1.0);
+    return fma(rem, r, r);
+}
+
+// ---------------------------------------------------------------------------
 // element geometry
 // ---------------------------------------------------------------------------
 

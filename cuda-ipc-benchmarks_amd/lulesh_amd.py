@@ -100,6 +100,7 @@ SIGNATURES = {
     "lulesh_probe_cbrt": (_i, [_dp, _dp, _i]),
     "lulesh_probe_copy": (_d, [_sz, _i]),
     "lulesh_probe_sqrt_div": (_i, [_dp, _dp, _dp, _i]),
+    "lulesh_probe_range_math": (_i, [_dp, _dp, _i]),
 }
 
 _lib = None
@@ -545,4 +546,13 @@ def probe_sqrt_div(a, b):
     b = np.ascontiguousarray(b, dtype=np.float64)
     out = np.empty(2 * a.size, dtype=np.float64)
     _chk(load().lulesh_probe_sqrt_div(_p(a), _p(b), _p(out), a.size), "probe_sqrt_div")
+    return out[0::2], out[1::2]
+
+
+def probe_range_math(a):
+    """The kernels' range-limited sqrt and reciprocal of a, on the GPU."""
+    init()
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.empty(2 * a.size, dtype=np.float64)
+    _chk(load().lulesh_probe_range_math(_p(a), _p(out), a.size), "probe_range_math")
     return out[0::2], out[1::2]

@@ -355,6 +355,10 @@ int lulesh_probe_cbrt(const double *in, double *out, int n);
 double lulesh_probe_copy(size_t n, int reps);
 /* out2[2i] = sqrt(a[i]), out2[2i+1] = a[i] / b[i], computed on the GPU */
 int lulesh_probe_sqrt_div(const double *a, const double *b, double *out2, int n);
+/* out2[2i] = sqrt(a[i]) and out2[2i+1] = 1 / a[i] as the EOS and
+ * kinematics kernels form them where the physics bounds the argument
+ * (sqrt: 2^-767 <= a[i] < inf; 1/x: 2^-767 <= |a[i]| < 2^1021) */
+int lulesh_probe_range_math(const double *a, double *out2, int n);
 
 #ifdef __cplusplus
 }

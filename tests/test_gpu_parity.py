@@ -62,6 +62,24 @@ def test_probe_sqrt_div_ieee(amd):
     assert_bitwise(q, a / b, "div")
 
 
+def test_probe_range_math(amd):
+    # the kernels' bare-FMA sqrt and 1/x against the correctly rounded
+    # results over the whole range they are used on: log-uniform inputs,
+    # both range ends, exact squares and powers of two
+    rng = np.random.default_rng(5)
+    lo, hi = 2.0 ** -767, 2.0 ** 1021
+    a = np.exp(rng.uniform(np.log(lo), np.log(hi), 200000))
+    m = rng.uniform(1.0, 2.0, 20000)           # one binade, densely
+    sq = np.floor(rng.uniform(1, 2 ** 26, 10000)) ** 2 * 2.0 ** rng.integers(-700, 900, 10000).astype(float)
+    ends = np.array([lo, np.nextafter(lo, 1.0), np.nextafter(hi, 0.0), 1.0, 2.0, 0.5, 4.0, 1e-9, 1e9])
+    a = np.concatenate([a, m, sq, ends])
+    s, r = amd.probe_range_math(a)
+    assert_bitwise(s, np.sqrt(a), "sqrt_pos")
+    assert_bitwise(r, 1.0 / a, "rcp_pos")
+    s, r = amd.probe_range_math(-a[:1000])
+    assert_bitwise(r, 1.0 / -a[:1000], "rcp_pos negative")
+
+
 def _compare_state(g, o, what):
     for f in NODE:
         assert_bitwise(g.field(f), o.field(f), f"{what} {f}")
