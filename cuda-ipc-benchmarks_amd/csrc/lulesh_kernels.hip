@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
     const double determ = gp(D.volo)[el] * vv;
     double dvdx[8], dvdy[8], dvdz[8];
     volume_derivs(x, y, z, dvdx, dvdy, dvdz);
-    const double volinv = 1.0 / determ;
+    const double volinv = rcp_pos(determ);   // volo * v: positive unless the volume error is raised
     double hg[8][4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -282,7 +282,7 @@ __device__ __forceinline__ NodeNew node_advance(const DomDev &D, const NodeState
                                                 int nd, unsigned symm, double fx, double fy, double fz,
                                                 double dt)
 {
-    const double one_over_nMass = 1. / gp(D.nodalMass)[nd];
+    const double one_over_nMass = rcp_pos(gp(D.nodalMass)[nd]);   // masses are positive
     double ax = fx * one_over_nMass, ay = fy * one_over_nMass, az = fz * one_over_nMass;
     if (symm & 1u) ax = 0.0;
     if (symm & 2u) ay = 0.0;
@@ -362,7 +362,7 @@ __device__ __forceinline__ void hg_modes(const double x[8], const double y[8], c
     }
     double dvdx[8], dvdy[8], dvdz[8];
     volume_derivs(x, y, z, dvdx, dvdy, dvdz);
-    const double volinv = 1.0 / determ;
+    const double volinv = rcp_pos(determ);   // volo * v: positive unless the volume error is raised
 #pragma unroll
     for (int c = 0; c < 8; ++c)
 #pragma unroll
@@ -448,10 +448,10 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     // draining the whole queue.
     auto fetch = [&](int kn) {           // issue the loads of node plane kn
         const int kc = kn < 0 ? 0 : (kn > nx ? nx : kn);
-        const int nd = own_in ? kc * s2 + own_j * s1 + own_i : 0;
+        const unsigned nd = opaque_idx(own_in ? kc * s2 + own_j * s1 + own_i : 0);
         pf_own[0] = gld(S.x, nd); pf_own[1] = gld(S.y, nd); pf_own[2] = gld(S.z, nd);
         pf_own[3] = gld(S.xd, nd); pf_own[4] = gld(S.yd, nd); pf_own[5] = gld(S.zd, nd);
-        const int nt = xt_sl >= 0 ? kc * s2 + xt_j * s1 + xt_i : 0;
+        const unsigned nt = opaque_idx(xt_sl >= 0 ? kc * s2 + xt_j * s1 + xt_i : 0);
         pf_xt[0] = gld(S.x, nt); pf_xt[1] = gld(S.y, nt); pf_xt[2] = gld(S.z, nt);
         pf_xt[3] = gld(S.xd, nt); pf_xt[4] = gld(S.yd, nt); pf_xt[5] = gld(S.zd, nt);
     };
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     // element fields of plane c (p, q, v, volo, hgcoef), fetched a plane ahead
     double ef[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ef_next[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     auto fetch_elem = [&](int c, double out[5]) {
-        const int el = (col_ok && c >= 0 && c < nx) ? (c * nx + b) * nx + a : 0;
+        const unsigned el = opaque_idx((col_ok && c >= 0 && c < nx) ? (c * nx + b) * nx + a : 0);
         out[0] = gld(D.p, el); out[1] = gld(D.q, el); out[2] = gld(D.v, el); out[3] = gld(D.volo, el);
         out[4] = gld(D.hgcoef, el);
     };
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         const double (*Up)[kPatch] = sn[(c + 1) & 1];
         // this plane's nodal masses (first: the update waits for them
         // alone), then the next planes' loads, in flight meanwhile
-        const double nmass = gld(D.nodalMass, (node_ok && c >= 0) ? c * s2 + nj * s1 + ni : 0);
+        const double nmass = gld(D.nodalMass, opaque_idx((node_ok && c >= 0) ? c * s2 + nj * s1 + ni : 0));
         fetch(c + 2);
         fetch_elem(c + 1, ef_next);
 
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                 }
             }
             if (c >= k0) {
-                const int nd = c * s2 + nj * s1 + ni;
+                const unsigned nd = opaque_idx(c * s2 + nj * s1 + ni);
                 if (store_f) {
                     gst(D.fpx, nd, f[0]);
                     gst(D.fpy, nd, f[1]);
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                 if (advance) {
                     // the node is complete: advance it from its old state
                     // (its own slot of the lower plane) into the new buffer
-                    const double one_over_nMass = 1. / nmass;
+                    const double one_over_nMass = rcp_pos(nmass);   // masses are positive
                     double ax = f[0] * one_over_nMass, ay = f[1] * one_over_nMass,
                            az = f[2] * one_over_nMass;
                     if (D.col == 0 && ni == 0) ax = 0.0;
@@ -855,7 +855,7 @@ __device__ __forceinline__ void node_update_pv(const DomDev *doms, const DomDev 
         sbn_sum(doms, W, P.fw_idx, gx, gy, gz);
         const auto old = gp(W.pv_snap) + (size_t)(par ^ 1) * 6 * W.pv_nslot + P.fw_slot;
         const size_t ns = W.pv_nslot;
-        const double one_over_nMass = 1. / gp(W.nodalMass)[P.fw_idx];
+        const double one_over_nMass = rcp_pos(gp(W.nodalMass)[P.fw_idx]);   // masses are positive
         double ax = gx * one_over_nMass, ay = gy * one_over_nMass, az = gz * one_over_nMass;
         if (symm & 1u) ax = 0.0;
         if (symm & 2u) ay = 0.0;
@@ -947,7 +947,7 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
         const Jac64 J = elem_jacobian64(xh, yh, zh);
         double b[3][4];
         shape_b4(J, b);
-        const double inv_detJ = 1.0 / J.volume;
+        const double inv_detJ = rcp_pos(J.volume);   // positive for an element that is not inverted
         // CalcElemVelocityGradient diagonal, lulesh.cc:1809-1822
         const double dxx = inv_detJ * (b[0][0] * (xv[0] - xv[6]) + b[0][1] * (xv[1] - xv[7]) +
                                        b[0][2] * (xv[2] - xv[4]) + b[0][3] * (xv[3] - xv[5]));
@@ -963,7 +963,7 @@ __device__ __forceinline__ void kin_element(const DomDev *doms, const DomDev &D,
     // range)
     const double ptiny = 1.e-36;
     const double vol = volo * vnew;
-    const double norm = 1.0 / (vol + ptiny);
+    const double norm = rcp_pos(vol + ptiny);   // >= ptiny unless the volume error is raised
     const double dxj = -0.25 * ((x[0] + x[1] + x[5] + x[4]) - (x[3] + x[2] + x[6] + x[7]));
     const double dyj = -0.25 * ((y[0] + y[1] + y[5] + y[4]) - (y[3] + y[2] + y[6] + y[7]));
     const double dzj = -0.25 * ((z[0] + z[1] + z[5] + z[4]) - (z[3] + z[2] + z[6] + z[7]));

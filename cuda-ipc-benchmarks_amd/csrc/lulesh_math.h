@@ -41,6 +41,16 @@ __device__ __forceinline__ T gld(const T *base, unsigned idx)
     return *(const __attribute__((address_space(1))) T *)(
         (const __attribute__((address_space(1))) char *)base + idx * (unsigned)sizeof(T));
 }
+// An index the compiler must treat as an opaque 32-bit value (an empty asm
+// statement that claims to update it): in a loop, strength reduction can
+// then not turn base + offset back into per-array 64-bit pointer
+// arithmetic, and gld()/gst() keep the scalar-base form.
+__device__ __forceinline__ unsigned opaque_idx(int i)
+{
+    unsigned u = (unsigned)i;
+    __asm__("" : "+v"(u));
+    return u;
+}
 // elements idx and idx+1 of a double array in one 16-byte load (8-byte
 // alignment is enough for global loads)
 __device__ __forceinline__ void gld_pair(const double *base, unsigned idx, double &a, double &b)
