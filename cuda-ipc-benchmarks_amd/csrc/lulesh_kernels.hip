@@ -75,8 +75,10 @@ __global__ __launch_bounds__(256) void k_time_increment(GroupCtl *const *groups,
     const int r = threadIdx.x & 63;
     if (gi >= ngroups) return;
     GroupCtl &G = *groups[gi];
+    // the DomCtl array follows the GroupCtl in its allocation: addressed
+    // directly, its loads need not wait for G.dctl
+    DomCtl *D = reinterpret_cast<DomCtl *>(&G + 1);
     if (group_stopped(G)) return;
-    DomCtl *D = G.dctl;
     if (!(D[0].time < G.stoptime) || (G.max_cycles >= 0 && D[0].cycle >= G.max_cycles)) {
         if (r == 0) G.done = 1;
         return;
@@ -403,7 +405,9 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     // its descriptor, which no store in the kernel can be taken to alias
     const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     GroupCtl *G = D.grp;
-    if (group_stopped(*G)) return;
+    // the stop flag is tested once the first plane's loads are in flight
+    // (reads only: a stopped group's arrays are still allocated)
+    const bool stopped = group_stopped(*G);
     const int nx = D.nx, s1 = nx + 1, s2 = s1 * s1;
     const int tid = threadIdx.x;
     const int ex = t.ex, ey = t.ey, px = ex + 1;
@@ -474,6 +478,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     };
 
     fetch(k0 - 1);
+    if (stopped) return;
     stage(k0 - 1);
     fetch(k0);
     stage(k0);
@@ -1028,7 +1033,9 @@ __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Ti
     // its descriptor, which no store in the kernel can be taken to alias
     const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     GroupCtl *G = D.grp;
-    if (group_stopped(*G)) return;
+    // the stop flag is tested once the node loads are in flight (reads
+    // only: a stopped group's arrays are still allocated)
+    const bool stopped = group_stopped(*G);
     const int el = t.start + threadIdx.x;
     if (el >= D.numElem) return;
     int n[8];
@@ -1056,6 +1063,7 @@ __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Ti
             gld_pair(A[f], n[7], V[f][7], V[f][6]);
         }
     }
+    if (stopped) return;
     kin_element(doms, D, G, el, dt, x, y, z, xv, yv, zv);
 }
 
@@ -1224,7 +1232,9 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     // its descriptor, which no store in the kernel can be taken to alias
     const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
     GroupCtl *G = D.grp;
-    if (group_stopped(*G)) return;
+    // the stop flag is tested after the element phase's loads (reads and
+    // LDS writes only: a stopped group's arrays are still allocated)
+    const bool stopped = group_stopped(*G);
     const double qstop = *gp(&G->qstop);
     // the tile's first EOS table entry, in flight during the element phase
     const auto tab = gp(D.evtab) + t.eoff;
@@ -1314,6 +1324,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         s_in[5][tid] = qquad;
         s_in[6][tid] = qlin;
     }
+    if (stopped) return;   // uniform over the workgroup
     __syncthreads();
 
     // --- EOS repetitions, dealt over the workgroup ---
