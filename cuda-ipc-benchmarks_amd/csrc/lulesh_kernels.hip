@@ -1129,8 +1129,20 @@ __device__ __forceinline__ double raise_to(double x, double c)
 }
 
 struct EosOut {
-    double p, e, q, bvc, pbvc;
+    double p, e, q;
 };
+
+// the relative compression 1/v - 1 of EvalEOSForElems (lulesh.cc:2669-2676),
+// zero at eosvmax; vnewc is already clamped to [eosvmin, eosvmax].  The
+// last pressure evaluation's bvc is c1s * (compression + 1) and its pbvc
+// c1s (lulesh.cc:2786-2789): the sound speed after the repetitions forms
+// both again from vnewc instead of keeping them from the last repetition.
+__device__ __forceinline__ double eos_compression(double vnewc)
+{
+    double compression = rcp_pos(vnewc) - 1.;
+    if (vnewc >= Phys::eosvmax) compression = 0.;
+    return compression;
+}
 
 // one rep of EvalEOSForElems (lulesh.cc:2677-2815); every rep recomputes
 // the same values from the same inputs
@@ -1138,11 +1150,11 @@ __device__ __forceinline__ EosOut eos_rep(double vnewc, double e_old, double del
                                           double q_old, double qq_old, double ql_old)
 {
     const double c1s = 2.0 / 3.0, sixth = 1.0 / 6.0;
-    double compression = rcp_pos(vnewc) - 1.;     // vnewc in [eosvmin, eosvmax]
+    const double compression = eos_compression(vnewc);
     const double vchalf = vnewc - delvc * .5;
     double compHalfStep = 1. / vchalf - 1.;
     if (vnewc <= Phys::eosvmin) compHalfStep = compression;
-    if (vnewc >= Phys::eosvmax) { p_old = 0.; compression = 0.; compHalfStep = 0.; }
+    if (vnewc >= Phys::eosvmax) { p_old = 0.; compHalfStep = 0.; }
     const double work = 0.;
     double e_new = e_old - 0.5 * delvc * (p_old + q_old) + 0.5 * work;
     e_new = raise_to(e_new, Phys::emin);
@@ -1189,7 +1201,7 @@ __device__ __forceinline__ EosOut eos_rep(double vnewc, double e_old, double del
         if (fabs(q_new) < Phys::q_cut) q_new = 0.;
     }
     EosOut o;
-    o.p = p_new; o.e = e_new; o.q = q_new; o.bvc = bvc; o.pbvc = pbvc;
+    o.p = p_new; o.e = e_new; o.q = q_new;
     return o;
 }
 
@@ -1225,8 +1237,10 @@ __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v)
 template <bool U>
 __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *tiles, int ntiles, int te)
 {
+    // 20 KB of LDS in all (eight workgroups per CU): the EOS inputs and
+    // the three outputs; the workgroup minima reuse an input row
     __shared__ double s_in[7][256];
-    __shared__ double s_out[5][256];
+    __shared__ double s_out[3][256];
     const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     // the tile's subdomain is uniform over the workgroup: one scalar copy of
     // its descriptor, which no store in the kernel can be taken to alias
@@ -1342,8 +1356,6 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
                 s_out[0][o] = r.p;
                 s_out[1][o] = r.e;
                 s_out[2][o] = r.q;
-                s_out[3][o] = r.bvc;
-                s_out[4][o] = r.pbvc;
             }
             ent = next;
         }
@@ -1357,7 +1369,8 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         gst(D.p, el, p_new);
         gst(D.e, el, e_new);
         gst(D.q, el, s_out[2][tid]);
-        const double ss = ss_of(s_out[4][tid], e_new, vnewc, s_out[3][tid], p_new);
+        const double c1s = 2.0 / 3.0;
+        const double ss = ss_of(c1s, e_new, vnewc, c1s * (eos_compression(vnewc) + 1.), p_new);
         gst(D.ss, el, ss);
         double vfin = vnewc;
         if (fabs(vfin - 1.0) < Phys::v_cut) vfin = 1.0;
@@ -1384,19 +1397,20 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     // reduce the workgroup first, then one atomic per workgroup and only
     // when it can lower the stored minimum (all workgroups of a subdomain
     // target the same two words)
-    __shared__ unsigned long long s_min[2][4];
+    // (the last input row is free since the barrier after the repetitions)
+    unsigned long long *s_min = reinterpret_cast<unsigned long long *>(&s_in[6][0]);
     cbits = wave_min_u64(cbits);
     hbits = wave_min_u64(hbits);
     if (lane == 0) {
-        s_min[0][tid >> 6] = cbits;
-        s_min[1][tid >> 6] = hbits;
+        s_min[tid >> 6] = cbits;
+        s_min[4 + (tid >> 6)] = hbits;
     }
     __syncthreads();
     if (tid == 0) {
         const int nw = (blockDim.x + 63) >> 6;
         for (int w = 1; w < nw; ++w) {
-            cbits = s_min[0][w] < cbits ? s_min[0][w] : cbits;
-            hbits = s_min[1][w] < hbits ? s_min[1][w] : hbits;
+            cbits = s_min[w] < cbits ? s_min[w] : cbits;
+            hbits = s_min[4 + w] < hbits ? s_min[4 + w] : hbits;
         }
         DomCtl &C = *D.ctl;
         if (cbits < *(volatile unsigned long long *)&C.dtcourant_bits) atomicMin(&C.dtcourant_bits, cbits);
