@@ -352,6 +352,8 @@ def bench_serve(args):
         "wall_s": t_end - t_start, "failed": len(bad),
         "offered_load": zc / float(arrivals[-1]),
         "sequential_plain_sample": {"requests": len(sample), "value": seq_rate},
+        # below 1 while the stream is arrival-bound (value ~ offered_load)
+        "speedup_vs_sequential": (zc / (t_end - t_start)) / seq_rate,
         "fifo_sequential_model_latency_s": {"p50": float(np.percentile(fifo, 50)),
                                             "p99": float(np.percentile(fifo, 99)),
                                             "mean": float(fifo.mean())},
