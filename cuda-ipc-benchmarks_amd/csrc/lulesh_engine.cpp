@@ -60,13 +60,15 @@ static void engine_drop_graph(Engine *e)
 
 // Column-tile shape for a mesh of nx elements per edge: ex x ey element
 // slots (<= 256) covering (ex-1) x (ey-1) nodes each, with an
-// (ex+1) x (ey+1) node patch per plane in LDS; pick the shape with the
-// fewest busy waves over the whole node plane (boundary tiles are partly
-// empty, so 16 x 16 is not always best for small meshes).
+// (ex+1) x (ey+1) node patch per plane in LDS.  The kernel's LDS allows two
+// workgroups per CU whatever their wave count, and a workgroup's plane
+// step takes about as long with three waves as with four, so the cost is
+// the number of tiles: pick the fewest tiles per node plane, then the
+// fewest busy waves, then the fewest element slots.
 static void tile_shape(int nx, int &ex, int &ey)
 {
     const int s1 = nx + 1;
-    long best_w = -1, best_s = -1;
+    long best_t = -1, best_w = -1, best_s = -1;
     ex = ey = 16;
     for (int a = 4; a <= 64; ++a)
         for (int b = 4; b <= 64 && a * b <= 256; ++b) {
@@ -74,7 +76,9 @@ static void tile_shape(int nx, int &ex, int &ey)
             const long tiles = (long)((s1 + a - 2) / (a - 1)) * ((s1 + b - 2) / (b - 1));
             const long waves = tiles * ((a * b + 63) / 64);   // busy waves per plane
             const long slots = tiles * a * b;
-            if (best_w < 0 || waves < best_w || (waves == best_w && slots < best_s)) {
+            if (best_t < 0 || tiles < best_t || (tiles == best_t && waves < best_w) ||
+                (tiles == best_t && waves == best_w && slots < best_s)) {
+                best_t = tiles;
                 best_w = waves;
                 best_s = slots;
                 ex = a;
