@@ -1144,17 +1144,17 @@ int lulesh_group_exchange(lulesh_group *g, int msgType, int nf, const int *field
 // element-wise probes of the device arithmetic (parity tests)
 int lulesh_probe_cbrt(const double *host_in, double *host_out, int n)
 {
-    if (n < 0) return LULESH_EINVAL;
+    if (n < 0 || (n > 0 && (!host_in || !host_out))) return LULESH_EINVAL;
     if (n == 0) return LULESH_OK;
     double *din = nullptr, *dout = nullptr;
-    HIPCHK(hipMalloc((void **)&din, (size_t)n * 8));
-    HIPCHK(hipMalloc((void **)&dout, (size_t)n * 8));
-    HIPCHK(hipMemcpy(din, host_in, (size_t)n * 8, hipMemcpyHostToDevice));
-    HIPCHK(launch_probe_cbrt(din, dout, n, 0));
-    HIPCHK(hipMemcpy(host_out, dout, (size_t)n * 8, hipMemcpyDeviceToHost));
-    (void)hipFree(din);
-    (void)hipFree(dout);
-    return LULESH_OK;
+    hipError_t e = hipMalloc((void **)&din, (size_t)n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&dout, (size_t)n * 8);
+    if (e == hipSuccess) e = hipMemcpy(din, host_in, (size_t)n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_cbrt(din, dout, n, 0);
+    if (e == hipSuccess) e = hipMemcpy(host_out, dout, (size_t)n * 8, hipMemcpyDeviceToHost);
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    return e == hipSuccess ? LULESH_OK : LULESH_EHIP;
 }
 
 double lulesh_probe_copy(size_t n, int reps)
@@ -1198,20 +1198,20 @@ int lulesh_probe_range_math(const double *a, double *out2, int n)
 
 int lulesh_probe_sqrt_div(const double *a, const double *b, double *out2, int n)
 {
-    if (n < 0) return LULESH_EINVAL;
+    if (n < 0 || (n > 0 && (!a || !b || !out2))) return LULESH_EINVAL;
     if (n == 0) return LULESH_OK;
     double *da = nullptr, *db = nullptr, *dout = nullptr;
-    HIPCHK(hipMalloc((void **)&da, (size_t)n * 8));
-    HIPCHK(hipMalloc((void **)&db, (size_t)n * 8));
-    HIPCHK(hipMalloc((void **)&dout, (size_t)n * 16));
-    HIPCHK(hipMemcpy(da, a, (size_t)n * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(db, b, (size_t)n * 8, hipMemcpyHostToDevice));
-    HIPCHK(launch_probe_sqrt_div(da, db, dout, n, 0));
-    HIPCHK(hipMemcpy(out2, dout, (size_t)n * 16, hipMemcpyDeviceToHost));
-    (void)hipFree(da);
-    (void)hipFree(db);
-    (void)hipFree(dout);
-    return LULESH_OK;
+    hipError_t e = hipMalloc((void **)&da, (size_t)n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&db, (size_t)n * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&dout, (size_t)n * 16);
+    if (e == hipSuccess) e = hipMemcpy(da, a, (size_t)n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, b, (size_t)n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_probe_sqrt_div(da, db, dout, n, 0);
+    if (e == hipSuccess) e = hipMemcpy(out2, dout, (size_t)n * 16, hipMemcpyDeviceToHost);
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dout) (void)hipFree(dout);
+    return e == hipSuccess ? LULESH_OK : LULESH_EHIP;
 }
 
 }  // extern "C"
