@@ -525,6 +525,10 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             for (int q = 0; q < 24; ++q) lds[q][tid] = 0.0;
         }
         __syncthreads();
+        // the gathers between barriers are short and latency-bound: they
+        // issue ahead of the other workgroup's element arithmetic on the
+        // same SIMD, so this workgroup reaches its next barrier sooner
+        __builtin_amdgcn_s_setprio(2);
         if (node_ok) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -547,6 +551,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             }
         }
         __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
         // ---- hourglass corners ----
         if (el_ok) {
             const double vv = ef[2];
@@ -578,6 +583,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             for (int q = 0; q < 24; ++q) lds[q][tid] = 0.0;
         }
         __syncthreads();
+        __builtin_amdgcn_s_setprio(2);
         if (node_ok) {
             double f[3];
 #pragma unroll
@@ -637,6 +643,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
 #pragma unroll
         for (int q = 0; q < 5; ++q) ef[q] = ef_next[q];
         __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
     }
     if (vol_err) raise_status(G, -1);
 }
