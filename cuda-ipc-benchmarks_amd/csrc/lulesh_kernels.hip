@@ -450,24 +450,24 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     // unused -- so the number of loads in flight is the same on all paths
     // and the waits for the older ones can be counted exactly instead of
     // draining the whole queue.
-    auto fetch = [&](int kn) {           // issue the loads of node plane kn
+    auto fetch = [&](int kn, double own[6], double xt[6]) {   // issue the loads of node plane kn
         const int kc = kn < 0 ? 0 : (kn > nx ? nx : kn);
         const unsigned nd = opaque_idx(own_in ? kc * s2 + own_j * s1 + own_i : 0);
-        pf_own[0] = gld(S.x, nd); pf_own[1] = gld(S.y, nd); pf_own[2] = gld(S.z, nd);
-        pf_own[3] = gld(S.xd, nd); pf_own[4] = gld(S.yd, nd); pf_own[5] = gld(S.zd, nd);
+        own[0] = gld(S.x, nd); own[1] = gld(S.y, nd); own[2] = gld(S.z, nd);
+        own[3] = gld(S.xd, nd); own[4] = gld(S.yd, nd); own[5] = gld(S.zd, nd);
         const unsigned nt = opaque_idx(xt_sl >= 0 ? kc * s2 + xt_j * s1 + xt_i : 0);
-        pf_xt[0] = gld(S.x, nt); pf_xt[1] = gld(S.y, nt); pf_xt[2] = gld(S.z, nt);
-        pf_xt[3] = gld(S.xd, nt); pf_xt[4] = gld(S.yd, nt); pf_xt[5] = gld(S.zd, nt);
+        xt[0] = gld(S.x, nt); xt[1] = gld(S.y, nt); xt[2] = gld(S.z, nt);
+        xt[3] = gld(S.xd, nt); xt[4] = gld(S.yd, nt); xt[5] = gld(S.zd, nt);
     };
-    auto stage = [&](int kn) {           // write the fetched plane kn to LDS
+    auto stage = [&](int kn, const double own[6], const double xt[6]) {   // write plane kn to LDS
         if (kn < 0 || kn > nx) return;
         double (*P)[kPatch] = sn[kn & 1];
         if (own_in)
 #pragma unroll
-            for (int f = 0; f < 6; ++f) P[f][own_sl] = pf_own[f];
+            for (int f = 0; f < 6; ++f) P[f][own_sl] = own[f];
         if (xt_sl >= 0)
 #pragma unroll
-            for (int f = 0; f < 6; ++f) P[f][xt_sl] = pf_xt[f];
+            for (int f = 0; f < 6; ++f) P[f][xt_sl] = xt[f];
     };
     // element fields of plane c (p, q, v, volo, hgcoef), fetched a plane ahead
     double ef[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ef_next[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -477,12 +477,16 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         out[4] = gld(D.hgcoef, el);
     };
 
-    fetch(k0 - 1);
-    if (stopped) return;
-    stage(k0 - 1);
-    fetch(k0);
-    stage(k0);
-    fetch_elem(k0 - 1, ef);
+    {
+        // the first two planes' loads all in flight before either is staged
+        double own1[6], xt1[6];
+        fetch(k0 - 1, pf_own, pf_xt);
+        fetch(k0, own1, xt1);
+        fetch_elem(k0 - 1, ef);
+        if (stopped) return;
+        stage(k0 - 1, pf_own, pf_xt);
+        stage(k0, own1, xt1);
+    }
     __syncthreads();
 
     // a non-positive volume (lulesh.cc:1263-1265, 1340-1349) is reported
@@ -497,7 +501,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         // this plane's nodal masses (first: the update waits for them
         // alone), then the next planes' loads, in flight meanwhile
         const double nmass = gld(D.nodalMass, opaque_idx((node_ok && c >= 0) ? c * s2 + nj * s1 + ni : 0));
-        fetch(c + 2);
+        fetch(c + 2, pf_own, pf_xt);
         fetch_elem(c + 1, ef_next);
 
         // ---- stress corners ----
@@ -639,7 +643,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         }
         // the lower plane is done with (this thread's own slot was read
         // just above): it takes node plane c+2
-        if (c + 2 <= k1) stage(c + 2);
+        if (c + 2 <= k1) stage(c + 2, pf_own, pf_xt);
 #pragma unroll
         for (int q = 0; q < 5; ++q) ef[q] = ef_next[q];
         __syncthreads();
