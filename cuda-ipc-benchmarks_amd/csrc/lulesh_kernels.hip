@@ -203,9 +203,10 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
         const double sig16 = (-gp(D.p)[el] - gp(D.q)[el]) * 0.0625;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-            gp(D.corner)[(0 * 8 + c) * E + el] = -(sig16 * bx[c]);
-            gp(D.corner)[(1 * 8 + c) * E + el] = -(sig16 * by[c]);
-            gp(D.corner)[(2 * 8 + c) * E + el] = -(sig16 * bz[c]);
+            // one scalar base per (component, corner) row, one offset for all
+            gst(D.corner + (0 * 8 + c) * E, el, -(sig16 * bx[c]));
+            gst(D.corner + (1 * 8 + c) * E, el, -(sig16 * by[c]));
+            gst(D.corner + (2 * 8 + c) * E, el, -(sig16 * bz[c]));
         }
     }
 
@@ -246,8 +247,8 @@ __global__ __launch_bounds__(256) void k_elem_force(const DomDev *doms, const Ti
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c)
-            gp(D.corner)[((3 + comp) * 8 + c) * E + el] =
-                coefficient * (hg[c][0] * h[0] + hg[c][1] * h[1] + hg[c][2] * h[2] + hg[c][3] * h[3]);
+            gst(D.corner + ((3 + comp) * 8 + c) * E, el,
+                coefficient * (hg[c][0] * h[0] + hg[c][1] * h[1] + hg[c][2] * h[2] + hg[c][3] * h[3]));
     }
 }
 
@@ -669,20 +670,22 @@ __device__ __forceinline__ void node_gather_u(const DomDev &D, int nd)
     // corners in batches of 8 (a hexahedral mesh node has at most 8 unless
     // the mesh is irregular): every load of a batch is issued before the
     // first add, then the adds run in list order
+    // an unstructured mesh has fewer than 2^26 elements, so a corner's
+    // offset within one component's 8 rows fits 32 bits: scalar component
+    // bases and one offset register per corner
     for (int b = 0; b < cnt; b += 8) {
-        size_t off[8];
+        unsigned off[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int ent = b + t < cnt ? gp(D.u_ncorner)[q0 + b + t] : 0;
-            off[t] = (size_t)(ent & 7) * E + (size_t)(ent >> 3);
+            off[t] = (unsigned)(ent & 7) * (unsigned)E + (unsigned)(ent >> 3);
         }
+        // loaded on every path (a missing entry reads element 0's corner 0)
         double v[6][8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const bool ok = b + t < cnt;
+        for (int t = 0; t < 8; ++t)
 #pragma unroll
-            for (int comp = 0; comp < 6; ++comp) v[comp][t] = ok ? gp(D.corner)[comp * 8 * E + off[t]] : 0.0;
-        }
+            for (int comp = 0; comp < 6; ++comp) v[comp][t] = gld(D.corner + comp * 8 * E, (unsigned)off[t]);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             if (b + t >= cnt) break;

@@ -95,7 +95,7 @@ void face_bits(const lulesh_mesh &m, const std::vector<char> &mark, std::vector<
 
 int mesh_finish(lulesh_mesh *m)
 {
-    if (m->numElem < 1 || m->numNode < 1 || (long long)m->numElem * 8 > INT_MAX ||
+    if (m->numElem < 1 || m->numNode < 1 || m->numElem >= kMaxMeshElems ||
         (long long)m->numNode * 8 > INT_MAX)
         return LULESH_EINVAL;
     if ((long long)m->x.size() != m->numNode || (long long)m->nodelist.size() != 8LL * m->numElem ||
@@ -147,7 +147,7 @@ int lulesh_mesh_read(const char *path, lulesh_mesh **out)
     };
     int ne, nn;
     if (!t.next_int(ne) || !t.next_int(nn)) return fail(LULESH_EIO);
-    if (ne < 1 || nn < 1 || (long long)ne * 8 > INT_MAX || (long long)nn * 8 > INT_MAX)
+    if (ne < 1 || nn < 1 || ne >= kMaxMeshElems || (long long)nn * 8 > INT_MAX)
         return fail(LULESH_EINVAL);
     m->numElem = ne;
     m->numNode = nn;
@@ -183,7 +183,7 @@ int lulesh_mesh_create(int numElem, int numNode, const double *x, const double *
                        int nSymmY, const int *symmZ, int nSymmZ, const int *freeSurf, int nFree,
                        lulesh_mesh **out)
 {
-    if (!out || numElem < 1 || numNode < 1 || (long long)numElem * 8 > INT_MAX ||
+    if (!out || numElem < 1 || numNode < 1 || numElem >= kMaxMeshElems ||
         (long long)numNode * 8 > INT_MAX || !x || !y || !z ||
         !nodelist || !nbr || nSymmX < 0 || nSymmY < 0 || nSymmZ < 0 || nFree < 0 ||
         (nSymmX && !symmX) || (nSymmY && !symmY) || (nSymmZ && !symmZ) || (nFree && !freeSurf))

@@ -19,6 +19,11 @@ struct lulesh_mesh {
 
 namespace lulesh {
 
+// element limit of an unstructured mesh: the ordered node gather addresses
+// one component's eight corner rows (8 x numElem doubles) with 32-bit
+// byte offsets
+constexpr int kMaxMeshElems = 1 << 26;
+
 // validates every index and derives elemBC and the octant corner
 int mesh_finish(lulesh_mesh *m);
 

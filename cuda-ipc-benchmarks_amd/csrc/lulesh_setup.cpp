@@ -115,8 +115,7 @@ bool problem_ok(int nx, int tp, int nr, int balance, int cost)
 
 bool mesh_problem_ok(int numElem, int nr, int balance, int cost)
 {
-    if (numElem < 1 || nr < 1 || balance < 0 || cost < 0) return false;
-    if ((long long)numElem * 8 > INT_MAX) return false;
+    if (numElem < 1 || numElem >= kMaxMeshElems || nr < 1 || balance < 0 || cost < 0) return false;
     return (long long)numElem * max_elem_rep(nr, cost) <= INT_MAX;
 }
 

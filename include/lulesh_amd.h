@@ -107,9 +107,9 @@ int lulesh_group_destroy(lulesh_group *g);
  * bits are then exactly XI_M|ETA_M|ZETA_M symmetric is the octant corner,
  * and a face whose four nodes are all free-surface nodes also gets its
  * *_SYMM bit (the reference's choice, kept).  LULESH_EIO: unreadable or
- * truncated file; LULESH_EINVAL: an index out of range, or numElem or
- * numNode above 2^28 - 1 (the kernels address each array with 32-bit
- * byte offsets).  lulesh_mesh_create takes the same content from arrays
+ * truncated file; LULESH_EINVAL: an index out of range, numElem above
+ * 2^26 - 1 or numNode above 2^28 - 1 (the kernels address the arrays with
+ * 32-bit byte offsets).  lulesh_mesh_create takes the same content from arrays
  * (nodelist [numElem][8], nbr [numElem][6]). */
 int lulesh_mesh_read(const char *path, lulesh_mesh **out);
 int lulesh_mesh_create(int numElem, int numNode, const double *x, const double *y, const double *z,

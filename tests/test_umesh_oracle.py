@@ -94,12 +94,13 @@ def test_library_reader_errors(tmp_path):
     lines[1 + 1126] = "0 5 6 7 1 2 3 99999"          # node id out of range
     bad.write_text("\n".join(lines))
     assert L.lulesh_mesh_read(str(bad).encode(), ctypes.byref(h)) == la.EINVAL
-    # element and node counts above 2^28 - 1 are refused before any array is read
+    # element counts above 2^26 - 1 and node counts above 2^28 - 1 are
+    # refused before any array is read
     one = np.zeros(8, dtype=np.float64)
     idx = np.zeros(8, dtype=np.int32)
     dp = one.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     ip = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-    for ne, nn in [(1, 1 << 28), (1 << 28, 8)]:
+    for ne, nn in [(1, 1 << 28), (1 << 28, 8), (1 << 26, 8)]:
         assert L.lulesh_mesh_create(ne, nn, dp, dp, dp, ip, ip, ip, 0, ip, 0, ip, 0, ip, 0,
                                     ctypes.byref(h)) == la.EINVAL
 
