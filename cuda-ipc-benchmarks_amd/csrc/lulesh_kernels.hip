@@ -1272,7 +1272,8 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     const int el = t.start + tid;
     const bool live = tid < te && el < D.numElem;
     bool q_err = false;   // raised after the element's work: no atomic on the load path
-    double vdov = 0.0;
+    // also read by the final phase: loaded once, with the element's other fields
+    double vdov = 0.0, mass = 0.0, volo = 0.0, arealg = 0.0;
     if (live) {
         // every load of the element first (none behind a branch), then the
         // arithmetic
@@ -1281,7 +1282,9 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         const double vnew_raw = gld(D.vnew, el);
         const double delx_xi = gld(D.delx_xi, el), delx_eta = gld(D.delx_eta, el);
         const double delx_zeta = gld(D.delx_zeta, el);
-        const double mass = gld(D.elemMass, el), volo = gld(D.volo, el);
+        mass = gld(D.elemMass, el);
+        volo = gld(D.volo, el);
+        arealg = gld(D.arealg, el);
         const double q_old = gld(D.q, el), e_old = gld(D.e, el), delv = gld(D.delv, el), p_old = gld(D.p, el);
         // --- MonoQ region ---
         int ix[6];
@@ -1390,11 +1393,10 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         if (fabs(vfin - 1.0) < Phys::v_cut) vfin = 1.0;
         gst(D.v, el, vfin);
         // the next cycle's hourglass coefficient needs exactly these ss, v
-        gst(D.hgcoef, el, hg_coefficient(ss, gld(D.elemMass, el), gld(D.volo, el), vfin));
+        gst(D.hgcoef, el, hg_coefficient(ss, mass, volo, vfin));
 
         // --- time constraints ---
         if (vdov != 0.) {
-            const double arealg = gld(D.arealg, el);
             const double qqc2 = 64.0 * Phys::qqc * Phys::qqc;
             double dtf = ss * ss;
             if (vdov < 0.) dtf = dtf + qqc2 * arealg * arealg * vdov * vdov;
