@@ -103,6 +103,23 @@ def test_single_domain_bitwise(amd, oracle_mod, nx, cycles):
     o.close()
 
 
+@pytest.mark.parametrize("nx,cycles", [(20, 30), (45, 12), (80, 6)])
+def test_force_tile_shapes_bitwise(amd, oracle_mod, nx, cycles):
+    # meshes whose force-march column tiles are not the 14 x 18 of -s 100
+    # (8 x 22, 9 x 24, 9 x 28 element slots) and whose z-chunks end in a
+    # short chunk: the same state as the oracle, bit for bit
+    print(f"case: tile shapes s={nx} {cycles} cycles", flush=True)
+    o = oracle_mod.Oracle(nx, cbrt_mode=oracle_mod.CBRT_EXACT)
+    assert o.run(cycles) == 0
+    g = amd.Group(nx)
+    g.set_max_cycles(cycles)
+    assert g.run() == 0
+    assert g.sync()[1:] == (o.cycle, o.time, o.deltatime)
+    _compare_state(g, o, f"s={nx} cycles={cycles}")
+    g.close()
+    o.close()
+
+
 def test_s16_full_run_bitwise(amd, oracle_mod):
     o = oracle_mod.Oracle(16, cbrt_mode=oracle_mod.CBRT_EXACT)
     o.run()
