@@ -1274,6 +1274,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     bool q_err = false;   // raised after the element's work: no atomic on the load path
     // also read by the final phase: loaded once, with the element's other fields
     double vdov = 0.0, mass = 0.0, volo = 0.0, arealg = 0.0;
+    bool clamped = false;   // vnew was moved into [eosvmin, eosvmax]
     if (live) {
         // every load of the element first (none behind a branch), then the
         // arithmetic
@@ -1347,6 +1348,7 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         // --- EOS inputs (vnew clamped to [eosvmin, eosvmax], lulesh.cc:2642-2650) ---
         double vnewc = vnew_raw;
         vnewc = fmin(raise_to(vnewc, Phys::eosvmin), Phys::eosvmax);
+        clamped = vnewc != vnew_raw;
         s_in[0][tid] = vnewc;
         s_in[1][tid] = e_old;
         s_in[2][tid] = delv;
@@ -1383,6 +1385,8 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
     if (live) {
         const double p_new = s_out[0][tid], e_new = s_out[1][tid];
         const double vnewc = s_in[0][tid];
+        // the reference clamps the vnew array itself (lulesh.cc:2642-2650)
+        if (clamped) gst(D.vnew, el, vnewc);
         gst(D.p, el, p_new);
         gst(D.e, el, e_new);
         gst(D.q, el, s_out[2][tid]);

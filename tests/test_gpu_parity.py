@@ -120,6 +120,26 @@ def test_force_tile_shapes_bitwise(amd, oracle_mod, nx, cycles):
     o.close()
 
 
+def test_eos_volume_clamps_bitwise(amd, oracle_mod):
+    # reference volumes scaled so that vnew falls below eosvmin in one
+    # element and above eosvmax in another: the EOS clamps (written back
+    # into vnew, as the reference does), the eosvmax branches and the
+    # compression at both ends of its range, bit for bit
+    print("case: EOS volume clamps, s=6, 5 cycles", flush=True)
+    o = oracle_mod.Oracle(6, cbrt_mode=oracle_mod.CBRT_EXACT)
+    g = amd.Group(6)
+    volo = g.field("volo")
+    volo[17] *= 1e11
+    volo[50] *= 1e-11
+    g.domains[0].set_field("volo", volo)
+    o.set_field("volo", volo)
+    g.set_max_cycles(5)
+    assert g.run() == 0 and o.run(5) == 0
+    assert g.field("vnew")[17] == 1e-9 and g.field("vnew")[50] == 1e9
+    assert g.sync()[1:] == (o.cycle, o.time, o.deltatime)
+    _compare_state(g, o, "EOS clamps")
+
+
 def test_s16_full_run_bitwise(amd, oracle_mod):
     o = oracle_mod.Oracle(16, cbrt_mode=oracle_mod.CBRT_EXACT)
     o.run()
