@@ -9,13 +9,19 @@
 //                         stress + FB hourglass corner forces, the ordered
 //                         node gather through LDS, and (subdomains without
 //                         halo exchange) the node update
-//   k_node_update         per node (multi-subdomain, after the SBN exchange)
+//   k_node_update_direct  per node of a decomposed group: the neighbours'
+//                         partial forces summed in receive-slot order
+//                         (CommSBN) and each shared node's final writer
+//                         pulled (CommSyncPosVel), no exchange launched
+//   k_node_update         per node (after an explicit SBN exchange)
 //   k_kinematics          per element: volume, arealg, vdov, MonoQ gradients
+//                         (and the neighbours' MonoQ ghosts, CommMonoQ)
 //   k_qeos                per element tile: MonoQ region, qstop, EOS x rep
 //                         dealt over the workgroup, Courant/hydro dt minima
-// plus the two-pass alternative of the force step (k_elem_force writes
-// corner forces, k_node_gather<fuse> gathers them per node) kept for
-// cross-checking.  Every kernel walks a tile table (subdomain, first item)
+// plus the two-pass form of the force step (k_elem_force writes corner
+// forces, k_node_gather<fuse> gathers them per node): the unstructured-mesh
+// path, and a cross-check of the march on the block.  Every kernel walks a
+// tile table (subdomain, first item)
 // so one launch covers every subdomain of every group on the GPU.  Integer
 // atomics are used only for the dt minima (MIN is exact) and the error
 // code; all force sums run in the reference's element order, so results
