@@ -64,7 +64,8 @@ static void engine_drop_graph(Engine *e)
 // workgroups per CU whatever their wave count, and a workgroup's plane
 // step takes about as long with three waves as with four, so the cost is
 // the number of tiles: pick the fewest tiles per node plane, then the
-// fewest busy waves, then the fewest element slots.
+// fewest busy waves, then the fewest element slots, then the wider x
+// extent (longer contiguous rows in each plane's global loads).
 static void tile_shape(int nx, int &ex, int &ey)
 {
     const int s1 = nx + 1;
@@ -77,7 +78,8 @@ static void tile_shape(int nx, int &ex, int &ey)
             const long waves = tiles * ((a * b + 63) / 64);   // busy waves per plane
             const long slots = tiles * a * b;
             if (best_t < 0 || tiles < best_t || (tiles == best_t && waves < best_w) ||
-                (tiles == best_t && waves == best_w && slots < best_s)) {
+                (tiles == best_t && waves == best_w && slots < best_s) ||
+                (tiles == best_t && waves == best_w && slots == best_s && a > ex)) {
                 best_t = tiles;
                 best_w = waves;
                 best_s = slots;

@@ -105,8 +105,8 @@ def test_single_domain_bitwise(amd, oracle_mod, nx, cycles):
 
 @pytest.mark.parametrize("nx,cycles", [(20, 30), (45, 12), (80, 6)])
 def test_force_tile_shapes_bitwise(amd, oracle_mod, nx, cycles):
-    # meshes whose force-march column tiles are not the 14 x 18 of -s 100
-    # (8 x 22, 9 x 24, 9 x 28 element slots) and whose z-chunks end in a
+    # meshes whose force-march column tiles are not the 18 x 14 of -s 100
+    # (22 x 8, 24 x 9, 28 x 9 element slots) and whose z-chunks end in a
     # short chunk: the same state as the oracle, bit for bit
     print(f"case: tile shapes s={nx} {cycles} cycles", flush=True)
     o = oracle_mod.Oracle(nx, cbrt_mode=oracle_mod.CBRT_EXACT)
