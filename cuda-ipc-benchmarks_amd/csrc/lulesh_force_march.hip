@@ -1,0 +1,302 @@
+// lulesh_force_march.hip -- the fused force kernel of the block path.
+//
+// Built on its own: the FP64-issue-bound march is scheduled for
+// instruction-level parallelism (-amdgpu-sched-strategy=max-ilp in the
+// Makefile), which the register-bound element kernels of lulesh_kernels.hip
+// would pay for with occupancy.
+#include <hip/hip_runtime.h>
+
+#include "lulesh_dev.h"
+#include "lulesh_kernel_util.h"
+#include "lulesh_kernels.h"
+#include "lulesh_math.h"
+
+namespace lulesh {
+
+// ---------------------------------------------------------------------------
+// Fused force + ordered node gather, z-marching (the production path).
+//
+// A workgroup owns a column of (ex-1) x (ey-1) nodes and a chunk of node
+// planes [k0, k1).  Its ex x ey threads (<= 256; 16 x 16 for large meshes,
+// shapes fitted to the mesh for small ones) are the elements of the
+// column's halo-extended element tile; the block walks element planes c = k0-1 .. k1-1.
+// At each plane every thread computes its element's 8 stress corner forces
+// and 8 hourglass corner forces (the same arithmetic as k_elem_force) into
+// LDS, and each node thread adds its four neighbouring elements' corners:
+// corners 2,3,1,0 of plane c complete node plane c, corners 6,7,5,4 start
+// node plane c+1.  That is exactly the reference's per-node order (elements
+// ascending: plane c-1 elements (i-1,j-1),(i,j-1),(i-1,j),(i,j), then the
+// same four of plane c), summed from 0.0, stress and hourglass separately,
+// f = stress + hourglass -- so the result is the reference's to the bit,
+// without the 48-doubles-per-element corner array ever touching HBM.
+// Elements outside the mesh contribute +0.0, which leaves a sum that
+// started from +0.0 unchanged.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, const ColTile *tiles,
+                                                        int ntiles, int kchunk, int advance, int store_f)
+{
+    __shared__ double lds[24][kMThreads];       // corner forces of one phase
+    __shared__ double sn[2][6][kPatch];          // node planes (x y z xd yd zd), by plane parity
+    const ColTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+    // the tile's subdomain is uniform over the workgroup: one scalar copy of
+    // its descriptor, which no store in the kernel can be taken to alias
+    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
+    GroupCtl *G = D.grp;
+    // the stop flag is tested once the first plane's loads are in flight
+    // (reads only: a stopped group's arrays are still allocated)
+    const bool stopped = group_stopped(*G);
+    const int nx = D.nx, s1 = nx + 1, s2 = s1 * s1;
+    const int tid = threadIdx.x;
+    const int ex = t.ex, ey = t.ey, px = ex + 1;
+    const int ta = tid % ex, tb = tid / ex;
+    const bool slot_ok = tb < ey;
+    const int a = t.i0 - 1 + ta, b = t.j0 - 1 + tb;
+    const bool col_ok = slot_ok && a >= 0 && a < nx && b >= 0 && b < nx;
+    const int ni = t.i0 + ta, nj = t.j0 + tb;
+    const bool node_ok = ta < ex - 1 && tb < ey - 1 && ni <= nx && nj <= nx;
+    const int k0 = t.k0;
+    const int k1 = min(t.k0 + kchunk, nx + 1);
+    // lower / upper corner slots seen from the node thread:
+    // own element, +x, +y, +x+y neighbours
+    const int t1 = tid + 1, t2 = tid + ex, t3 = tid + ex + 1;
+    const NodeState S = ns_old(D);
+    const double dt = D.ctl->deltatime;
+
+    // Node patch: slot (pi, pj) holds node (i0-1+pi, j0-1+pj) of a node
+    // plane.  The element's corners 0..3 are slots e0, e0+1, e0+px+1,
+    // e0+px of its lower plane, 4..7 the same slots of the upper one.  A
+    // thread loads the slot of the node it owns, (ta+1, tb+1) = corner 2
+    // of its element; the first ex+ey+1 threads also load one slot of the
+    // patch's row 0 / column 0.
+    const int e0 = tb * px + ta;
+    const int cs[4] = {e0, e0 + 1, e0 + px + 1, e0 + px};
+    const int own_sl = e0 + px + 1;
+    const int own_i = ni, own_j = nj;
+    const bool own_in = slot_ok && own_i <= nx && own_j <= nx;
+    int xt_sl = -1, xt_i = 0, xt_j = 0;
+    if (tid < ex + ey + 1) {
+        const int pi = tid < px ? tid : 0, pj = tid < px ? 0 : tid - px + 1;
+        xt_sl = pj * px + pi;
+        xt_i = t.i0 - 1 + pi;
+        xt_j = t.j0 - 1 + pj;
+        if (xt_i < 0 || xt_i > nx || xt_j < 0 || xt_j > nx) xt_sl = -1;
+    }
+    double pf_own[6], pf_xt[6];
+    // The loads of a plane are issued on every path -- at a valid node
+    // (node 0) where the slot or the plane does not exist, the values then
+    // unused -- so the number of loads in flight is the same on all paths
+    // and the waits for the older ones can be counted exactly instead of
+    // draining the whole queue.
+    auto fetch = [&](int kn, double own[6], double xt[6]) {   // issue the loads of node plane kn
+        const int kc = kn < 0 ? 0 : (kn > nx ? nx : kn);
+        const unsigned nd = opaque_idx(own_in ? kc * s2 + own_j * s1 + own_i : 0);
+        own[0] = gld(S.x, nd); own[1] = gld(S.y, nd); own[2] = gld(S.z, nd);
+        own[3] = gld(S.xd, nd); own[4] = gld(S.yd, nd); own[5] = gld(S.zd, nd);
+        const unsigned nt = opaque_idx(xt_sl >= 0 ? kc * s2 + xt_j * s1 + xt_i : 0);
+        xt[0] = gld(S.x, nt); xt[1] = gld(S.y, nt); xt[2] = gld(S.z, nt);
+        xt[3] = gld(S.xd, nt); xt[4] = gld(S.yd, nt); xt[5] = gld(S.zd, nt);
+    };
+    auto stage = [&](int kn, const double own[6], const double xt[6]) {   // write plane kn to LDS
+        if (kn < 0 || kn > nx) return;
+        double (*P)[kPatch] = sn[kn & 1];
+        if (own_in)
+#pragma unroll
+            for (int f = 0; f < 6; ++f) P[f][own_sl] = own[f];
+        if (xt_sl >= 0)
+#pragma unroll
+            for (int f = 0; f < 6; ++f) P[f][xt_sl] = xt[f];
+    };
+    // element fields of plane c (p, q, v, volo, hgcoef), fetched a plane ahead
+    double ef[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ef_next[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    auto fetch_elem = [&](int c, double out[5]) {
+        const unsigned el = opaque_idx((col_ok && c >= 0 && c < nx) ? (c * nx + b) * nx + a : 0);
+        out[0] = gld(D.p, el); out[1] = gld(D.q, el); out[2] = gld(D.v, el); out[3] = gld(D.volo, el);
+        out[4] = gld(D.hgcoef, el);
+    };
+
+    {
+        // the first two planes' loads all in flight before either is staged
+        double own1[6], xt1[6];
+        fetch(k0 - 1, pf_own, pf_xt);
+        fetch(k0, own1, xt1);
+        fetch_elem(k0 - 1, ef);
+        if (stopped) return;
+        stage(k0 - 1, pf_own, pf_xt);
+        stage(k0, own1, xt1);
+    }
+    __syncthreads();
+
+    // a non-positive volume (lulesh.cc:1263-1265, 1340-1349) is reported
+    // once, after the planes: no atomic sits on the loop's path
+    bool vol_err = false;
+    double ps[3] = {0.0, 0.0, 0.0}, ph[3] = {0.0, 0.0, 0.0};   // node plane c+1 partials
+    double fs[3] = {0.0, 0.0, 0.0};                              // node plane c stress sums
+    for (int c = k0 - 1; c < k1; ++c) {
+        const bool el_ok = col_ok && c >= 0 && c < nx;
+        const double (*Lo)[kPatch] = sn[c & 1];
+        const double (*Up)[kPatch] = sn[(c + 1) & 1];
+        // this plane's nodal masses (first: the update waits for them
+        // alone), then the next planes' loads, in flight meanwhile
+        const double nmass = gld(D.nodalMass, opaque_idx((node_ok && c >= 0) ? c * s2 + nj * s1 + ni : 0));
+        fetch(c + 2, pf_own, pf_xt);
+        fetch_elem(c + 1, ef_next);
+
+        // ---- stress corners ----
+        if (el_ok) {
+            double x[8], y[8], z[8];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                x[q] = Lo[0][cs[q]]; y[q] = Lo[1][cs[q]]; z[q] = Lo[2][cs[q]];
+                x[q + 4] = Up[0][cs[q]]; y[q + 4] = Up[1][cs[q]]; z[q + 4] = Up[2][cs[q]];
+            }
+            // sigma / 16 against the normals x 16: the same products.  The
+            // division is exact: p and q are 0 or at least their cut-offs
+            const double sig16 = (-ef[0] - ef[1]) * 0.0625;
+            vol_err |= jac_volume(x, y, z) <= 0.0;
+            double bx[8], by[8], bz[8];
+            node_normals16(x, y, z, bx, by, bz);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                lds[0 * 8 + q][tid] = -(sig16 * bx[q]);
+                lds[1 * 8 + q][tid] = -(sig16 * by[q]);
+                lds[2 * 8 + q][tid] = -(sig16 * bz[q]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 24; ++q) lds[q][tid] = 0.0;
+        }
+        __syncthreads();
+        // the gathers between barriers are short and latency-bound: they
+        // issue ahead of the other workgroup's element arithmetic on the
+        // same SIMD, so this workgroup reaches its next barrier sooner
+        __builtin_amdgcn_s_setprio(2);
+        if (node_ok) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (c >= k0) {
+                    double v = ps[d];
+                    v += lds[d * 8 + 2][tid];
+                    v += lds[d * 8 + 3][t1];
+                    v += lds[d * 8 + 1][t2];
+                    v += lds[d * 8 + 0][t3];
+                    fs[d] = v;
+                }
+                if (c + 1 < k1) {
+                    double v = 0.0;
+                    v += lds[d * 8 + 6][tid];
+                    v += lds[d * 8 + 7][t1];
+                    v += lds[d * 8 + 5][t2];
+                    v += lds[d * 8 + 4][t3];
+                    ps[d] = v;
+                }
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
+        // ---- hourglass corners ----
+        if (el_ok) {
+            const double vv = ef[2];
+            vol_err |= vv <= 0.0;
+            HgModes H;
+            {
+                double x[8], y[8], z[8];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    x[q] = Lo[0][cs[q]]; y[q] = Lo[1][cs[q]]; z[q] = Lo[2][cs[q]];
+                    x[q + 4] = Up[0][cs[q]]; y[q + 4] = Up[1][cs[q]]; z[q + 4] = Up[2][cs[q]];
+                }
+                hg_modes(x, y, z, ef[3] * vv, ef[4], H);
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double vl[8], h[8];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    vl[q] = Lo[3 + d][cs[q]];
+                    vl[q + 4] = Up[3 + d][cs[q]];
+                }
+                hg_force(H, vl, h);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) lds[d * 8 + q][tid] = h[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 24; ++q) lds[q][tid] = 0.0;
+        }
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(2);
+        if (node_ok) {
+            double f[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (c >= k0) {
+                    double v = ph[d];
+                    v += lds[d * 8 + 2][tid];
+                    v += lds[d * 8 + 3][t1];
+                    v += lds[d * 8 + 1][t2];
+                    v += lds[d * 8 + 0][t3];
+                    f[d] = fs[d] + v;
+                }
+                if (c + 1 < k1) {
+                    double v = 0.0;
+                    v += lds[d * 8 + 6][tid];
+                    v += lds[d * 8 + 7][t1];
+                    v += lds[d * 8 + 5][t2];
+                    v += lds[d * 8 + 4][t3];
+                    ph[d] = v;
+                }
+            }
+            if (c >= k0) {
+                const unsigned nd = opaque_idx(c * s2 + nj * s1 + ni);
+                if (store_f) {
+                    gst(D.fpx, nd, f[0]);
+                    gst(D.fpy, nd, f[1]);
+                    gst(D.fpz, nd, f[2]);
+                }
+                if (advance) {
+                    // the node is complete: advance it from its old state
+                    // (its own slot of the lower plane) into the new buffer
+                    const double one_over_nMass = rcp_pos(nmass);   // masses are positive
+                    double ax = f[0] * one_over_nMass, ay = f[1] * one_over_nMass,
+                           az = f[2] * one_over_nMass;
+                    if (D.col == 0 && ni == 0) ax = 0.0;
+                    if (D.row == 0 && nj == 0) ay = 0.0;
+                    if (D.plane == 0 && c == 0) az = 0.0;
+                    double vx = Lo[3][own_sl] + ax * dt;
+                    if (fabs(vx) < Phys::u_cut) vx = 0.0;
+                    double vy = Lo[4][own_sl] + ay * dt;
+                    if (fabs(vy) < Phys::u_cut) vy = 0.0;
+                    double vz = Lo[5][own_sl] + az * dt;
+                    if (fabs(vz) < Phys::u_cut) vz = 0.0;
+                    const NodeState N = ns_new(D);
+                    gst(N.xd, nd, vx);
+                    gst(N.yd, nd, vy);
+                    gst(N.zd, nd, vz);
+                    gst(N.x, nd, Lo[0][own_sl] + vx * dt);
+                    gst(N.y, nd, Lo[1][own_sl] + vy * dt);
+                    gst(N.z, nd, Lo[2][own_sl] + vz * dt);
+                }
+            }
+        }
+        // the lower plane is done with (this thread's own slot was read
+        // just above): it takes node plane c+2
+        if (c + 2 <= k1) stage(c + 2, pf_own, pf_xt);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) ef[q] = ef_next[q];
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
+    }
+    if (vol_err) raise_status(G, -1);
+}
+
+
+hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, bool advance,
+                              bool store_f, hipStream_t s)
+{
+    if (nt == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_force_march, dim3(nt), dim3(kMThreads), 0, s, d, t, nt, kchunk, advance ? 1 : 0,
+                       store_f ? 1 : 0);
+    return hipGetLastError();
+}
+
+}  // namespace lulesh
