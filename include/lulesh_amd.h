@@ -88,7 +88,10 @@ const char *lulesh_version(void);
  * balance and cost as the reference's -s/-r/-b/-c options
  * (LULESH/omp_4.0/lulesh-init.cc:16-184, lulesh-util.cc:60-173).  Replaces
  * Domain::Domain + InitMeshDecomp + the initial nodalMass exchange
- * (lulesh.cc:3140-3161). */
+ * (lulesh.cc:3140-3161).  LULESH_EINVAL when nx^3 * 8 passes 2^31 - 1
+ * (32-bit byte offsets into an element array: nx <= 645) or nx^3 times the
+ * largest EOS repetition count does (10 (1 + cost) with nr >= 5, else
+ * 1 + cost: nx <= 475 at nr 11, cost 1). */
 int lulesh_group_create(int nx, int tp, int nr, int balance, int cost,
                         lulesh_group **out);
 int lulesh_group_destroy(lulesh_group *g);

@@ -177,3 +177,25 @@ def test_full_run_energy_is_finite(oracle_mod):
     e = o.field("e")
     assert np.isfinite(e).all() and e.max() > 0
     assert o.verify()["origin_energy"] == e[0]
+
+
+def test_corner_mesh_reproduces_block_corner(oracle_mod):
+    # For its first cycles the blast stays near the origin: the m^3 corner
+    # of the -s nx block, cut out as a mesh of its own (same coordinates and
+    # deposit, free faces where the block continues), evolves bit for bit
+    # as the corner of the full block, and its outer faces stay untouched.
+    # A run too small for the oracle to repeat in full is checked this way.
+    nx, m, cycles = 40, 12, 20
+    ref = oracle_mod.Oracle(nx, cbrt_mode=oracle_mod.CBRT_EXACT)
+    o = oracle_mod.Oracle.from_mesh(U.corner_mesh(nx, m), nr=1, balance=1, cost=0,
+                                    cbrt_mode=oracle_mod.CBRT_EXACT,
+                                    elem_bc=oracle_mod.Oracle(m).elem_bc(), e_deposit=U.einit(nx))
+    assert o.deltatime == ref.deltatime
+    assert ref.run(cycles) == 0 and o.run(cycles) == 0
+    assert (o.cycle, o.time, o.deltatime) == (ref.cycle, ref.time, ref.deltatime)
+    for f in U.CORNER_ELEM:
+        assert_bitwise(o.field(f), U.corner_of(ref.field(f), nx, m), f"corner {f}")
+    for f in U.CORNER_NODE:
+        assert_bitwise(o.field(f), U.corner_of(ref.field(f), nx + 1, m + 1), f"corner {f}")
+    e = o.field("e").reshape(m, m, m)
+    assert e[0, 0, 0] > 0 and not e[-1].any() and not e[:, -1].any() and not e[:, :, -1].any()

@@ -58,6 +58,31 @@ def structured_mesh(nx):
                 symm=symm, free=free)
 
 
+def corner_mesh(nx, m):
+    """The m^3 elements of the -s nx block nearest the origin, as a mesh of
+    their own: the block's coordinates (spacing 1.125/nx), its symmetry
+    planes, and free faces where the block continues.  Until the blast
+    reaches those faces, a run on it leaves the corner in the state the
+    full block's run leaves it."""
+    c = structured_mesh(m)
+    en = m + 1
+    i = np.arange(en)
+    k3, j3, i3 = np.meshgrid(i, i, i, indexing="ij")
+    for name, idx in (("x", i3), ("y", j3), ("z", k3)):
+        c[name] = (1.125 * idx / float(nx)).ravel()
+    return c
+
+
+CORNER_ELEM = ["e", "p", "q", "v", "ss", "volo", "elemMass", "delv", "vdov", "arealg", "vnew",
+               "delx_xi", "delx_eta", "delx_zeta", "delv_xi", "delv_eta", "delv_zeta"]
+CORNER_NODE = ["x", "y", "z", "xd", "yd", "zd"]
+
+
+def corner_of(field, n, m):
+    """The [k][j][i] < m corner of a field laid out n per edge."""
+    return field[:n ** 3].reshape(n, n, n)[:m, :m, :m].ravel()
+
+
 def renumber_nodes(m, perm):
     """Node n of m becomes node perm[n]."""
     perm = np.asarray(perm)
