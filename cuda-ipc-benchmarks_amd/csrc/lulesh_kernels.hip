@@ -45,6 +45,12 @@ namespace lulesh {
 // doms/dom0: the engine's subdomain descriptors and the index of each
 // group's first one; the counted cycle n reads node-state buffer (n-1)&1
 // and writes buffer n&1
+__device__ __forceinline__ void ns_unpack(const NodeState *s, double *p[6])
+{
+    p[0] = *gp(&s->x); p[1] = *gp(&s->y); p[2] = *gp(&s->z);
+    p[3] = *gp(&s->xd); p[4] = *gp(&s->yd); p[5] = *gp(&s->zd);
+}
+
 __global__ __launch_bounds__(256) void k_time_increment(GroupCtl *const *groups, int ngroups, DomDev *doms,
                                                          const int *dom0)
 {
@@ -52,23 +58,52 @@ __global__ __launch_bounds__(256) void k_time_increment(GroupCtl *const *groups,
     const int gi = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int r = threadIdx.x & 63;
     if (gi >= ngroups) return;
-    GroupCtl &G = *groups[gi];
-    // the DomCtl array follows the GroupCtl in its allocation: addressed
-    // directly, its loads need not wait for G.dctl
-    DomCtl *D = reinterpret_cast<DomCtl *>(&G + 1);
-    if (group_stopped(G)) return;
-    if (!(D[0].time < G.stoptime) || (G.max_cycles >= 0 && D[0].cycle >= G.max_cycles)) {
-        if (r == 0) G.done = 1;
+    // Every load is issued ahead of the branches that use it, in as few
+    // dependent rounds as the addresses allow: the group pointer and its
+    // first subdomain's index; the group's loop state, subdomain 0's time
+    // state and node-state buffers; then (groups of several subdomains
+    // only) the other subdomains'.  The DomCtl array follows the GroupCtl
+    // in its allocation.
+    GroupCtl *const Gp = *gp(&groups[gi]);
+    const int d0 = *gp(&dom0[gi]);
+    DomCtl *const D = reinterpret_cast<DomCtl *>(Gp + 1);
+    const double stoptime = *gp(&Gp->stoptime);
+    const int max_cycles = *gp(&Gp->max_cycles), stop = *gp(&Gp->done) | *gp(&Gp->status);
+    const int ndom = *gp(&Gp->ndom);
+    double time = *gp(&D[0].time), olddt = *gp(&D[0].deltatime);
+    unsigned long long dcb = *gp(&D[0].dtcourant_bits), dhb = *gp(&D[0].dthydro_bits);
+    int cyc = *gp(&D[0].cycle);
+    double *b0[6], *b1[6];
+    ns_unpack(&doms[d0].ns[0], b0);
+    ns_unpack(&doms[d0].ns[1], b1);
+    // the values are needed here: the compiler may not sink their loads
+    // behind the branches below (one wait instead of one per branch)
+    __asm__ volatile("" ::"v"(stoptime), "v"(max_cycles), "v"(stop), "v"(ndom), "v"(time), "v"(olddt),
+                     "v"(dcb), "v"(dhb), "v"(cyc), "v"(b0[0]), "v"(b0[1]), "v"(b0[2]), "v"(b0[3]),
+                     "v"(b0[4]), "v"(b0[5]), "v"(b1[0]), "v"(b1[1]), "v"(b1[2]), "v"(b1[3]), "v"(b1[4]),
+                     "v"(b1[5]));
+    if (stop != 0) return;
+    if (!(time < stoptime) || (max_cycles >= 0 && cyc >= max_cycles)) {
+        if (r == 0) Gp->done = 1;
         return;
     }
-    const bool mine = r < G.ndom;
+    const bool mine = r < ndom;
+    if (mine && r > 0) {
+        time = D[r].time;
+        olddt = D[r].deltatime;
+        dcb = D[r].dtcourant_bits;
+        dhb = D[r].dthydro_bits;
+        cyc = D[r].cycle;
+        ns_unpack(&doms[d0 + r].ns[0], b0);
+        ns_unpack(&doms[d0 + r].ns[1], b1);
+    }
     // each subdomain's candidate, then the MIN over the group (the
     // reference's MPI_Allreduce(MIN), lulesh.cc:227-236; MIN is exact, so
     // any reduction order gives the sequential result)
     double gnewdt = 1.0e+20;
     if (mine) {
-        const double dc = __longlong_as_double((long long)D[r].dtcourant_bits);
-        const double dh = __longlong_as_double((long long)D[r].dthydro_bits);
+        const double dc = __longlong_as_double((long long)dcb);
+        const double dh = __longlong_as_double((long long)dhb);
         if (dc < gnewdt) gnewdt = dc / 2.0;
         if (dh < gnewdt) gnewdt = dh * 2.0 / 3.0;
     }
@@ -79,10 +114,9 @@ __global__ __launch_bounds__(256) void k_time_increment(GroupCtl *const *groups,
         newdt = o < newdt ? o : newdt;
     }
     if (!mine) return;
-    DomCtl &d = D[r];
-    double targetdt = G.stoptime - d.time;
-    if ((Phys::dtfixed <= 0.0) && (d.cycle != 0)) {
-        const double olddt = d.deltatime;
+    double deltatime = olddt;
+    double targetdt = stoptime - time;
+    if ((Phys::dtfixed <= 0.0) && (cyc != 0)) {
         double dt = newdt;
         const double ratio = dt / olddt;
         if (ratio >= 1.0) {
@@ -90,19 +124,30 @@ __global__ __launch_bounds__(256) void k_time_increment(GroupCtl *const *groups,
             else if (ratio > Phys::deltatimemultub) dt = olddt * Phys::deltatimemultub;
         }
         if (dt > Phys::dtmax) dt = Phys::dtmax;
-        d.deltatime = dt;
+        deltatime = dt;
     }
-    if ((targetdt > d.deltatime) && (targetdt < (4.0 * d.deltatime / 3.0)))
-        targetdt = 2.0 * d.deltatime / 3.0;
-    if (targetdt < d.deltatime) d.deltatime = targetdt;
-    d.time += d.deltatime;
-    ++d.cycle;
+    if ((targetdt > deltatime) && (targetdt < (4.0 * deltatime / 3.0)))
+        targetdt = 2.0 * deltatime / 3.0;
+    if (targetdt < deltatime) deltatime = targetdt;
+    const int cycle = cyc + 1;
+    DomCtl &d = D[r];
+    d.deltatime = deltatime;
+    d.time = time + deltatime;
+    d.cycle = cycle;
     // CalcTimeConstraintsForElems restarts from 1e20 (lulesh.cc:3019-3020)
     d.dtcourant_bits = dbits(1.0e+20);
     d.dthydro_bits = dbits(1.0e+20);
-    DomDev &dd = doms[dom0[gi] + r];
-    dd.cur = dd.ns[d.cycle & 1];
-    dd.old = dd.ns[(d.cycle & 1) ^ 1];
+    // the counted cycle n reads node-state buffer (n-1)&1, writes buffer n&1
+    const bool odd = cycle & 1;
+    double *cur[6], *old[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        cur[f] = odd ? b1[f] : b0[f];
+        old[f] = odd ? b0[f] : b1[f];
+    }
+    DomDev &dd = doms[d0 + r];
+    dd.cur = NodeState{cur[0], cur[1], cur[2], cur[3], cur[4], cur[5]};
+    dd.old = NodeState{old[0], old[1], old[2], old[3], old[4], old[5]};
 }
 
 // node ids of element (i,j,k) in the structured block (BuildMesh,

@@ -87,7 +87,10 @@ def _compare_state(g, o, what):
         assert_bitwise(g.field(f), o.field(f), f"{what} {f}")
 
 
-@pytest.mark.parametrize("nx,cycles", [(4, 1), (4, -1), (8, 5), (8, -1), (16, 40)])
+# sizes 1, 2, 3, 5 and 7: a single force tile wider than the mesh,
+# single-plane z-chunks, EOS tiles of a few elements
+@pytest.mark.parametrize("nx,cycles", [(1, -1), (2, -1), (3, -1), (4, 1), (4, -1), (5, -1),
+                                       (7, -1), (8, 5), (8, -1), (16, 40)])
 def test_single_domain_bitwise(amd, oracle_mod, nx, cycles):
     o = oracle_mod.Oracle(nx, cbrt_mode=oracle_mod.CBRT_EXACT)
     assert o.run(cycles) == 0
