@@ -42,6 +42,27 @@ def test_batch_full_runs_match_plain_runs(amd, oracle_mod):
     assert_bitwise(b.group(0).field("e"), o.field("e"), "problem 0 vs oracle e")
 
 
+def test_batch_tiny_problems_vs_oracle(amd, oracle_mod):
+    # problems of 1, 2 and 3 elements per edge batched with a larger one
+    # (tiles far smaller than a workgroup, 1 to 23 cycles to stoptime while
+    # the larger one keeps stepping): each ends in the state the oracle
+    # reaches alone, bit for bit
+    print("case: batch of tiny problems to stoptime", flush=True)
+    mix = [(1, 11, 1, 1), (2, 11, 1, 1), (3, 1, 1, 0), (20, 11, 1, 1)]
+    nx, nr, bal, cost = (list(v) for v in zip(*mix))
+    b = amd.Batch(nx, nr, bal, cost)
+    assert b.run() == 0
+    cycles, status = b.results()
+    assert (status == 0).all()
+    for i, s in enumerate(nx):
+        o = oracle_mod.Oracle(s, cbrt_mode=oracle_mod.CBRT_EXACT)
+        assert o.run() == 0
+        assert cycles[i] == o.cycle
+        for f in ["x", "y", "z", "xd", "yd", "zd", "e", "p", "q", "v", "ss"]:
+            assert_bitwise(b.group(i).field(f), o.field(f), f"problem {i} (s={s}) {f}")
+        o.close()
+
+
 @pytest.mark.parametrize("tiles,kchunk", [((64, 128), 4), ((256, 256), 1), ((128, 64), 0)])
 def test_batch_tiles_do_not_change_results(amd, tiles, kchunk):
     print(f"case: batch tiles {tiles} kchunk {kchunk}", flush=True)
