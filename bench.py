@@ -15,7 +15,12 @@ Default run: --steps 200 timed cycles after --warmup 20, then
     of the same problem, or the C restatement in oracle/ when that binary is
     absent.
 
-Other modes (extra JSON lines are not emitted by default):
+The headline line also carries the batched and served halves of the metric
+as sub-objects ("batched": the 64-problem ensemble to stoptime vs sequential
+plain runs of the same mix; "served": a Poisson stream of --serve-requests
+requests at --serve-rate per second); --no-extra skips them.
+
+Other modes (one JSON line each):
   --mode batched   64-problem ensemble vs sequential plain runs
   --mode serve     Poisson-arrival serving loop, p50/p99 latency
   --mode multidomain  2x2x2 subdomains of -s 45 on one GPU
@@ -120,33 +125,77 @@ def profiled_value(key, k, nx, fused_update=False):
     return None
 
 
-def cpu_baseline(nx, target_s=15.0):
-    """The reference's own CPU code on the host cores, bounded sample."""
+def host_cpu_info():
+    """CPU model, physical cores and logical CPUs of this host, and the
+    CPUs this process may run on."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    return {"cpu_model": model, "host_cores": len(cores) or logical, "host_logical_cpus": logical,
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+# host threads a benchmark process may use on the GPU machines: one GPU's
+# job is allotted 16 CPUs whatever the host has
+CPU_SHARE = 16
+
+
+def cpu_baseline(nx, target_s=8.0, samples=3):
+    """The reference's own CPU code on the host cores: median of `samples`
+    bounded runs of the same problem (-s nx -i K, K sized to ~target_s)."""
     ref = os.path.join(REPO, "oracle", "_ref", "lulesh_ref")
-    cores = len(os.sched_getaffinity(0))
-    threads = max(2, min(cores, 16))
+    info = host_cpu_info()
+    threads = max(2, min(info["affinity_cpus"], CPU_SHARE))
+    why = (f"{threads} threads: the CPUs this job may use (min of its affinity set, "
+           f"{info['affinity_cpus']}, and the {CPU_SHARE}-CPU share of one GPU's job)")
     if os.path.exists(ref):
-        # probe a few cycles, then size the sample to ~target_s
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                    OMP_PLACES="cores")
-        its = 3
-        for _ in range(2):
+
+        def run(its):
             t0 = time.time()
             out = subprocess.run([ref, "-s", str(nx), "-i", str(its)], env=env,
                                  capture_output=True, text=True, timeout=600).stdout
             el = time.time() - t0
-            if el > target_s / 3 or its >= 200:
-                break
-            its = max(its + 1, min(200, int(its * target_s / max(el, 1e-3))))
-        grind = None
-        for line in out.splitlines():
-            if line.startswith("Grind time"):
-                grind = float(line.split("=")[1].split()[0])
-        if grind:
-            return {"value": 1e6 / grind, "unit": "zone-cycles/s", "cores": threads,
-                    "kind": "reference",
-                    "sample": f"omp_4.0 lulesh -s {nx} -i {its} (unmodified reference sources, "
-                              f"g++ -O3 -fopenmp, OMP_NUM_THREADS={threads})"}
+            for line in out.splitlines():
+                if line.startswith("Grind time"):
+                    return float(line.split("=")[1].split()[0]), el
+            return None, el
+
+        # probe a few cycles, then size each sample to ~target_s
+        its = 3
+        grind, el = run(its)
+        its = max(its + 1, min(400, int(its * target_s / max(el, 1e-3))))
+        rates = []
+        for _ in range(samples):
+            grind, _ = run(its)
+            if grind:
+                rates.append(1e6 / grind)
+        if rates:
+            return dict({"value": float(np.median(rates)), "unit": "zone-cycles/s",
+                         "cores": threads, "kind": "reference",
+                         "sample": f"omp_4.0 lulesh -s {nx} -i {its}, median of {len(rates)} runs "
+                                   f"(unmodified reference sources, g++ -O3 -fopenmp, "
+                                   f"OMP_NUM_THREADS={threads}, OMP_PROC_BIND=close)",
+                         "samples": rates, "threads_note": why}, **info)
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle
     o = pyoracle.Oracle(nx)
@@ -156,19 +205,47 @@ def cpu_baseline(nx, target_s=15.0):
         o.run(1)
         its += 1
     el = time.time() - t0
-    return {"value": nx ** 3 * its / el, "unit": "zone-cycles/s", "cores": 1, "kind": "port",
-            "sample": f"oracle C restatement -s {nx}, {its} cycles, 1 thread"}
+    return dict({"value": nx ** 3 * its / el, "unit": "zone-cycles/s", "cores": 1, "kind": "port",
+                 "sample": f"oracle C restatement -s {nx}, {its} cycles, 1 thread"}, **info)
+
+
+# Cycles of a Sedov -s 100 run before its stop time (3517); the spin-up
+# below never lets warm-up + spin-up + timed cycles reach it.
+S100_CYCLES = 3517
+
+
+def clock_spinup(g, chunk, cycle, steps, busy_s=0.15, stop_cycles=S100_CYCLES):
+    """Keeps the GPU busy with real cycles of the run for ~busy_s before the
+    timed region.  An MI355X coming out of idle raises its clocks over the
+    first ~0.1 s of load: the force kernel takes ~108-111 us per launch over
+    the first 20 cycles of a fresh process and ~104 us from about cycle 35
+    on (rocprofv3 kernel trace), so a timed region that starts a few cycles
+    after start-up measures the clock ramp, not the kernels.  Returns the
+    cycles run."""
+    n = 0
+    room = max(0, stop_cycles - cycle - steps - chunk - 10)
+    t0 = time.perf_counter()
+    while n + chunk <= room and time.perf_counter() - t0 < busy_s:
+        g.enqueue(chunk)
+        g.sync()
+        n += chunk
+    return n
 
 
 def bench_plain(args):
     nx = args.size
     E, N = nx ** 3, (nx + 1) ** 3
     g = lulesh_amd.Group(nx, 1, 11, 1, 1)
-    # warmup (also builds and caches the captured graph)
     g.enqueue(args.warmup)
     g.sync()
-    chunk = args.chunk
-    g.enqueue(chunk)      # capture the timed graph shape outside the timed region
+    # the timed loop enqueues graphs of `chunk` cycles and one of the
+    # remainder: both shapes are captured (and run) before the timed region
+    chunk = max(1, min(args.chunk, args.steps))
+    g.enqueue(chunk)
+    if args.steps % chunk:
+        g.enqueue(args.steps % chunk)
+    rc, c0, _, _ = g.sync()
+    spin = clock_spinup(g, chunk, c0, args.steps)
     rc, c0, _, _ = g.sync()
     t0 = time.perf_counter()
     done = 0
@@ -212,7 +289,8 @@ def bench_plain(args):
         "dtype": "f64",
         "data": "synthetic: Sedov blast initial condition (deterministic set-up of the reference)",
         "config": {"workload": f"LULESH Sedov -s {nx} -r 11 -b 1 -c 1, single domain, plain path",
-                   "elements": E, "nodes": N, "graph_cycles_per_launch": chunk},
+                   "elements": E, "nodes": N, "graph_cycles_per_launch": chunk,
+                   "clock_spinup_cycles": spin},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(dom, nx, fu), "algorithmic_bytes": b},
@@ -229,6 +307,7 @@ def bench_plain(args):
             "valu_issue_busy": profiled_value("valu_busy", dom, nx, fu)}
     g.close()
     if not args.no_cpu:
+        progress("reference CPU baseline (3 bounded samples)")
         line["cpu_baseline"] = cpu_baseline(nx)
     return line
 
@@ -365,6 +444,8 @@ def bench_multidomain(args):
     g = lulesh_amd.Group(45, 2, 11, 1, 1)
     g.enqueue(args.warmup)
     g.enqueue(args.chunk)
+    if args.steps % args.chunk:
+        g.enqueue(args.steps % args.chunk)
     rc, c0, _, _ = g.sync()
     t0 = time.perf_counter()
     done = 0
@@ -397,6 +478,8 @@ def bench_umesh(args):
     m.close()
     g.enqueue(args.warmup)
     g.enqueue(args.chunk)
+    if args.steps % args.chunk:
+        g.enqueue(args.steps % args.chunk)
     rc, c0, _, _ = g.sync()
     t0 = time.perf_counter()
     done = 0
@@ -453,6 +536,12 @@ def main():
     ap.add_argument("--slots", type=int, default=64, help="serve: max resident problems")
     ap.add_argument("--seq-sample", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="plain mode: skip the batched and served sub-objects")
+    ap.add_argument("--serve-requests", type=int, default=256,
+                    help="plain mode: requests of the served sub-object")
+    ap.add_argument("--serve-rate", type=float, default=12.0,
+                    help="plain mode: Poisson arrivals per second of the served sub-object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -476,7 +565,43 @@ def main():
         line = bench_umesh(args)
     else:
         line = bench_plain(args)
+        if not args.no_extra:
+            line.update(headline_extras(args))
     print(json.dumps(line), flush=True)
+
+
+def progress(msg):
+    print(f"# bench: {msg}", file=sys.stderr, flush=True)
+
+
+def headline_extras(args):
+    """The batched and served halves of the metric, as sub-objects of the
+    headline line: the 64-problem ensemble to stoptime against sequential
+    plain runs of the same mix, and a served Poisson stream (fewer requests
+    than the 2,000 of the full stream, stated, so the default run stays
+    within minutes)."""
+    progress("batched ensemble (64 problems to stoptime) and its sequential plain runs")
+    b = bench_batched(args)
+    batched = {"value": b["value"], "unit": b["unit"], "workload": b["config"]["workload"],
+               "problems": 64, "seed": args.seed, "zone_cycles": b["config"]["zone_cycles"],
+               "autotune": b["config"]["autotune"], "sequential_plain_value": b["sequential_plain"]["value"],
+               "speedup_vs_sequential": b["speedup_vs_sequential"],
+               "max_origin_energy_rel_diff_vs_plain": b["max_origin_energy_rel_diff_vs_plain"],
+               "cycles_match_plain": b["cycles_match_plain"]}
+    sa = argparse.Namespace(**vars(args))
+    sa.requests, sa.rate = args.serve_requests, args.serve_rate
+    sa.seq_sample = min(args.seq_sample, args.serve_requests)
+    progress(f"served stream ({sa.requests} requests at {sa.rate}/s)")
+    v = bench_serve(sa)
+    served = {"value": v["value"], "unit": v["unit"], "workload": v["config"]["workload"],
+              "requests": sa.requests, "full_stream_requests": 2000, "rate_per_s": sa.rate,
+              "slots": sa.slots, "seed": args.seed, "offered_load": v["offered_load"],
+              "latency_s": v["latency_s"], "wall_s": v["wall_s"], "failed": v["failed"],
+              "sequential_plain_sample_value": v["sequential_plain_sample"]["value"],
+              "speedup_vs_sequential": v["speedup_vs_sequential"],
+              "fifo_sequential_model_latency_s": v["fifo_sequential_model_latency_s"],
+              "max_origin_energy_rel_diff_vs_plain": v["max_origin_energy_rel_diff_vs_plain"]}
+    return {"batched": batched, "served": served}
 
 
 if __name__ == "__main__":

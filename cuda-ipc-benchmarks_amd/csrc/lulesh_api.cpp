@@ -6,8 +6,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <limits.h>
+#include <stddef.h>
+
 #include <algorithm>
 #include <chrono>
+#include <new>
 #include <map>
 #include <unordered_map>
 #include <memory>
@@ -635,19 +639,23 @@ int group_reset_ctl(lulesh_group *g)
 
 int group_set_param(lulesh_group *g)
 {
-    // stoptime / qstop / max_cycles live in the device GroupCtl; let the
-    // group's queued cycles finish before rewriting it
-    HIPCHK(hipStreamSynchronize(g->stream));
-    GroupCtl G;
-    HIPCHK(hipMemcpy(&G, g->gctl, sizeof(G), hipMemcpyDeviceToHost));
-    G.stoptime = g->stoptime;
-    G.qstop = g->qstop;
-    G.max_cycles = g->max_cycles;
+    // stoptime / qstop / max_cycles live in the device GroupCtl.  The group
+    // may have cycles queued on its own stream, a caller's stream or a
+    // batch's stream: the whole device is drained first, and only the
+    // parameter words and the done flag are written, so a status a kernel
+    // raised is never overwritten.
+    HIPCHK(hipDeviceSynchronize());
+    static_assert(offsetof(GroupCtl, qstop) == offsetof(GroupCtl, stoptime) + sizeof(double),
+                  "stoptime and qstop are adjacent");
+    static_assert(offsetof(GroupCtl, done) == offsetof(GroupCtl, max_cycles) + sizeof(int),
+                  "max_cycles and done are adjacent");
+    const double sq[2] = {g->stoptime, g->qstop};
     // the stop test is re-evaluated at the next TimeIncrement, as the
     // reference's loop condition is on every iteration: a raised cap or
     // stop time resumes a group that had reached the old one
-    G.done = 0;
-    HIPCHK(hipMemcpy(g->gctl, &G, sizeof(G), hipMemcpyHostToDevice));
+    const int md[2] = {g->max_cycles, 0};
+    HIPCHK(hipMemcpy(&g->gctl->stoptime, sq, sizeof(sq), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(&g->gctl->max_cycles, md, sizeof(md), hipMemcpyHostToDevice));
     return LULESH_OK;
 }
 
@@ -732,18 +740,29 @@ const char *lulesh_status_string(int status)
 
 const char *lulesh_version(void) { return "lulesh_amd 0.1 (gfx950)"; }
 
+// Host allocation failures inside the library (std::bad_alloc) end as
+// LULESH_ENOMEM at the C boundary; no C++ exception crosses it.
 int lulesh_group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **out)
 {
-    lulesh_group *g = nullptr;
-    const int rc = group_create(nx, tp, nr, balance, cost, &g);
-    if (rc) { group_free(g); return rc; }
-    *out = g;
-    return LULESH_OK;
+    if (!out) return LULESH_EINVAL;
+    try {
+        lulesh_group *g = nullptr;
+        const int rc = group_create(nx, tp, nr, balance, cost, &g);
+        if (rc) { group_free(g); return rc; }
+        *out = g;
+        return LULESH_OK;
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
 }
 
 int lulesh_group_create_mesh(const lulesh_mesh *m, int nr, int balance, int cost, lulesh_group **out)
 {
-    return group_create_mesh(m, nr, balance, cost, out);
+    try {
+        return group_create_mesh(m, nr, balance, cost, out);
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
 }
 
 int lulesh_group_destroy(lulesh_group *g)
@@ -1047,6 +1066,57 @@ int lulesh_comm_get_recv_buffer(lulesh_domain *d, double *host, size_t n)
     HIPCHK(hipStreamSynchronize(g->stream));
     HIPCHK(hipMemcpy(host, base, (size_t)d->host.comBufSize * 8, hipMemcpyDeviceToHost));
     return d->host.comBufSize;
+}
+
+// host-only layout queries: a decomposition described by (nx, tp, rank)
+static bool decomp_ok(int nx, int tp, int rank)
+{
+    return nx >= 1 && tp >= 1 && tp <= 4 && rank >= 0 && rank < tp * tp * tp &&
+           (long long)nx * nx * nx * 8 <= INT_MAX;
+}
+
+int lulesh_shm_recv_offset(int nx, int tp, int recvRank, int dCol, int dRow, int dPlane,
+                           int xferFields, int doRecv, int planeOnly)
+{
+    if (!decomp_ok(nx, tp, recvRank) || xferFields < 1 || xferFields > 6) return LULESH_EINVAL;
+    HostDomain h;
+    host_decomp(h, nx, tp, recvRank);
+    return recv_offset(h, dCol, dRow, dPlane, xferFields, doRecv != 0, planeOnly != 0);
+}
+
+int lulesh_comm_buf_size_of(int nx, int tp, int rank)
+{
+    if (!decomp_ok(nx, tp, rank)) return LULESH_EINVAL;
+    HostDomain h;
+    host_decomp(h, nx, tp, rank);
+    return h.comBufSize;
+}
+
+int lulesh_region_layout(int nx, int tp, int rank, int nr, int balance, int cost, int *region,
+                         int *rep, size_t n)
+{
+    if (!decomp_ok(nx, tp, rank) || !problem_ok(nx, tp, nr, balance, cost)) return LULESH_EINVAL;
+    if (n < (size_t)nx * nx * nx) return LULESH_EINVAL;
+    try {
+        HostDomain h;
+        host_decomp(h, nx, tp, rank);
+        host_regions(h, nr, balance, cost);
+        if (region) memcpy(region, h.region.data(), h.region.size() * sizeof(int));
+        if (rep) memcpy(rep, h.elemRep.data(), h.elemRep.size() * sizeof(int));
+        return h.numElem;
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
+}
+
+int lulesh_domain_get_eos_table(lulesh_domain *d, uint16_t *host, size_t n)
+{
+    if (!d || !host) return LULESH_EINVAL;
+    const long long total = d->host.evtotal;
+    if (n < (size_t)total) return LULESH_EINVAL;
+    HIPCHK(hipStreamSynchronize(d->g->stream));
+    if (total) HIPCHK(hipMemcpy(host, d->dev.evtab, (size_t)total * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    return (int)total;
 }
 
 const lulesh_comm_backend_ops *lulesh_device_local_backend(void) { return &kDeviceLocal; }

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -49,14 +50,26 @@ static const int kSbn[3] = {LULESH_FX, LULESH_FY, LULESH_FZ};
 static const int kPosVel[6] = {LULESH_X, LULESH_Y, LULESH_Z, LULESH_XD, LULESH_YD, LULESH_ZD};
 static const int kMonoQ[3] = {LULESH_DELV_XI, LULESH_DELV_ETA, LULESH_DELV_ZETA};
 
+// Graph execs may still be queued or running on the stream they were last
+// launched on: that stream is drained before an exec is destroyed.
+static void drop_cached(Engine::CachedGraph &c)
+{
+    if (c.exec) {
+        (void)hipStreamSynchronize(c.stream);
+        (void)hipGraphExecDestroy(c.exec);
+    }
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c = Engine::CachedGraph{};
+}
+
 static void engine_drop_graph(Engine *e)
 {
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    e->exec = nullptr;
-    e->graph = nullptr;
-    e->graph_cycles = 0;
+    for (auto &c : e->graphs) drop_cached(c);
+    e->graphs.clear();
 }
+
+// at most this many captured shapes are kept per engine
+static const size_t kGraphCache = 4;
 
 // Column-tile shape for a mesh of nx elements per edge: ex x ey element
 // slots (<= 256) covering (ex-1) x (ey-1) nodes each, with an
@@ -521,24 +534,41 @@ int engine_enqueue(Engine *e, int ncycles, hipStream_t s)
         }
         return LULESH_OK;
     }
-    if (!e->exec || e->graph_cycles != ncycles || e->graph_stream != s) {
-        engine_drop_graph(e);
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = LULESH_OK;
-        for (int c = 0; c < ncycles && rc == LULESH_OK; ++c) rc = engine_enqueue_cycle(e, s);
-        hipGraph_t graph = nullptr;
-        const hipError_t ce = hipStreamEndCapture(s, &graph);
-        if (rc) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc;
+    for (size_t i = 0; i < e->graphs.size(); ++i) {
+        if (e->graphs[i].ncycles == ncycles && e->graphs[i].stream == s) {
+            Engine::CachedGraph c = e->graphs[i];
+            e->graphs.erase(e->graphs.begin() + (long)i);
+            e->graphs.push_back(c);
+            HIPCHK(hipGraphLaunch(c.exec, s));
+            return LULESH_OK;
         }
-        HIPCHK(ce);
-        e->graph = graph;
-        HIPCHK(hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0));
-        e->graph_cycles = ncycles;
-        e->graph_stream = s;
     }
-    HIPCHK(hipGraphLaunch(e->exec, s));
+    if (e->graphs.size() >= kGraphCache) {   // least recently used
+        drop_cached(e->graphs.front());
+        e->graphs.erase(e->graphs.begin());
+    }
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = LULESH_OK;
+    for (int c = 0; c < ncycles && rc == LULESH_OK; ++c) rc = engine_enqueue_cycle(e, s);
+    Engine::CachedGraph c;
+    c.ncycles = ncycles;
+    c.stream = s;
+    const hipError_t ce = hipStreamEndCapture(s, &c.graph);
+    if (rc) {
+        if (c.graph) (void)hipGraphDestroy(c.graph);
+        return rc;
+    }
+    if (ce != hipSuccess) {
+        if (c.graph) (void)hipGraphDestroy(c.graph);
+        HIPCHK(ce);
+    }
+    const hipError_t ie = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(c.graph);
+        HIPCHK(ie);
+    }
+    e->graphs.push_back(c);
+    HIPCHK(hipGraphLaunch(c.exec, s));
     return LULESH_OK;
 }
 
@@ -557,7 +587,7 @@ struct lulesh_batch {
 
 extern "C" {
 
-int lulesh_batch_create(int nprob, const int *nx, const int *nr, const int *balance, const int *cost,
+static int batch_create(int nprob, const int *nx, const int *nr, const int *balance, const int *cost,
                         lulesh_batch **out)
 {
     if (nprob < 1 || !nx || !nr || !balance || !cost || !out) return LULESH_EINVAL;
@@ -580,6 +610,16 @@ int lulesh_batch_create(int nprob, const int *nx, const int *nr, const int *bala
     if (rc) { lulesh_batch_destroy(b); return rc; }
     *out = b;
     return LULESH_OK;
+}
+
+int lulesh_batch_create(int nprob, const int *nx, const int *nr, const int *balance, const int *cost,
+                        lulesh_batch **out)
+{
+    try {
+        return batch_create(nprob, nx, nr, balance, cost, out);
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
 }
 
 int lulesh_batch_destroy(lulesh_batch *b)
@@ -815,7 +855,7 @@ int lulesh_server_destroy(lulesh_server *s)
 
 double lulesh_server_now(lulesh_server *s) { return s ? s->now() : 0.0; }
 
-int lulesh_server_submit(lulesh_server *s, int nx, int nr, int balance, int cost, double arrival)
+static int server_submit(lulesh_server *s, int nx, int nr, int balance, int cost, double arrival)
 {
     if (!s || !problem_ok(nx, 1, nr, balance, cost)) return LULESH_EINVAL;
     Request r;
@@ -827,6 +867,15 @@ int lulesh_server_submit(lulesh_server *s, int nx, int nr, int balance, int cost
     s->queue.push_back(r);
     s->results.push_back(Result());
     return r.id;
+}
+
+int lulesh_server_submit(lulesh_server *s, int nx, int nr, int balance, int cost, double arrival)
+{
+    try {
+        return server_submit(s, nx, nr, balance, cost, arrival);
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
 }
 
 // admit arrived requests in arrival order while a slot and HBM are free
@@ -852,7 +901,7 @@ static int server_admit(lulesh_server *s)
     return LULESH_OK;
 }
 
-int lulesh_server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream)
+static int server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream)
 {
     if (!s) return LULESH_EINVAL;
     (void)stream;
@@ -905,6 +954,15 @@ int lulesh_server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream)
         }
     }
     return completed;
+}
+
+int lulesh_server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream)
+{
+    try {
+        return server_poll(s, cycles, stream);
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
 }
 
 int lulesh_server_result(lulesh_server *s, int id, int *status, int *cycles, double *energy, double *tad)

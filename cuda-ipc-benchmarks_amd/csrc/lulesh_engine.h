@@ -116,10 +116,16 @@ struct Engine {
     bool direct = false;
     bool direct_pv = false;
     bool umesh = false;         // members are unstructured meshes
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int graph_cycles = 0;
-    hipStream_t graph_stream = nullptr;
+    // captured graphs of n cycles, one per (n, stream), most recent last;
+    // a caller that alternates a chunk and its remainder replays both
+    // without capturing inside its loop
+    struct CachedGraph {
+        int ncycles = 0;
+        hipStream_t stream = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<CachedGraph> graphs;
     bool use_graph = true;
 };
 

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <new>
 #include <string>
 
 namespace lulesh {
@@ -43,6 +44,14 @@ struct Tokens {
         if (!next_long(l) || l < INT_MIN || l > INT_MAX) return false;
         v = (int)l;
         return true;
+    }
+    // whether the rest of the text can still hold `n` numbers (each takes
+    // at least one character and a separator): counts read from the header
+    // are checked with it before anything is sized by them
+    bool room_for(long long n) const
+    {
+        const long long left = (long long)(text.size() - (size_t)(p - text.c_str()));
+        return n >= 0 && n <= (left + 1) / 2;
     }
     // fscanf's %lf conversion: strtod, correctly rounded
     bool next_double(double &v)
@@ -134,9 +143,8 @@ using namespace lulesh;
 
 extern "C" {
 
-int lulesh_mesh_read(const char *path, lulesh_mesh **out)
+static int mesh_read(const char *path, lulesh_mesh **out)
 {
-    if (!path || !out) return LULESH_EINVAL;
     Tokens t;
     if (!read_file(path, t.text)) return LULESH_EIO;
     t.p = t.text.c_str();
@@ -149,6 +157,8 @@ int lulesh_mesh_read(const char *path, lulesh_mesh **out)
     if (!t.next_int(ne) || !t.next_int(nn)) return fail(LULESH_EIO);
     if (ne < 1 || nn < 1 || ne >= kMaxMeshElems || (long long)nn * 8 > INT_MAX)
         return fail(LULESH_EINVAL);
+    // a short file fails here, before the counts size any array
+    if (!t.room_for(3LL * nn + 14LL * ne + 4)) return fail(LULESH_EIO);
     m->numElem = ne;
     m->numNode = nn;
     m->x.resize(nn);
@@ -168,6 +178,7 @@ int lulesh_mesh_read(const char *path, lulesh_mesh **out)
         int cnt;
         if (!t.next_int(cnt)) return fail(LULESH_EIO);
         if (cnt < 0 || cnt > INT_MAX / 2) return fail(LULESH_EINVAL);
+        if (!t.room_for(cnt)) return fail(LULESH_EIO);
         s->resize((size_t)cnt);
         for (auto &v : *s)
             if (!t.next_int(v)) return fail(LULESH_EIO);
@@ -178,7 +189,17 @@ int lulesh_mesh_read(const char *path, lulesh_mesh **out)
     return LULESH_OK;
 }
 
-int lulesh_mesh_create(int numElem, int numNode, const double *x, const double *y, const double *z,
+int lulesh_mesh_read(const char *path, lulesh_mesh **out)
+{
+    if (!path || !out) return LULESH_EINVAL;
+    try {
+        return mesh_read(path, out);
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
+}
+
+static int mesh_create(int numElem, int numNode, const double *x, const double *y, const double *z,
                        const int *nodelist, const int *nbr, const int *symmX, int nSymmX, const int *symmY,
                        int nSymmY, const int *symmZ, int nSymmZ, const int *freeSurf, int nFree,
                        lulesh_mesh **out)
@@ -207,6 +228,19 @@ int lulesh_mesh_create(int numElem, int numNode, const double *x, const double *
     }
     *out = m;
     return LULESH_OK;
+}
+
+int lulesh_mesh_create(int numElem, int numNode, const double *x, const double *y, const double *z,
+                       const int *nodelist, const int *nbr, const int *symmX, int nSymmX, const int *symmY,
+                       int nSymmY, const int *symmZ, int nSymmZ, const int *freeSurf, int nFree,
+                       lulesh_mesh **out)
+{
+    try {
+        return mesh_create(numElem, numNode, x, y, z, nodelist, nbr, symmX, nSymmX, symmY, nSymmY, symmZ,
+                           nSymmZ, freeSurf, nFree, out);
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
 }
 
 int lulesh_mesh_destroy(lulesh_mesh *m)

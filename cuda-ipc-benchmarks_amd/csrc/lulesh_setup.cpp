@@ -105,9 +105,26 @@ long long max_elem_rep(int nr, int cost)
     return (nr + 15) / 20 >= 1 ? 10 * (1 + (long long)cost) : 1 + (long long)cost;
 }
 
+// CreateRegionIndexSets accumulates the region weights (i+1)^balance in
+// an int (lulesh-init.cc:384-388): every partial sum must stay below
+// INT_MAX, or the bins wrap and the region search runs off their end.  The
+// region count is also capped (one int per region on the host).
+static bool region_weights_ok(int nr, int balance)
+{
+    if (nr < 1 || nr > kMaxRegions || balance < 0) return false;
+    if (nr == 1) return true;
+    double sum = 0.0;
+    for (int i = 0; i < nr; ++i) {
+        sum = floor(sum + pow((double)(i + 1), (double)balance));
+        if (!(sum <= (double)INT_MAX)) return false;
+    }
+    return true;
+}
+
 bool problem_ok(int nx, int tp, int nr, int balance, int cost)
 {
     if (nx < 1 || tp < 1 || tp > 4 || nr < 1 || balance < 0 || cost < 0) return false;
+    if (!region_weights_ok(nr, balance)) return false;
     const long long ne = (long long)nx * nx * nx;
     if (ne * 8 > 2147483647LL) return false;
     return ne * max_elem_rep(nr, cost) <= 2147483647LL;
@@ -116,6 +133,7 @@ bool problem_ok(int nx, int tp, int nr, int balance, int cost)
 bool mesh_problem_ok(int numElem, int nr, int balance, int cost)
 {
     if (numElem < 1 || numElem >= kMaxMeshElems || nr < 1 || balance < 0 || cost < 0) return false;
+    if (!region_weights_ok(nr, balance)) return false;
     return (long long)numElem * max_elem_rep(nr, cost) <= INT_MAX;
 }
 
@@ -143,7 +161,7 @@ static void build_eval_table(HostDomain &h)
     }
 }
 
-void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost)
+void host_decomp(HostDomain &h, int nx, int tp, int rank)
 {
     h.nx = nx;
     h.tp = tp;
@@ -187,7 +205,13 @@ void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int bala
         if (present[f]) { h.ghost[f] = pidx; pidx += nx * nx; }
         else h.ghost[f] = -1;
     }
+}
 
+void host_regions(HostDomain &h, int nr, int balance, int cost) { create_regions(h, nr, balance, cost); }
+
+void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost)
+{
+    host_decomp(h, nx, tp, rank);
     create_regions(h, nr, balance, cost);
     build_eval_table(h);
 

@@ -31,6 +31,11 @@ struct HostDomain {
     std::vector<unsigned char> u_nsymm;
 };
 
+// decomposition of rank `rank` of tp^3 subdomains: position, neighbour
+// flags, comm-buffer sizes (SetupCommBuffers) and ghost offsets
+void host_decomp(HostDomain &h, int nx, int tp, int rank);
+// region layout (glibc rand()) and elemRep of a decomposed subdomain
+void host_regions(HostDomain &h, int nr, int balance, int cost);
 // decomposition, comm-buffer sizes, region layout (glibc rand()), e0 and
 // dt0; the mesh, volumes and nodal masses are built on the device
 void build_host_domain(HostDomain &h, int nx, int tp, int rank, int nr, int balance, int cost);
@@ -40,8 +45,12 @@ void build_host_domain_mesh(HostDomain &h, const lulesh_mesh &m, int nr, int bal
 size_t domain_device_bytes(int nx, bool corner_array, long long evtotal);
 // largest elemRep the region layout of (nr, cost) can assign
 long long max_elem_rep(int nr, int cost);
+// most regions a problem may have
+constexpr int kMaxRegions = 1 << 20;
 // problem parameters the library accepts: 32-bit element and node
-// indexing, at most 4x4x4 subdomains, an EOS table under 2^31 entries
+// indexing, at most 4x4x4 subdomains, an EOS table under 2^31 entries,
+// at most kMaxRegions regions whose weights sum_i (i+1)^balance stay
+// below 2^31 - 1 (the reference sums them in an int)
 bool problem_ok(int nx, int tp, int nr, int balance, int cost);
 bool mesh_problem_ok(int numElem, int nr, int balance, int cost);
 

@@ -75,6 +75,10 @@ SIGNATURES = {
     "lulesh_comm_buf_size": (_i, [_vp]),
     "lulesh_comm_recv_offset": (_i, [_vp, _i, _i, _i, _i, _i, _i]),
     "lulesh_comm_get_recv_buffer": (_i, [_vp, _dp, _sz]),
+    "lulesh_shm_recv_offset": (_i, [_i, _i, _i, _i, _i, _i, _i, _i, _i]),
+    "lulesh_comm_buf_size_of": (_i, [_i, _i, _i]),
+    "lulesh_region_layout": (_i, [_i, _i, _i, _i, _i, _i, _ip, _ip, _sz]),
+    "lulesh_domain_get_eos_table": (_i, [_vp, ctypes.POINTER(ctypes.c_uint16), _sz]),
     "lulesh_device_local_backend": (_vp, []),
     "lulesh_group_set_comm_backend": (_i, [_vp, _vp, _vp]),
     "lulesh_group_exchange": (_i, [_vp, _i, _i, _ip, _vp]),
@@ -191,6 +195,13 @@ class Domain:
         return out
 
     # the reference's comm interface
+    def eos_table(self):
+        """The EOS work table the EOS kernel walks (uint16 entries)."""
+        L = load()
+        out = np.empty(max(1, int(np.sum(self.elem_rep(), dtype=np.int64))), dtype=np.uint16)
+        n = _chk(L.lulesh_domain_get_eos_table(self.h, _p(out, ctypes.c_uint16), out.size), "eos_table")
+        return out[:n]
+
     def comm_recv(self, msg_type, xfer, dx, dy, dz, do_recv, plane_only):
         return load().CommRecv(self.h, msg_type, xfer, dx, dy, dz, int(do_recv), int(plane_only))
 
@@ -521,6 +532,26 @@ class Server:
         _chk(load().lulesh_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(d),
                                         ctypes.byref(b)), "stats")
         return dict(queued=q.value, active=a.value, done=d.value, hbm_in_use=b.value)
+
+
+# host-only layout queries (no GPU needed)
+def shm_recv_offset(nx, tp, recv_rank, dc, dr, dp, xfer, do_recv, plane_only):
+    """shmRecvOffset of a tp^3 decomposition of nx^3-element subdomains."""
+    return load().lulesh_shm_recv_offset(nx, tp, recv_rank, dc, dr, dp, xfer, int(do_recv), int(plane_only))
+
+
+def comm_buf_size_of(nx, tp, rank):
+    return _chk(load().lulesh_comm_buf_size_of(nx, tp, rank), "comm_buf_size_of")
+
+
+def region_layout(nx, tp=1, rank=0, nr=11, balance=1, cost=1):
+    """(region, elemRep) of rank `rank`: CreateRegionIndexSets' layout."""
+    ne = nx ** 3
+    reg = np.empty(ne, dtype=np.int32)
+    rep = np.empty(ne, dtype=np.int32)
+    _chk(load().lulesh_region_layout(nx, tp, rank, nr, balance, cost, _p(reg, ctypes.c_int),
+                                     _p(rep, ctypes.c_int), ne), "region_layout")
+    return reg, rep
 
 
 def probe_cbrt(x):

@@ -91,7 +91,10 @@ const char *lulesh_version(void);
  * (lulesh.cc:3140-3161).  LULESH_EINVAL when nx^3 * 8 passes 2^31 - 1
  * (32-bit byte offsets into an element array: nx <= 645) or nx^3 times the
  * largest EOS repetition count does (10 (1 + cost) with nr >= 5, else
- * 1 + cost: nx <= 475 at nr 11, cost 1). */
+ * 1 + cost: nx <= 475 at nr 11, cost 1), and when the region weights
+ * overflow the reference's int sum (sum over regions of (i+1)^balance above
+ * 2^31 - 1, lulesh-init.cc:384-388: e.g. -r 11 -b 9) or nr passes 2^20.
+ * LULESH_ENOMEM when host or device memory runs out. */
 int lulesh_group_create(int nx, int tp, int nr, int balance, int cost,
                         lulesh_group **out);
 int lulesh_group_destroy(lulesh_group *g);
@@ -223,6 +226,13 @@ int lulesh_domain_field_ptr(lulesh_domain *d, int field, double **dev_ptr);
  * the per-element repetition count of main (lulesh.cc:3171-3192). */
 int lulesh_domain_get_elem_rep(lulesh_domain *d, int *host, size_t n);
 int lulesh_domain_get_region(lulesh_domain *d, int *host, size_t n);
+/* Host copy of the EOS work table the EOS kernel walks (the repetitions of
+ * EvalEOSForElems, lulesh.cc:2677-2815, made explicit): one uint16 per EOS
+ * evaluation, elements ascending, element i listed elemRep[i] times; low
+ * 15 bits = element index mod 2^15, bit 15 set on the element's last
+ * repetition.  Returns the number of entries (the sum of elemRep), or the
+ * status when n is smaller. */
+int lulesh_domain_get_eos_table(lulesh_domain *d, uint16_t *host, size_t n);
 
 /* ------------------------------------------------------------------ */
 /* halo exchange: the reference's comm interface                       */
@@ -254,6 +264,22 @@ int lulesh_comm_recv_offset(const lulesh_domain *d, int dcol, int drow, int dpla
                             int xferFields, int doRecv, int planeOnly);
 /* Host copy of a subdomain's receive buffer (comBufSize doubles). */
 int lulesh_comm_get_recv_buffer(lulesh_domain *d, double *host, size_t n);
+
+/* The same layout without a group (host only; usable without a GPU):
+ * shmRecvOffset(domain, recvRank, dCol, dRow, dPlane, xferFields, doRecv,
+ * planeOnly) of cuda comm/comm_backend.h:75-138 for subdomains of nx^3
+ * elements in a tp^3 decomposition (-1: no message expected from that
+ * direction), and comBufSize of rank `rank` (lulesh-init.cc:317-372). */
+int lulesh_shm_recv_offset(int nx, int tp, int recvRank, int dCol, int dRow, int dPlane,
+                           int xferFields, int doRecv, int planeOnly);
+int lulesh_comm_buf_size_of(int nx, int tp, int rank);
+/* Region layout of CreateRegionIndexSets (lulesh-init.cc:377-484, glibc
+ * rand() seeded with the rank) and elemRep of main (lulesh.cc:3171-3192)
+ * for rank `rank` of a tp^3 decomposition, host only: region[i] is the
+ * 0-based region of element i, rep[i] its EOS repetition count.  n = the
+ * capacity of each array (nx^3 needed).  Returns nx^3. */
+int lulesh_region_layout(int nx, int tp, int rank, int nr, int balance, int cost,
+                         int *region, int *rep, size_t n);
 
 /* Backend hook table: the run-time form of the compile-time backend macros
  * of LULESH/cuda/src/comm/comm_backend.h:7-22,156-173 (commAllocRecv,

@@ -11,6 +11,11 @@
 // With LULESH_REF_DUMP=<path> set, the dump is a raw little-endian file:
 //   int32 nx, int32 cycle, float64 time, float64 deltatime,
 //   then per field (e p q v ss x y z xd yd zd) its float64 values.
+// With LULESH_REF_REGIONS=<path> set, the region layout of
+// CreateRegionIndexSets (lulesh-init.cc:377-484) and the elemRep of main
+// (lulesh.cc:3171-3192) are written as int32: numElem, numReg, then
+// regNumList[numElem] (1-based region of each element), elemRep[numElem],
+// regElemSize[numReg].
 #include <stdio.h>
 #include <stdlib.h>
 #include "lulesh.h"
@@ -42,6 +47,21 @@ void VerifyAndWriteFinalOutput(Real_t elapsed_time, Domain& locDom,
          put(f, locDom.m_v, ne); put(f, locDom.m_ss, ne);
          put(f, locDom.m_x, nn); put(f, locDom.m_y, nn); put(f, locDom.m_z, nn);
          put(f, locDom.m_xd, nn); put(f, locDom.m_yd, nn); put(f, locDom.m_zd, nn);
+         fclose(f);
+      }
+   }
+   const char *rpath = getenv("LULESH_REF_REGIONS");
+   if (rpath != NULL && rpath[0] != '\0') {
+      FILE *f = fopen(rpath, "wb");
+      if (f != NULL) {
+         int hdr[2] = { (int)locDom.numElem(), (int)locDom.numReg() };
+         fwrite(hdr, sizeof(int), 2, f);
+         fwrite(locDom.regNumList(), sizeof(Index_t), (size_t)locDom.numElem(), f);
+         fwrite(locDom.m_elemRep.data(), sizeof(Index_t), (size_t)locDom.numElem(), f);
+         for (Int_t r = 0; r < locDom.numReg(); ++r) {
+            Index_t n = locDom.regElemSize(r);
+            fwrite(&n, sizeof(Index_t), 1, f);
+         }
          fclose(f);
       }
    }

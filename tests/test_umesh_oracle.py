@@ -105,6 +105,52 @@ def test_library_reader_errors(tmp_path):
                                     ctypes.byref(h)) == la.EINVAL
 
 
+def test_library_reader_huge_counts(tmp_path):
+    """Counts in the header are checked against what the rest of the file
+    can hold before any array is sized by them: a few dozen bytes claiming
+    2^28 - 1 nodes, or a symmetry set of 10^9 entries, fail with EIO at
+    once instead of committing gigabytes of host memory."""
+    import ctypes
+    import resource
+    import lulesh_amd as la
+    L = la.load()
+    h = ctypes.c_void_p()
+    cases = {
+        "huge_nodes.lmesh": "1 268435455\n0 0 0\n",
+        "huge_elems.lmesh": "67108863 8\n" + "0 0 0\n" * 8,
+        "huge_set.lmesh": None,
+        "mid_record.lmesh": None,
+    }
+    text = open(LMESH).read()
+    tok = text.split()
+    ne, nn = int(tok[0]), int(tok[1])
+    head = 2 + 3 * nn + 14 * ne
+    # a valid file up to the first symmetry set, which claims 10^9 entries
+    cases["huge_set.lmesh"] = " ".join(tok[:head]) + " 1000000000 1 2 3\n"
+    # a file cut in the middle of a node's coordinate triple
+    cases["mid_record.lmesh"] = " ".join(tok[:2 + 3 * (nn // 2) + 1]) + "\n"
+    before = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss
+    for name, body in cases.items():
+        p = tmp_path / name
+        p.write_text(body)
+        assert L.lulesh_mesh_read(str(p).encode(), ctypes.byref(h)) == la.EIO, name
+    grown_kb = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss - before
+    assert grown_kb < 256 * 1024   # far below the 6 GB a 2^28-node file would size
+
+
+def test_region_weights_refused_at_the_boundary():
+    """-r 11 -b 9 sums the region weights past 2^31 - 1 in the reference's
+    int (lulesh-init.cc:384-388): group creation refuses it before touching
+    the GPU, as it refuses the other bad arguments."""
+    import ctypes
+    import lulesh_amd as la
+    L = la.load()
+    h = ctypes.c_void_p()
+    for nx, nr, b, c in [(8, 11, 9, 1), (8, 100, 5, 1), (8, 0, 1, 1), (8, 11, -1, 1), (8, 1 << 21, 0, 1)]:
+        assert L.lulesh_group_create(nx, 1, nr, b, c, ctypes.byref(h)) == la.EINVAL
+        assert L.lulesh_server_submit(None, nx, nr, b, c, 0.0) == la.EINVAL
+
+
 def test_block_boundary_bits(oracle_mod):
     # the block as an unstructured mesh: the reader's bits are the block's
     # elemBC with every FREE face marked SYMM instead (the reference's
