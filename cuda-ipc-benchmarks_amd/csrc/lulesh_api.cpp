@@ -248,7 +248,28 @@ static int dl_send_finish(lulesh_domain *, lulesh_stream_t, void *) { return LUL
 static double *dl_recv_base(lulesh_domain *d, void *) { return d->recv; }
 
 static const lulesh_comm_backend_ops kDeviceLocal = {
-    "device_local", dl_alloc_recv, dl_teardown, dl_pack_dest, dl_send_finish, dl_recv_base};
+    "device_local", dl_alloc_recv, dl_teardown, dl_pack_dest, dl_send_finish, dl_recv_base,
+    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+
+// ---------------------------------------------------------------------------
+// device_copy backend (the reference's Mode A, comm_ipc_common.h:192-225):
+// every message is packed into the sender's own send buffer, then copied
+// device to device into the receiver's slot -- one copy per message, as
+// COMM_SEND_MSG / COMM_SEND_CORNER do over IPC.
+// ---------------------------------------------------------------------------
+static double *dc_pack_dest(lulesh_domain *, lulesh_domain *, size_t, void *) { return nullptr; }
+
+static int dc_send(lulesh_domain *, lulesh_domain *to, size_t offset, const double *src, size_t count,
+                   lulesh_stream_t stream, void *)
+{
+    const hipError_t e = hipMemcpyAsync(to->recv + offset, src, count * sizeof(double),
+                                        hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    return e == hipSuccess ? LULESH_OK : LULESH_EHIP;
+}
+
+static const lulesh_comm_backend_ops kDeviceCopy = {
+    "device_copy", dl_alloc_recv, dl_teardown, dc_pack_dest, dl_send_finish, dl_recv_base,
+    nullptr, nullptr, nullptr, nullptr, dc_send, dc_send, nullptr, nullptr, nullptr, nullptr};
 
 // ---------------------------------------------------------------------------
 // comm plans
@@ -277,6 +298,15 @@ int group_plan(lulesh_group *g, int msgType, int nf, const int *fields, bool doF
     auto it = g->plans.find(key);
     if (it != g->plans.end()) { *out = it->second.get(); return LULESH_OK; }
 
+    const lulesh_comm_backend_ops *ops = g->backend;
+    // a backend may pack locally: every subdomain gets its send buffer
+    if (ops != &kDeviceLocal)
+        for (auto *d : g->doms)
+            if (!d->send) {
+                HIPCHK(hipMalloc((void **)&d->send, (size_t)std::max(1, d->host.comBufSize) * sizeof(double)));
+                HIPCHK(hipMemset(d->send, 0, (size_t)std::max(1, d->host.comBufSize) * sizeof(double)));
+            }
+    const int mode = msgType == LULESH_MSG_COMM_SBN ? 0 : msgType == LULESH_MSG_SYNC_POS_VEL ? 1 : 2;
     PlanInput in;
     in.tp = g->tp;
     for (auto *d : g->doms) in.doms.push_back(&d->host);
@@ -286,17 +316,29 @@ int group_plan(lulesh_group *g, int msgType, int nf, const int *fields, bool doF
     in.pack_dest = [g](int from, int to, size_t off) {
         return g->backend->pack_dest(g->doms[from], g->doms[to], off, g->backend_ctx);
     };
+    in.send_base = [g](int from) { return g->doms[from]->send; };
     in.recv_base = [g](int dom) -> const double * {
         return g->backend->recv_base(g->doms[dom], g->backend_ctx);
     };
+    in.skip_corners = (mode == 0 && ops->add_corner) || (mode == 1 && ops->copy_corner);
+    in.skip_all = mode == 2 && ops->monoq_copy;
     HostPlan hp;
     const int rc = build_plan(in, hp);
     if (rc != 0) return LULESH_EDIRECTION;
+    for (const MsgRef &m : hp.sends)
+        if (m.local && !g->doms[m.from]->send) return LULESH_EINVAL;
 
     std::unique_ptr<DevPlan> p(new DevPlan());
     p->msgType = msgType;
     p->nfields = nf;
-    p->mode = msgType == LULESH_MSG_COMM_SBN ? 0 : msgType == LULESH_MSG_SYNC_POS_VEL ? 1 : 2;
+    p->mode = mode;
+    p->sends = hp.sends;
+    p->recvs = hp.recvs;
+    p->send_start = hp.send_start;
+    p->recv_start = hp.recv_start;
+    p->fields.assign(fields, fields + nf);
+    p->corner_hook = in.skip_corners;
+    p->monoq_hook = in.skip_all;
     p->npack = (int)hp.pack.size();
     p->ntgt = (int)hp.tgt.size();
     p->pack_start = hp.pack_start;
@@ -319,25 +361,107 @@ int group_plan(lulesh_group *g, int msgType, int nf, const int *fields, bool doF
         }
     HIPCHK(hipMalloc((void **)&p->fptr, fp.size() * sizeof(double *)));
     HIPCHK(hipMemcpy(p->fptr, fp.data(), fp.size() * sizeof(double *), hipMemcpyHostToDevice));
+    p->host_fptr = fp;
     *out = p.get();
     g->plans[key] = std::move(p);
     return LULESH_OK;
 }
 
+// where messages to d land: COMM_RECV_POST_BUF, else the unpack buffer
+static double *post_buf(lulesh_group *g, lulesh_domain *d)
+{
+    const lulesh_comm_backend_ops *ops = g->backend;
+    return ops->recv_post_buf ? ops->recv_post_buf(d, g->backend_ctx) : ops->recv_base(d, g->backend_ctx);
+}
+
+// words of one message over all its fields (a corner message carries one
+// word per field)
+static size_t msg_words(const DevPlan *p, const MsgRef &m) { return (size_t)p->nfields * m.cnt; }
+
+// CommSend of ranks [r0, r1): the pack kernel, then each locally packed
+// message handed to the backend (COMM_SEND_MSG / COMM_SEND_CORNER)
 int plan_pack(lulesh_group *g, DevPlan *p, int rank, hipStream_t s, bool gated = false)
 {
     const int a = rank < 0 ? 0 : p->pack_start[rank];
     const int b = rank < 0 ? p->npack : p->pack_start[rank + 1];
     HIPCHK(launch_pack(p->pack + a, b - a, p->fptr, p->nfields, gated ? g->gctl : nullptr, s));
+    const lulesh_comm_backend_ops *ops = g->backend;
+    const int m0 = rank < 0 ? 0 : p->send_start[rank];
+    const int m1 = rank < 0 ? (int)p->sends.size() : p->send_start[rank + 1];
+    for (int i = m0; i < m1; ++i) {
+        const MsgRef &m = p->sends[i];
+        if (!m.local) continue;
+        lulesh_domain *from = g->doms[m.from], *to = g->doms[m.to];
+        const double *src = from->send + m.send_off;
+        const size_t words = msg_words(p, m);
+        auto hook = m.kind == 3 ? ops->send_corner : ops->send_msg;
+        if (hook) {
+            const int rc = hook(from, to, m.recv_off, src, words, (lulesh_stream_t)s, g->backend_ctx);
+            if (rc) return rc;
+        } else {
+            HIPCHK(hipMemcpyAsync(post_buf(g, to) + m.recv_off, src, words * sizeof(double),
+                                  hipMemcpyDefault, s));
+        }
+    }
     return LULESH_OK;
 }
 
+// CommSBN / CommSyncPosVel / CommMonoQ of one rank (or all): staging of the
+// delivered messages into the unpack buffer (COMM_UNPACK_H2D), the unpack
+// kernel, then the messages a backend hook applies itself (corners,
+// COMM_ADD_CORNER / COMM_COPY_CORNER; MonoQ planes, COMM_MONOQ_COPY)
 int plan_unpack(lulesh_group *g, DevPlan *p, int rank, hipStream_t s, bool gated = false)
 {
+    const lulesh_comm_backend_ops *ops = g->backend;
+    const int m0 = rank < 0 ? 0 : p->recv_start[rank];
+    const int m1 = rank < 0 ? (int)p->recvs.size() : p->recv_start[rank + 1];
+    for (int i = m0; i < m1; ++i) {
+        const MsgRef &m = p->recvs[i];
+        lulesh_domain *d = g->doms[m.to];
+        double *base = ops->recv_base(d, g->backend_ctx);
+        const double *post = post_buf(g, d);
+        if (post == base) continue;
+        const size_t bytes = msg_words(p, m) * sizeof(double);
+        if (ops->unpack_h2d) {
+            const int rc = ops->unpack_h2d(d, base + m.recv_off, post + m.recv_off, bytes, (lulesh_stream_t)s,
+                                           g->backend_ctx);
+            if (rc) return rc;
+        } else {
+            HIPCHK(hipMemcpyAsync(base + m.recv_off, post + m.recv_off, bytes, hipMemcpyDefault, s));
+        }
+    }
     const int a = rank < 0 ? 0 : p->tgt_start[rank];
     const int b = rank < 0 ? p->ntgt : p->tgt_start[rank + 1];
     HIPCHK(launch_unpack(p->tgt + a, b - a, p->src, p->fptr, p->nfields, p->mode,
                          gated ? g->gctl : nullptr, s));
+    for (int i = m0; i < m1 && (p->corner_hook || p->monoq_hook); ++i) {
+        const MsgRef &m = p->recvs[i];
+        lulesh_domain *d = g->doms[m.to];
+        const double *base = ops->recv_base(d, g->backend_ctx);
+        if (p->corner_hook && m.kind == 3) {
+            if (gated && p->mode == 0) {
+                // a cycle after the group's stop condition must leave its
+                // forces alone: in the stepping engine the corner adds run
+                // as the library's own launch, which tests the stop flag
+                // (the copies of the other hooks rewrite what is already
+                // there, so they may run)
+                HIPCHK(launch_apply_corner(&p->host_fptr[(size_t)m.to * p->nfields], p->nfields, m.first,
+                                           base + m.recv_off, true, g->gctl, s));
+                continue;
+            }
+            auto hook = p->mode == 0 ? ops->add_corner : ops->copy_corner;
+            const int rc = hook(d, m.first, base + m.recv_off, p->nfields, p->fields.data(), (lulesh_stream_t)s,
+                                g->backend_ctx);
+            if (rc) return rc;
+        } else if (p->monoq_hook) {
+            for (int f = 0; f < p->nfields; ++f) {
+                double *dst = p->host_fptr[(size_t)m.to * p->nfields + f] + m.first;
+                const int rc = ops->monoq_copy(d, dst, base + m.recv_off + (size_t)f * m.cnt,
+                                               (size_t)m.cnt * sizeof(double), (lulesh_stream_t)s, g->backend_ctx);
+                if (rc) return rc;
+            }
+        }
+    }
     return LULESH_OK;
 }
 
@@ -666,9 +790,11 @@ void group_free(lulesh_group *g)
     g->plans.clear();
     for (auto *d : g->doms) {
         if (d->recv) g->backend->teardown(d, d->recv, g->backend_ctx);
+        if (d->send) (void)hipFree(d->send);
         if (d->arena) (void)hipFree(d->arena);
         delete d;
     }
+    if (g->backend && g->backend->runtime_shutdown) (void)g->backend->runtime_shutdown(g->backend_ctx);
     if (g->ctl_mem) (void)hipFree(g->ctl_mem);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
@@ -985,7 +1111,12 @@ int CommRecv(lulesh_domain *d, int msgType, int xferFields, int dx, int dy, int 
     const int rc = check_dims(d, msgType, dx, dy, dz);
     if (rc) return rc;
     if ((msgType == LULESH_MSG_MONOQ) != (planeOnly != 0)) return LULESH_EINVAL;
-    (void)doRecv;
+    lulesh_group *g = d->g;
+    if (g->ndom == 1) return LULESH_OK;   // numRanks == 1: nothing to post
+    if (g->backend->recv_skip) {
+        const int r = g->backend->recv_skip(d, msgType, xferFields, doRecv, planeOnly, g->backend_ctx);
+        if (r < 0) return r;
+    }
     return LULESH_OK;
 }
 
@@ -1121,12 +1252,20 @@ int lulesh_domain_get_eos_table(lulesh_domain *d, uint16_t *host, size_t n)
 
 const lulesh_comm_backend_ops *lulesh_device_local_backend(void) { return &kDeviceLocal; }
 
+const lulesh_comm_backend_ops *lulesh_device_copy_backend(void) { return &kDeviceCopy; }
+
 int lulesh_group_set_comm_backend(lulesh_group *g, const lulesh_comm_backend_ops *ops, void *ctx)
 {
     if (!g || g->umesh || !ops || !ops->alloc_recv || !ops->teardown || !ops->pack_dest ||
         !ops->send_finish || !ops->recv_base)
         return LULESH_EINVAL;
-    HIPCHK(hipStreamSynchronize(g->stream));
+    HIPCHK(hipDeviceSynchronize());
+    if (ops->runtime_init) {   // COMM_RUNTIME_INIT
+        const int rc = ops->runtime_init(ctx);
+        if (rc) return rc;
+    }
+    const lulesh_comm_backend_ops *old_ops = g->backend;
+    void *old_ctx = g->backend_ctx;
     // receive buffers belong to the backend: hand them over
     g->phase_setup_ms = 0.0;
     for (auto *d : g->doms) {
@@ -1138,15 +1277,32 @@ int lulesh_group_set_comm_backend(lulesh_group *g, const lulesh_comm_backend_ops
         HIPCHK(hipDeviceSynchronize());
         g->phase_setup_ms = std::max(
             g->phase_setup_ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
-        if (old) g->backend->teardown(d, old, g->backend_ctx);
+        if (old) old_ops->teardown(d, old, old_ctx);
         d->recv = fresh;
     }
+    if (old_ops && old_ops->runtime_shutdown) (void)old_ops->runtime_shutdown(old_ctx);
     g->comm_down = false;
     g->backend = ops;
     g->backend_ctx = ctx;
     g->plans.clear();
     g->direct.reset();
     if (g->engine) { engine_destroy(g->engine); g->engine = nullptr; }
+    return LULESH_OK;
+}
+
+int lulesh_comm_apply_corner(lulesh_domain *d, int node, const double *src, int xferFields,
+                             const int *fields, int add, lulesh_stream_t stream)
+{
+    if (!d || !src || !fields || xferFields < 1 || xferFields > 6 || node < 0 ||
+        node >= d->host.numNode)
+        return LULESH_EINVAL;
+    double *f[6];
+    for (int i = 0; i < xferFields; ++i) {
+        size_t len = 0;
+        f[i] = field_dev_ptr(d, fields[i], &len);
+        if (!f[i] || fields[i] >= 16) return LULESH_EINVAL;   // node fields only
+    }
+    HIPCHK(launch_apply_corner(f, xferFields, node, src, add != 0, nullptr, stream_of(d->g, stream)));
     return LULESH_OK;
 }
 

@@ -99,22 +99,38 @@ int build_plan(const PlanInput &in, HostPlan &out)
     out.pack.clear();
     out.tgt.clear();
     out.src.clear();
+    out.sends.clear();
+    out.recvs.clear();
     out.pack_start.assign(ndom + 1, 0);
     out.tgt_start.assign(ndom + 1, 0);
+    out.send_start.assign(ndom + 1, 0);
+    out.recv_start.assign(ndom + 1, 0);
     std::vector<int> idx;
 
     for (int s = 0; s < ndom; ++s) {
         out.pack_start[s] = (int)out.pack.size();
+        out.send_start[s] = (int)out.sends.size();
         const HostDomain &hs = *in.doms[s];
+        // the sender's own buffer slots (commDataSend, lulesh-comm.cc:357-844)
+        int spmsg = 0, semsg = 0, scmsg = 0;
         for (const Dir &o : kSendOrder) {
-            if (planeOnly && ndims(o) > 1) break;
+            const int nd = ndims(o);
+            if (planeOnly && nd > 1) break;
             if (!has_nbr(hs, o.dc, o.dr, o.dp) || (o.gated && !doFlag)) continue;
             const int r = s + o.dc + o.dr * tp + o.dp * tp * tp;
             const int off = recv_offset(*in.doms[r], -o.dc, -o.dr, -o.dp, nf, doFlag, planeOnly);
             if (off < 0) return -6;
+            const size_t soff = (size_t)spmsg * nf * hs.maxPlaneSize + (size_t)semsg * nf * hs.maxEdgeSize +
+                                (size_t)scmsg * 16;
+            if (nd == 1) ++spmsg;
+            else if (nd == 2) ++semsg;
+            else ++scmsg;
             boundary_set(n, o.dc, o.dr, o.dp, idx);
             double *dst = in.pack_dest(s, r, (size_t)off);
+            const bool local = dst == nullptr;
+            if (local) dst = in.send_base(s) + soff;
             const int cnt = (int)idx.size();
+            out.sends.push_back(MsgRef{s, r, nd, (size_t)off, soff, cnt, local, idx[0]});
             for (int i = 0; i < cnt; ++i) {
                 PackEnt e;
                 e.dst = dst + i;
@@ -127,9 +143,11 @@ int build_plan(const PlanInput &in, HostPlan &out)
         }
     }
     out.pack_start[ndom] = (int)out.pack.size();
+    out.send_start[ndom] = (int)out.sends.size();
 
     for (int r = 0; r < ndom; ++r) {
         out.tgt_start[r] = (int)out.tgt.size();
+        out.recv_start[r] = (int)out.recvs.size();
         const HostDomain &hr = *in.doms[r];
         // receiving index -> its messages in slot order
         std::map<int, std::vector<UnpackSrc>> lists;
@@ -143,7 +161,10 @@ int build_plan(const PlanInput &in, HostPlan &out)
             const int off = pmsg * nf * hr.maxPlaneSize + emsg * nf * hr.maxEdgeSize + cmsg * 16;
             boundary_set(n, o.dc, o.dr, o.dp, idx);
             const int cnt = (int)idx.size();
-            for (int i = 0; i < cnt; ++i) {
+            const int s = r + o.dc + o.dr * tp + o.dp * tp * tp;
+            out.recvs.push_back(MsgRef{s, r, nd, (size_t)off, 0, cnt, false, planeOnly ? ghost : idx[0]});
+            const bool skip = in.skip_all || (in.skip_corners && nd == 3);
+            for (int i = 0; i < cnt && !skip; ++i) {
                 UnpackSrc u;
                 u.src = base + off + i;
                 u.cnt = cnt;
@@ -167,6 +188,7 @@ int build_plan(const PlanInput &in, HostPlan &out)
         }
     }
     out.tgt_start[ndom] = (int)out.tgt.size();
+    out.recv_start[ndom] = (int)out.recvs.size();
     return 0;
 }
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "lulesh_amd.h"
+#include "lulesh_comm.h"
 #include "lulesh_dev.h"
 #include "lulesh_setup.h"
 
@@ -42,6 +43,13 @@ struct DevPlan {
     double **fptr = nullptr;
     int npack = 0, ntgt = 0;
     std::vector<int> pack_start, tgt_start;
+    // the step's messages (host side), for the backend's send / staging /
+    // corner / MonoQ hooks
+    std::vector<MsgRef> sends, recvs;
+    std::vector<int> send_start, recv_start;
+    std::vector<int> fields;
+    std::vector<double *> host_fptr;   // [rank * nfields + f]
+    bool corner_hook = false, monoq_hook = false;
     ~DevPlan();
 };
 }  // namespace lulesh
@@ -54,6 +62,7 @@ struct lulesh_domain {
     void *arena = nullptr;
     size_t arena_bytes = 0;
     double *recv = nullptr;
+    double *send = nullptr;     // send buffer (comBufSize), for backends that pack locally
     double *fpart[3] = {nullptr, nullptr, nullptr};   // own partial forces (groups with halos)
 };
 

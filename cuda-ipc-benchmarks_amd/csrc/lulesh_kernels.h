@@ -32,6 +32,10 @@ hipError_t launch_pack(const PackEnt *e, int n, const double *const *fptr, int n
                        const GroupCtl *gate, hipStream_t s);
 hipError_t launch_unpack(const UnpackTgt *t, int n, const UnpackSrc *src, double *const *fptr,
                          int nfields, int mode, const GroupCtl *gate, hipStream_t s);
+// one corner message of nf words at src applied to node idx (add or copy);
+// skipped once the gate's group has stopped
+hipError_t launch_apply_corner(double *const f[6], int nf, int idx, const double *src, bool add,
+                               const GroupCtl *gate, hipStream_t s);
 // Sedov set-up of one subdomain on the device: coordinates (block only),
 // volo, elemMass, v = 1, nodal masses
 hipError_t launch_setup(const DomDev &D, bool umesh, hipStream_t s);

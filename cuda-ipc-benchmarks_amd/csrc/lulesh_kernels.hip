@@ -1161,6 +1161,24 @@ __global__ __launch_bounds__(256) void k_unpack(const UnpackTgt *tg, int n, cons
     }
 }
 
+// a corner message applied on its own (backends with corner hooks): one
+// word per field onto one node, added (CommSBN) or copied (CommSyncPosVel)
+struct CornerArgs {
+    double *f[6];
+    const double *src;
+    int idx, nf, add;
+};
+
+__global__ __launch_bounds__(64) void k_apply_corner(CornerArgs a, const GroupCtl *gate)
+{
+    if (gate && group_stopped(*gate)) return;
+    const int f = threadIdx.x;
+    if (f >= a.nf) return;
+    const double v = a.src[f];
+    if (a.add) a.f[f][a.idx] += v;
+    else a.f[f][a.idx] = v;
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -1250,6 +1268,19 @@ hipError_t launch_unpack(const UnpackTgt *t, int n, const UnpackSrc *src, double
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_unpack, dim3(grid_of(n, 256)), dim3(256), 0, s, t, n, src, fptr, nfields, mode,
                        gate);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_corner(double *const f[6], int nf, int idx, const double *src, bool add,
+                               const GroupCtl *gate, hipStream_t s)
+{
+    CornerArgs a;
+    for (int i = 0; i < 6; ++i) a.f[i] = i < nf ? f[i] : nullptr;
+    a.src = src;
+    a.idx = idx;
+    a.nf = nf;
+    a.add = add ? 1 : 0;
+    hipLaunchKernelGGL(k_apply_corner, dim3(1), dim3(64), 0, s, a, gate);
     return hipGetLastError();
 }
 

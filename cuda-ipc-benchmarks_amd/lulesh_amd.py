@@ -80,6 +80,8 @@ SIGNATURES = {
     "lulesh_region_layout": (_i, [_i, _i, _i, _i, _i, _i, _ip, _ip, _sz]),
     "lulesh_domain_get_eos_table": (_i, [_vp, ctypes.POINTER(ctypes.c_uint16), _sz]),
     "lulesh_device_local_backend": (_vp, []),
+    "lulesh_device_copy_backend": (_vp, []),
+    "lulesh_comm_apply_corner": (_i, [_vp, _i, _dp, _i, _ip, _i, _vp]),
     "lulesh_group_set_comm_backend": (_i, [_vp, _vp, _vp]),
     "lulesh_group_exchange": (_i, [_vp, _i, _i, _ip, _vp]),
     "lulesh_batch_create": (_i, [_i, _ip, _ip, _ip, _ip, _pvp]),
@@ -421,6 +423,14 @@ class Group:
     def exchange(self, msg_type, fields):
         ids = np.array([FIELDS[f] for f in fields], dtype=np.int32)
         return load().lulesh_group_exchange(self.h, msg_type, len(ids), _p(ids, ctypes.c_int), None)
+
+    def set_comm_backend(self, name):
+        """Installs one of the library's comm backends: "device_local"
+        (remote pack) or "device_copy" (pack locally, one copy per message)."""
+        L = load()
+        ops = {"device_local": L.lulesh_device_local_backend,
+               "device_copy": L.lulesh_device_copy_backend}[name]()
+        _chk(L.lulesh_group_set_comm_backend(self.h, ops, None), "set_comm_backend")
 
 
 class Batch:

@@ -281,30 +281,94 @@ int lulesh_comm_buf_size_of(int nx, int tp, int rank);
 int lulesh_region_layout(int nx, int tp, int rank, int nr, int balance, int cost,
                          int *region, int *rep, size_t n);
 
-/* Backend hook table: the run-time form of the compile-time backend macros
- * of LULESH/cuda/src/comm/comm_backend.h:7-22,156-173 (commAllocRecv,
- * commTeardown, COMM_RECV_SKIP, COMM_PACK_DEST, COMM_SEND_FINISH,
- * COMM_RECV_BASE).  The library ships one backend, "device_local", for
- * subdomains that share one GPU's HBM. */
+/* Backend hook table: the run-time form of the compile-time backend API of
+ * LULESH/cuda/src/comm/comm_backend.h:7-22 (dispatch :140-173).  The first
+ * five hooks are required; every later one is optional -- zero-initialise
+ * the struct and leave a hook NULL for the device_local behaviour named in
+ * its comment.  A backend decides HOW message words move; which words, in
+ * which order and to which offsets stays the library's (the reference's
+ * wire layout).  Hooks are called on the host while the library enqueues
+ * an exchange on `stream`; they must only enqueue stream-ordered work (the
+ * stepping engine captures exchanges into HIP graphs).
+ *
+ * The library ships two: "device_local" (remote pack, Mode C of
+ * LULESH/cuda/src/comm/comm_ipc_common.h:177-190: the pack kernel writes
+ * into the receiver's buffer) and "device_copy" (Mode A, :192-225: pack
+ * into the sender's own send buffer, then one device-to-device copy per
+ * message into the receiver's slot). */
 typedef struct lulesh_comm_backend_ops {
     const char *name;
-    /* receive buffer of `comBufSize` doubles for subdomain d */
+    /* commAllocRecv: receive buffer of comBufSize doubles for subdomain d */
     int (*alloc_recv)(lulesh_domain *d, size_t comBufSize, double **recv, void *ctx);
+    /* commTeardown */
     int (*teardown)(lulesh_domain *d, double *recv, void *ctx);
-    /* where the pack kernels of sender `from` write the message to
-     * receiver `to` that lands at `offset` doubles into its buffer */
+    /* COMM_PACK_DEST: where the pack kernel of sender `from` writes the
+     * message to receiver `to` that lands at `offset` doubles into the
+     * receiver's buffer; NULL = into the sender's own send buffer, at its
+     * CommSend slot, for send_msg / send_corner to deliver */
     double *(*pack_dest)(lulesh_domain *from, lulesh_domain *to, size_t offset, void *ctx);
-    /* completion of all sends of `from` (stream-ordered for device_local) */
+    /* COMM_SEND_FINISH: after all sends of `from` */
     int (*send_finish)(lulesh_domain *from, lulesh_stream_t stream, void *ctx);
-    /* buffer the unpack kernels of d read */
+    /* COMM_RECV_BASE: the device buffer the unpack kernels read */
     double *(*recv_base)(lulesh_domain *d, void *ctx);
+
+    /* ---- optional ---- */
+    /* COMM_RUNTIME_INIT / COMM_RUNTIME_SHUTDOWN: when installed on a group
+     * (before alloc_recv) / when the group lets the backend go (replaced or
+     * destroyed).  NULL: nothing. */
+    int (*runtime_init)(void *ctx);
+    int (*runtime_shutdown)(void *ctx);
+    /* COMM_RECV_SKIP: called by CommRecv; a status < 0 fails it.  NULL:
+     * nothing to post (one-sided). */
+    int (*recv_skip)(lulesh_domain *d, int msgType, int xferFields, int doRecv, int planeOnly,
+                     void *ctx);
+    /* COMM_RECV_POST_BUF: where delivered messages land (device or pinned
+     * host memory, comBufSize doubles).  NULL: recv_base. */
+    double *(*recv_post_buf)(lulesh_domain *d, void *ctx);
+    /* COMM_SEND_MSG / COMM_SEND_CORNER: deliver one locally packed face or
+     * edge / corner message: `count` doubles at device address `src` go to
+     * `offset` of the receiver's post buffer.  NULL: one stream-ordered
+     * copy into recv_post_buf(to) + offset. */
+    int (*send_msg)(lulesh_domain *from, lulesh_domain *to, size_t offset, const double *src,
+                    size_t count, lulesh_stream_t stream, void *ctx);
+    int (*send_corner)(lulesh_domain *from, lulesh_domain *to, size_t offset, const double *src,
+                       size_t count, lulesh_stream_t stream, void *ctx);
+    /* COMM_UNPACK_H2D: before a subdomain's unpack, per message, when the
+     * post buffer is not recv_base: copy `bytes` from `src` (post buffer)
+     * to `dst` (recv_base).  NULL: one stream-ordered copy. */
+    int (*unpack_h2d)(lulesh_domain *d, double *dst, const double *src, size_t bytes,
+                      lulesh_stream_t stream, void *ctx);
+    /* COMM_ADD_CORNER / COMM_COPY_CORNER: apply one corner message (one
+     * word per field at device address `src`) to node `node` of d, adding
+     * (CommSBN) or overwriting (CommSyncPosVel); lulesh_comm_apply_corner
+     * is the library's own launch.  Corner messages are the last a corner
+     * node receives, so they are applied after the unpack of the faces and
+     * edges.  NULL: in the unpack kernel. */
+    int (*add_corner)(lulesh_domain *d, int node, const double *src, int xferFields,
+                      const int *fields, lulesh_stream_t stream, void *ctx);
+    int (*copy_corner)(lulesh_domain *d, int node, const double *src, int xferFields,
+                       const int *fields, lulesh_stream_t stream, void *ctx);
+    /* COMM_MONOQ_COPY: one field of one MonoQ message, `bytes` from `src`
+     * (recv_base) into its ghost plane at device address `dst`.  NULL: in
+     * the unpack kernel. */
+    int (*monoq_copy)(lulesh_domain *d, double *dst, const double *src, size_t bytes,
+                      lulesh_stream_t stream, void *ctx);
 } lulesh_comm_backend_ops;
 
 const lulesh_comm_backend_ops *lulesh_device_local_backend(void);
-/* Installs a backend on a group before its first exchange (the set-up
- * exchange of lulesh_group_create uses device_local). */
+const lulesh_comm_backend_ops *lulesh_device_copy_backend(void);
+/* Installs a backend on a group (the set-up exchange of lulesh_group_create
+ * used device_local): runtime_init, then a fresh receive buffer per
+ * subdomain from alloc_recv; the previous backend's buffers are torn down
+ * and its runtime_shutdown called. */
 int lulesh_group_set_comm_backend(lulesh_group *g, const lulesh_comm_backend_ops *ops,
                                   void *ctx);
+/* The library's corner unpack (the reference's AddCorner / CopyCorner
+ * kernels, lulesh-comms-gpu.cu:324-339), for backends that override the
+ * corner hooks: node `node` of d gets (add = 1) or takes (add = 0) the
+ * xferFields words at device address src, field by field. */
+int lulesh_comm_apply_corner(lulesh_domain *d, int node, const double *src, int xferFields,
+                             const int *fields, int add, lulesh_stream_t stream);
 /* One whole exchange step on every subdomain of the group: CommSend on
  * all, then the matching unpack on all, through the installed backend. */
 int lulesh_group_exchange(lulesh_group *g, int msgType, int xferFields,
