@@ -15,6 +15,7 @@
 #include <map>
 #include <unordered_map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -283,6 +284,7 @@ static std::string plan_key(int msgType, int nf, const int *fields, bool doFlag,
 
 DevPlan::~DevPlan()
 {
+    if (!owned) return;
     if (pack) (void)hipFree(pack);
     if (tgt) (void)hipFree(tgt);
     if (src) (void)hipFree(src);
@@ -362,6 +364,7 @@ int group_plan(lulesh_group *g, int msgType, int nf, const int *fields, bool doF
     HIPCHK(hipMalloc((void **)&p->fptr, fp.size() * sizeof(double *)));
     HIPCHK(hipMemcpy(p->fptr, fp.data(), fp.size() * sizeof(double *), hipMemcpyHostToDevice));
     p->host_fptr = fp;
+    for (auto *d : g->doms) p->send_base.push_back(d->send);
     *out = p.get();
     g->plans[key] = std::move(p);
     return LULESH_OK;
@@ -392,7 +395,7 @@ int plan_pack(lulesh_group *g, DevPlan *p, int rank, hipStream_t s, bool gated =
         const MsgRef &m = p->sends[i];
         if (!m.local) continue;
         lulesh_domain *from = g->doms[m.from], *to = g->doms[m.to];
-        const double *src = from->send + m.send_off;
+        const double *src = p->send_base[m.from] + m.send_off;
         const size_t words = msg_words(p, m);
         auto hook = m.kind == 3 ? ops->send_corner : ops->send_msg;
         if (hook) {
@@ -825,6 +828,39 @@ int group_read_ctl(lulesh_group *g, GroupCtl *G, DomCtl *D0, hipStream_t s)
 }  // namespace lulesh
 
 // ===========================================================================
+// scratch of lulesh_group_exchange_noalloc
+// ===========================================================================
+// One file-scope device array: the plan tables and send staging of an
+// allocation-free exchange step.  Each GPU the library runs on holds its
+// own copy until the process ends.  Host code never uses the array's name
+// as an address: its device address is queried once per device (at
+// lulesh_init, else at the first such call on the device) and kept in
+// g_scratch.
+static __device__ __align__(256) unsigned char scratch[LULESH_EXCHANGE_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+// the current device's scratch address (queried on first use)
+static int scratch_base(unsigned char **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return LULESH_EINVAL;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch[dev];
+    return LULESH_OK;
+}
+
+// ===========================================================================
 // C ABI
 // ===========================================================================
 extern "C" {
@@ -846,7 +882,8 @@ int lulesh_init(int device)
         fprintf(stderr, "lulesh_amd: device %d is %s, built for gfx950\n", device, prop.gcnArchName);
         return LULESH_EINVAL;
     }
-    return LULESH_OK;
+    unsigned char *sc = nullptr;   // the scratch address of this device
+    return scratch_base(&sc);
 }
 
 const char *lulesh_status_string(int status)
@@ -936,9 +973,22 @@ int lulesh_group_set_keep_forces(lulesh_group *g, int keep)
     return LULESH_OK;
 }
 
+// drops an engine whose PosVel snapshots an allocation-free exchange made
+// stale (the exchange itself could not free it)
+static void drop_stale_engine(lulesh_group *g)
+{
+    if (g->engine_stale && g->engine) {
+        (void)hipDeviceSynchronize();
+        engine_destroy(g->engine);
+        g->engine = nullptr;
+    }
+    g->engine_stale = false;
+}
+
 int lulesh_group_enqueue(lulesh_group *g, int ncycles, lulesh_stream_t stream)
 {
     if (!g || ncycles < 0) return LULESH_EINVAL;
+    drop_stale_engine(g);
     if (!g->engine) {
         std::vector<lulesh_group *> one{g};
         const int rc = engine_create(one, &g->engine);
@@ -980,6 +1030,7 @@ int lulesh_group_run(lulesh_group *g, int cycles_per_sync, lulesh_stream_t strea
 int lulesh_group_profile(lulesh_group *g, int ncycles, double ms[7])
 {
     if (!g || ncycles < 0 || !ms) return LULESH_EINVAL;
+    drop_stale_engine(g);
     if (!g->engine) {
         std::vector<lulesh_group *> one{g};
         const int rc = engine_create(one, &g->engine);
@@ -1365,6 +1416,117 @@ int lulesh_group_exchange(lulesh_group *g, int msgType, int nf, const int *field
     if (!g || g->umesh || !fields) return LULESH_EINVAL;
     group_state_changed(g);
     return group_exchange(g, msgType, nf, fields, stream_of(g, stream));
+}
+
+// lulesh_group_exchange with its plan tables and send staging in the
+// device scratch array instead of allocations: the same plan (built on the
+// host), the same pack / send / unpack path, the same results.  Scratch
+// layout: the send staging of every subdomain (comBufSize doubles each,
+// 256-byte aligned), then the pack entries, unpack targets, unpack sources
+// and field pointers.  Every scratch word a step reads, the same step
+// writes first, so nothing needs clearing.  Returns after the step
+// completes (the next call may overwrite the scratch).
+int lulesh_group_exchange_noalloc(lulesh_group *g, int msgType, int nf, const int *fields,
+                                  lulesh_stream_t stream)
+{
+    if (!g || g->umesh || !fields || nf < 1 || nf > 6) return LULESH_EINVAL;
+    if (msgType != LULESH_MSG_COMM_SBN && msgType != LULESH_MSG_SYNC_POS_VEL && msgType != LULESH_MSG_MONOQ)
+        return LULESH_EINVAL;
+    if (g->ndom == 1) return LULESH_OK;
+    if (g->comm_down) return LULESH_EINVAL;
+    unsigned char *base = nullptr;
+    int rc = scratch_base(&base);
+    if (rc) return rc;
+    try {
+        const int ndom = g->ndom;
+        const bool doFlag = msgType != LULESH_MSG_SYNC_POS_VEL;
+        const bool planeOnly = msgType == LULESH_MSG_MONOQ;
+        const int mode = msgType == LULESH_MSG_COMM_SBN ? 0 : msgType == LULESH_MSG_SYNC_POS_VEL ? 1 : 2;
+        const lulesh_comm_backend_ops *ops = g->backend;
+        std::vector<double *> fp((size_t)ndom * nf);
+        for (int r = 0; r < ndom; ++r)
+            for (int f = 0; f < nf; ++f) {
+                size_t len;
+                fp[(size_t)r * nf + f] = field_dev_ptr(g->doms[r], fields[f], &len);
+                if (!fp[(size_t)r * nf + f]) return LULESH_EINVAL;
+            }
+        std::vector<double *> stage(ndom);
+        size_t off = 0;
+        for (int r = 0; r < ndom; ++r) {
+            stage[r] = (double *)(base + off);
+            off += align256((size_t)std::max(1, g->doms[r]->host.comBufSize) * sizeof(double));
+        }
+        PlanInput in;
+        in.tp = g->tp;
+        for (auto *d : g->doms) in.doms.push_back(&d->host);
+        in.nfields = nf;
+        in.doFlag = doFlag;
+        in.planeOnly = planeOnly;
+        in.pack_dest = [g](int from, int to, size_t o) {
+            return g->backend->pack_dest(g->doms[from], g->doms[to], o, g->backend_ctx);
+        };
+        in.send_base = [&stage](int from) { return stage[from]; };
+        in.recv_base = [g](int dom) -> const double * {
+            return g->backend->recv_base(g->doms[dom], g->backend_ctx);
+        };
+        in.skip_corners = (mode == 0 && ops->add_corner) || (mode == 1 && ops->copy_corner);
+        in.skip_all = mode == 2 && ops->monoq_copy;
+        HostPlan hp;
+        if (build_plan(in, hp) != 0) return LULESH_EDIRECTION;
+        const size_t o_pack = align256(off);
+        const size_t o_tgt = align256(o_pack + hp.pack.size() * sizeof(PackEnt));
+        const size_t o_src = align256(o_tgt + hp.tgt.size() * sizeof(UnpackTgt));
+        const size_t o_fp = align256(o_src + hp.src.size() * sizeof(UnpackSrc));
+        const size_t end = o_fp + fp.size() * sizeof(double *);
+        if (end > (size_t)LULESH_EXCHANGE_SCRATCH_BYTES) {
+            fprintf(stderr, "lulesh_amd: lulesh_group_exchange_noalloc: this step needs %zu bytes of "
+                            "scratch, the library has %zu\n", end, (size_t)LULESH_EXCHANGE_SCRATCH_BYTES);
+            return LULESH_ENOMEM;
+        }
+        hipStream_t s = stream_of(g, stream);
+        if (!hp.pack.empty())
+            HIPCHK(hipMemcpyAsync(base + o_pack, hp.pack.data(), hp.pack.size() * sizeof(PackEnt),
+                                  hipMemcpyHostToDevice, s));
+        if (!hp.tgt.empty())
+            HIPCHK(hipMemcpyAsync(base + o_tgt, hp.tgt.data(), hp.tgt.size() * sizeof(UnpackTgt),
+                                  hipMemcpyHostToDevice, s));
+        if (!hp.src.empty())
+            HIPCHK(hipMemcpyAsync(base + o_src, hp.src.data(), hp.src.size() * sizeof(UnpackSrc),
+                                  hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(base + o_fp, fp.data(), fp.size() * sizeof(double *), hipMemcpyHostToDevice, s));
+        DevPlan p;
+        p.owned = false;
+        p.msgType = msgType;
+        p.nfields = nf;
+        p.mode = mode;
+        p.pack = (PackEnt *)(base + o_pack);
+        p.tgt = (UnpackTgt *)(base + o_tgt);
+        p.src = (UnpackSrc *)(base + o_src);
+        p.fptr = (double **)(base + o_fp);
+        p.npack = (int)hp.pack.size();
+        p.ntgt = (int)hp.tgt.size();
+        p.pack_start = hp.pack_start;
+        p.tgt_start = hp.tgt_start;
+        p.sends = hp.sends;
+        p.recvs = hp.recvs;
+        p.send_start = hp.send_start;
+        p.recv_start = hp.recv_start;
+        p.fields.assign(fields, fields + nf);
+        p.host_fptr = fp;
+        p.send_base = stage;
+        p.corner_hook = in.skip_corners;
+        p.monoq_hook = in.skip_all;
+        rc = plan_pack(g, &p, -1, s);
+        for (int r = 0; r < ndom && rc == LULESH_OK; ++r) rc = ops->send_finish(g->doms[r], s, g->backend_ctx);
+        if (rc == LULESH_OK) rc = plan_unpack(g, &p, -1, s);
+        // drained before the host tables go and before the next call may
+        // overwrite the scratch
+        HIPCHK(hipStreamSynchronize(s));
+        g->engine_stale = true;   // node state changed behind a decomposed group's engine
+        return rc;
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
 }
 
 // element-wise probes of the device arithmetic (parity tests)

@@ -49,7 +49,9 @@ struct DevPlan {
     std::vector<int> send_start, recv_start;
     std::vector<int> fields;
     std::vector<double *> host_fptr;   // [rank * nfields + f]
+    std::vector<double *> send_base;   // per rank: where its local packs are staged
     bool corner_hook = false, monoq_hook = false;
+    bool owned = true;                 // pack/tgt/src/fptr allocated for this plan
     ~DevPlan();
 };
 }  // namespace lulesh
@@ -84,6 +86,9 @@ struct lulesh_group {
     std::map<std::string, std::unique_ptr<lulesh::DevPlan>> plans;
     std::unique_ptr<lulesh::DirectTables> direct;   // built on first use
     lulesh::Engine *engine = nullptr;   // standalone stepping
+    // the node state changed behind the engine's back (PosVel snapshots
+    // stale) where it could not be freed at once: rebuilt at the next step
+    bool engine_stale = false;
     hipStream_t stream = nullptr;
 };
 

@@ -84,6 +84,7 @@ SIGNATURES = {
     "lulesh_comm_apply_corner": (_i, [_vp, _i, _dp, _i, _ip, _i, _vp]),
     "lulesh_group_set_comm_backend": (_i, [_vp, _vp, _vp]),
     "lulesh_group_exchange": (_i, [_vp, _i, _i, _ip, _vp]),
+    "lulesh_group_exchange_noalloc": (_i, [_vp, _i, _i, _ip, _vp]),
     "lulesh_batch_create": (_i, [_i, _ip, _ip, _ip, _ip, _pvp]),
     "lulesh_batch_destroy": (_i, [_vp]),
     "lulesh_batch_set_max_cycles": (_i, [_vp, _i]),
@@ -423,6 +424,11 @@ class Group:
     def exchange(self, msg_type, fields):
         ids = np.array([FIELDS[f] for f in fields], dtype=np.int32)
         return load().lulesh_group_exchange(self.h, msg_type, len(ids), _p(ids, ctypes.c_int), None)
+
+    def exchange_noalloc(self, msg_type, fields):
+        """The same exchange step without allocating (scratch array)."""
+        ids = np.array([FIELDS[f] for f in fields], dtype=np.int32)
+        return load().lulesh_group_exchange_noalloc(self.h, msg_type, len(ids), _p(ids, ctypes.c_int), None)
 
     def set_comm_backend(self, name):
         """Installs one of the library's comm backends: "device_local"

@@ -374,6 +374,23 @@ int lulesh_comm_apply_corner(lulesh_domain *d, int node, const double *src, int 
 int lulesh_group_exchange(lulesh_group *g, int msgType, int xferFields,
                           const int *fields, lulesh_stream_t stream);
 
+/* Bytes of the device scratch array lulesh_group_exchange_noalloc uses,
+ * one per GPU, held until the process ends. */
+#define LULESH_EXCHANGE_SCRATCH_BYTES (32u << 20)
+/* lulesh_group_exchange without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (tight
+ * serving loops, drivers that forbid hipMalloc).  Same step, same backend
+ * hooks, same results as lulesh_group_exchange, with the step's plan
+ * tables and send staging in the library's scratch array instead of
+ * allocations (lulesh_group_exchange allocates them on a step's first
+ * use): calls no hipMalloc or hipFree.  Returns after the step has
+ * completed on the GPU.  LULESH_ENOMEM when a step needs more than
+ * LULESH_EXCHANGE_SCRATCH_BYTES (8 subdomains of -s 45 need about 4 MB
+ * for the six PosVel fields).  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (from several threads) race. */
+int lulesh_group_exchange_noalloc(lulesh_group *g, int msgType, int xferFields,
+                                  const int *fields, lulesh_stream_t stream);
+
 /* ------------------------------------------------------------------ */
 /* batched ensemble                                                    */
 /* ------------------------------------------------------------------ */
