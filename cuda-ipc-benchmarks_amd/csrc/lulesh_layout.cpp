@@ -1,0 +1,58 @@
+// lulesh_layout.cpp -- host-only layout queries of the C ABI: the receive
+// offsets, receive-buffer sizes and region layouts of a decomposition,
+// without a group or a GPU (the same host code the groups use).
+#include <limits.h>
+#include <string.h>
+
+#include <new>
+
+#include "lulesh_amd.h"
+#include "lulesh_comm.h"
+#include "lulesh_setup.h"
+
+using namespace lulesh;
+
+extern "C" {
+
+// a decomposition described by (nx, tp, rank)
+static bool decomp_ok(int nx, int tp, int rank)
+{
+    return nx >= 1 && tp >= 1 && tp <= 4 && rank >= 0 && rank < tp * tp * tp &&
+           (long long)nx * nx * nx * 8 <= INT_MAX;
+}
+
+int lulesh_shm_recv_offset(int nx, int tp, int recvRank, int dCol, int dRow, int dPlane,
+                           int xferFields, int doRecv, int planeOnly)
+{
+    if (!decomp_ok(nx, tp, recvRank) || xferFields < 1 || xferFields > 6) return LULESH_EINVAL;
+    HostDomain h;
+    host_decomp(h, nx, tp, recvRank);
+    return recv_offset(h, dCol, dRow, dPlane, xferFields, doRecv != 0, planeOnly != 0);
+}
+
+int lulesh_comm_buf_size_of(int nx, int tp, int rank)
+{
+    if (!decomp_ok(nx, tp, rank)) return LULESH_EINVAL;
+    HostDomain h;
+    host_decomp(h, nx, tp, rank);
+    return h.comBufSize;
+}
+
+int lulesh_region_layout(int nx, int tp, int rank, int nr, int balance, int cost, int *region,
+                         int *rep, size_t n)
+{
+    if (!decomp_ok(nx, tp, rank) || !problem_ok(nx, tp, nr, balance, cost)) return LULESH_EINVAL;
+    if (n < (size_t)nx * nx * nx) return LULESH_EINVAL;
+    try {
+        HostDomain h;
+        host_decomp(h, nx, tp, rank);
+        host_regions(h, nr, balance, cost);
+        if (region) memcpy(region, h.region.data(), h.region.size() * sizeof(int));
+        if (rep) memcpy(rep, h.elemRep.data(), h.elemRep.size() * sizeof(int));
+        return h.numElem;
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
+}
+
+}  // extern "C"
