@@ -581,9 +581,19 @@ using namespace lulesh;
 // ===========================================================================
 struct lulesh_batch {
     std::vector<lulesh_group *> groups;
-    Engine *engine = nullptr;
+    Engine *engine = nullptr;           // steps engine_groups
+    std::vector<lulesh_group *> engine_groups;
+    long long engine_zones = 0;         // elements of engine_groups
+    int tile_e = 0, tile_n = 0;         // the caller's tile shapes (0: default)
     hipStream_t stream = nullptr;
 };
+
+static long long zones_of(const lulesh_group *g)
+{
+    long long z = 0;
+    for (auto *d : g->doms) z += d->host.numElem;
+    return z;
+}
 
 extern "C" {
 
@@ -608,6 +618,8 @@ static int batch_create(int nprob, const int *nx, const int *nr, const int *bala
     }
     const int rc = engine_create(b->groups, &b->engine);
     if (rc) { lulesh_batch_destroy(b); return rc; }
+    b->engine_groups = b->groups;
+    for (auto *g : b->groups) b->engine_zones += zones_of(g);
     *out = b;
     return LULESH_OK;
 }
@@ -647,7 +659,12 @@ int lulesh_batch_set_max_cycles(lulesh_batch *b, int m)
 int lulesh_batch_set_tiles(lulesh_batch *b, int tile_e, int tile_n)
 {
     if (!b) return LULESH_EINVAL;
-    return engine_set_tiles(b->engine, tile_e, tile_n);
+    const int rc = engine_set_tiles(b->engine, tile_e, tile_n);
+    if (rc == LULESH_OK) {
+        if (tile_e > 0) b->tile_e = tile_e;
+        if (tile_n > 0) b->tile_n = tile_n;
+    }
+    return rc;
 }
 
 int lulesh_batch_group(lulesh_batch *b, int i, lulesh_group **out)
@@ -657,18 +674,62 @@ int lulesh_batch_group(lulesh_batch *b, int i, lulesh_group **out)
     return LULESH_OK;
 }
 
-static int batch_poll(lulesh_batch *b, bool *all_done, int *first_err)
+// the members still running, and the first error any member raised
+static int batch_poll(lulesh_batch *b, std::vector<lulesh_group *> &running, int *first_err)
 {
     // every group's GroupCtl sits at the head of its own allocation
-    *all_done = true;
+    running.clear();
     *first_err = 0;
     HIPCHK(hipStreamSynchronize(b->stream));
     for (auto *g : b->groups) {
         GroupCtl G;
         HIPCHK(hipMemcpy(&G, g->gctl, sizeof(G), hipMemcpyDeviceToHost));
         if (G.status && !*first_err) *first_err = G.status;
-        if (!G.done && !G.status) *all_done = false;
+        if (!G.done && !G.status) running.push_back(g);
     }
+    return LULESH_OK;
+}
+
+// an engine over `members` (z elements) in place of the current one
+static int batch_rebuild(lulesh_batch *b, const std::vector<lulesh_group *> &members, long long z)
+{
+    Engine *e = nullptr;
+    int rc = engine_create(members, &e);
+    if (rc == LULESH_OK && (b->tile_e || b->tile_n)) rc = engine_set_tiles(e, b->tile_e, b->tile_n);
+    if (rc) {
+        if (e) engine_destroy(e);
+        return rc;
+    }
+    engine_destroy(b->engine);
+    b->engine = e;
+    b->engine_groups = members;
+    b->engine_zones = z;
+    return LULESH_OK;
+}
+
+// Keeps the chip full as members finish: once the running members hold
+// under 5/8 of the elements the engine steps, the engine is rebuilt over
+// them alone, so its column tiles are re-chunked for the smaller set (more,
+// shorter tiles) and finished members' tiles are no longer launched.
+// Results do not depend on the tiling.
+static int batch_shrink(lulesh_batch *b, const std::vector<lulesh_group *> &running)
+{
+    long long z = 0;
+    for (auto *g : running) z += zones_of(g);
+    if (running.empty() || z * 8 >= b->engine_zones * 5) return LULESH_OK;
+    return batch_rebuild(b, running, z);
+}
+
+// a member re-armed after the engine dropped it (a raised cycle cap or stop
+// time): the engine covers every running member again
+static int batch_cover(lulesh_batch *b, const std::vector<lulesh_group *> &running)
+{
+    for (auto *g : running)
+        if (std::find(b->engine_groups.begin(), b->engine_groups.end(), g) == b->engine_groups.end()) {
+            long long z = 0;
+            for (auto *h : running) z += zones_of(h);
+            return batch_rebuild(b, running, z);
+        }
     return LULESH_OK;
 }
 
@@ -677,14 +738,21 @@ int lulesh_batch_run(lulesh_batch *b, int cycles_per_sync, lulesh_stream_t strea
     if (!b) return LULESH_EINVAL;
     (void)stream;
     if (cycles_per_sync <= 0) cycles_per_sync = 32;
+    std::vector<lulesh_group *> running;
+    int err0;
+    int rc0 = batch_poll(b, running, &err0);
+    if (rc0 == LULESH_OK) rc0 = batch_cover(b, running);
+    if (rc0) return rc0;
+    if (running.empty()) return err0;
     for (;;) {
         int rc = engine_enqueue(b->engine, cycles_per_sync, b->stream);
         if (rc) return rc;
-        bool done;
         int err;
-        rc = batch_poll(b, &done, &err);
+        rc = batch_poll(b, running, &err);
         if (rc) return rc;
-        if (done) return err;
+        if (running.empty()) return err;
+        rc = batch_shrink(b, running);
+        if (rc) return rc;
     }
 }
 
@@ -753,6 +821,8 @@ int lulesh_batch_autotune(lulesh_batch *b, int probe_cycles, int *best_tile_e, i
     (void)hipEventDestroy(e1);
     int rc = engine_set_tiles(b->engine, be, bn);
     if (!rc) rc = engine_set_kchunk(b->engine, bk);
+    b->tile_e = be;
+    b->tile_n = bn;
     if (best_tile_e) *best_tile_e = be;
     if (best_tile_n) *best_tile_n = bn;
     if (best_kchunk) *best_kchunk = bk ? bk : b->engine->kchunk_used;

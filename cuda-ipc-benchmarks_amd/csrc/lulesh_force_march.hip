@@ -35,7 +35,10 @@ namespace lulesh {
 __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, const ColTile *tiles,
                                                         int ntiles, int kchunk, int advance, int store_f)
 {
-    __shared__ double lds[24][kMThreads];       // corner forces of one phase
+    // corner forces of one phase, in pairs: lds[d][p][t] = component d of
+    // element t's corners p and p+4 (the node-plane-c and node-plane-c+1
+    // corners one node gathers from the same element), one 16-byte access
+    __shared__ double2 lds[3][4][kMThreads];
     __shared__ double sn[2][6][kPatch];          // node planes (x y z xd yd zd), by plane parity
     const ColTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     // the tile's subdomain is uniform over the workgroup: one scalar copy of
@@ -156,14 +159,16 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             double bx[8], by[8], bz[8];
             node_normals16(x, y, z, bx, by, bz);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                lds[0 * 8 + q][tid] = -(sig16 * bx[q]);
-                lds[1 * 8 + q][tid] = -(sig16 * by[q]);
-                lds[2 * 8 + q][tid] = -(sig16 * bz[q]);
+            for (int q = 0; q < 4; ++q) {
+                lds[0][q][tid] = make_double2(-(sig16 * bx[q]), -(sig16 * bx[q + 4]));
+                lds[1][q][tid] = make_double2(-(sig16 * by[q]), -(sig16 * by[q + 4]));
+                lds[2][q][tid] = make_double2(-(sig16 * bz[q]), -(sig16 * bz[q + 4]));
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < 24; ++q) lds[q][tid] = 0.0;
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) lds[d][q][tid] = make_double2(0.0, 0.0);
         }
         __syncthreads();
         // the gathers between barriers are short and latency-bound: they
@@ -173,20 +178,21 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         if (node_ok) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
+                const double2 p2 = lds[d][2][tid], p3 = lds[d][3][t1], p1 = lds[d][1][t2], p0 = lds[d][0][t3];
                 if (c >= k0) {
                     double v = ps[d];
-                    v += lds[d * 8 + 2][tid];
-                    v += lds[d * 8 + 3][t1];
-                    v += lds[d * 8 + 1][t2];
-                    v += lds[d * 8 + 0][t3];
+                    v += p2.x;
+                    v += p3.x;
+                    v += p1.x;
+                    v += p0.x;
                     fs[d] = v;
                 }
                 if (c + 1 < k1) {
                     double v = 0.0;
-                    v += lds[d * 8 + 6][tid];
-                    v += lds[d * 8 + 7][t1];
-                    v += lds[d * 8 + 5][t2];
-                    v += lds[d * 8 + 4][t3];
+                    v += p2.y;
+                    v += p3.y;
+                    v += p1.y;
+                    v += p0.y;
                     ps[d] = v;
                 }
             }
@@ -217,11 +223,13 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                 }
                 hg_force(H, vl, h);
 #pragma unroll
-                for (int q = 0; q < 8; ++q) lds[d * 8 + q][tid] = h[q];
+                for (int q = 0; q < 4; ++q) lds[d][q][tid] = make_double2(h[q], h[q + 4]);
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < 24; ++q) lds[q][tid] = 0.0;
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) lds[d][q][tid] = make_double2(0.0, 0.0);
         }
         __syncthreads();
         __builtin_amdgcn_s_setprio(2);
@@ -229,20 +237,21 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
             double f[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
+                const double2 p2 = lds[d][2][tid], p3 = lds[d][3][t1], p1 = lds[d][1][t2], p0 = lds[d][0][t3];
                 if (c >= k0) {
                     double v = ph[d];
-                    v += lds[d * 8 + 2][tid];
-                    v += lds[d * 8 + 3][t1];
-                    v += lds[d * 8 + 1][t2];
-                    v += lds[d * 8 + 0][t3];
+                    v += p2.x;
+                    v += p3.x;
+                    v += p1.x;
+                    v += p0.x;
                     f[d] = fs[d] + v;
                 }
                 if (c + 1 < k1) {
                     double v = 0.0;
-                    v += lds[d * 8 + 6][tid];
-                    v += lds[d * 8 + 7][t1];
-                    v += lds[d * 8 + 5][t2];
-                    v += lds[d * 8 + 4][t3];
+                    v += p2.y;
+                    v += p3.y;
+                    v += p1.y;
+                    v += p0.y;
                     ph[d] = v;
                 }
             }
