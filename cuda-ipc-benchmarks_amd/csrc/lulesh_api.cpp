@@ -759,6 +759,9 @@ int group_reset_ctl(lulesh_group *g)
         D[r].dtcourant_bits = bb;
         D[r].dthydro_bits = bb;
         D[r].cycle = 0;
+        D[r].ntime = D[r].time;
+        D[r].ndeltatime = D[r].deltatime;
+        D[r].ncycle = 0;
     }
     HIPCHK(hipMemcpy(g->ctl_mem, buf.data(), buf.size(), hipMemcpyHostToDevice));
     return LULESH_OK;

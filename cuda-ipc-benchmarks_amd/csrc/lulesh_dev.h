@@ -118,12 +118,21 @@ struct DomDev {
     struct GroupCtl *grp;    // its group's loop state
 };
 
-// per-subdomain time state (every rank of the reference keeps its own)
+// per-subdomain time state (every rank of the reference keeps its own).
+// time, deltatime and cycle are the state after the last completed cycle;
+// ntime, ndeltatime and ncycle the running cycle's, published by the
+// cycle's first kernel (its TimeIncrement) and copied into the former by
+// the kinematics kernel, so the first kernel of a cycle can compute the
+// increment in every workgroup from state nothing in its launch writes.
+// Kernels between the two read the n-fields.
 struct DomCtl {
     double time, deltatime;
     unsigned long long dtcourant_bits, dthydro_bits;   // atomicMin slots
     int cycle;
     int pad;
+    double ntime, ndeltatime;
+    int ncycle;
+    int npad;
 };
 
 // one allocation per group: GroupCtl followed by its ndom DomCtl
@@ -148,6 +157,12 @@ struct Tile {
     int ecount;
     int pad;
 };
+
+// Tile::pad of an element tile that is its subdomain's first (the one
+// that completes the running cycle's time state in the kinematics kernel),
+// and ColTile::pad[0] of a subdomain's first column tile (the one that
+// publishes the cycle's time increment in the force kernel)
+constexpr int kFirstTile = 2;
 
 // EOS evaluation table entry: element index mod 2^15 in the low 15 bits,
 // bit 15 set on the element's last repetition

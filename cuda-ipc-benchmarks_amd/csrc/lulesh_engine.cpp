@@ -152,7 +152,8 @@ static int build_tiles(Engine *e)
             for (int k0 = 0; k0 < s1; k0 += kc)
                 for (int j0 = 0; j0 < s1; j0 += ey - 1)
                     for (int i0 = 0; i0 < s1; i0 += ex - 1)
-                        ct.push_back(ColTile{di, i0, j0, k0, ex, ey, {0, 0}});
+                        ct.push_back(ColTile{di, i0, j0, k0, ex, ey,
+                                             {(k0 | j0 | i0) == 0 ? kFirstTile : 0, 0}});
             ++di;
         }
     e->kchunk_used = kc;
@@ -167,7 +168,7 @@ static int build_tiles(Engine *e)
             for (int s = 0; s < h.numElem; s += e->tile_e) {
                 const int end = std::min(s + e->tile_e, h.numElem);
                 const long long a = h.evprefix64[s / 64], b = h.evprefix64[(end + 63) / 64];
-                et.push_back(Tile{di, s, a, (int)(b - a), 0});
+                et.push_back(Tile{di, s, a, (int)(b - a), s == 0 ? kFirstTile : 0});
             }
             for (int s = 0; s < h.numNode; s += e->tile_n) nt.push_back(Tile{di, s, 0, 0, 0});
             ++di;
@@ -261,6 +262,8 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
 {
     Engine *e = new Engine();
     e->groups = groups;
+    const char *nti = getenv("LULESH_NO_FUSED_TI");
+    e->fused_ti = !(nti && nti[0] == '1');
     const char *ng = getenv("LULESH_NO_GRAPH");
     e->use_graph = !(ng && ng[0] == '1');
     const char *fp = getenv("LULESH_FORCE_PATH");
@@ -465,11 +468,14 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
 {
     const int ng = (int)e->groups.size();
     int rc = LULESH_OK;
-    STEP(0, HIPCHK(launch_time_increment(e->d_groups, ng, e->d_doms, e->d_dom0, s)));
+    // the force march computes the cycle's time increment itself; the
+    // two-pass (unstructured) path has a launch of its own for it
+    if (!e->march || !e->fused_ti)
+        STEP(0, HIPCHK(launch_time_increment(e->d_groups, ng, e->d_doms, e->d_dom0, s)));
     bool updated;   // node update done inside the force step
     if (e->march) {
         STEP(1, HIPCHK(launch_force_march(e->d_doms, e->d_ctiles, e->n_ctiles, e->kchunk_used,
-                                          e->fuse_update, e->store_forces, s)));
+                                          e->fuse_update, e->store_forces, e->fused_ti, s)));
         updated = e->fuse_update;
     } else {
         STEP(1, HIPCHK(launch_elem_force(e->umesh, e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));

@@ -121,6 +121,10 @@ struct Engine {
     // (every member double-buffers its node state, no halo exchange sits
     // between force and update); LULESH_NO_FUSED_UPDATE=1 keeps them apart
     bool fuse_update = true;
+    // the force march computes each cycle's time increment in every
+    // workgroup (no time-increment launch); LULESH_NO_FUSED_TI=1 launches
+    // k_time_increment first instead
+    bool fused_ti = true;
     bool store_forces = true;   // fx..fz written (exchange or keep_forces)
     // halo groups on the device_local backend: CommSBN summed inside the
     // node update, CommSyncPosVel written by each shared node's final

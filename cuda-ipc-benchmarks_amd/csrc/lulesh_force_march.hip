@@ -32,22 +32,36 @@ namespace lulesh {
 // Elements outside the mesh contribute +0.0, which leaves a sum that
 // started from +0.0 unchanged.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, const ColTile *tiles,
-                                                        int ntiles, int kchunk, int advance, int store_f)
+__global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColTile *tiles, int ntiles,
+                                                        int kchunk, int advance, int store_f, int ti)
 {
     // corner forces of one phase, in pairs: lds[d][p][t] = component d of
     // element t's corners p and p+4 (the node-plane-c and node-plane-c+1
     // corners one node gathers from the same element), one 16-byte access
     __shared__ double2 lds[3][4][kMThreads];
     __shared__ double sn[2][6][kPatch];          // node planes (x y z xd yd zd), by plane parity
+    __shared__ double ti_dt;                     // this cycle's deltatime, or -1: the group stops
     const ColTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
     // the tile's subdomain is uniform over the workgroup: one scalar copy of
     // its descriptor, which no store in the kernel can be taken to alias
-    const DomDev D = doms[__builtin_amdgcn_readfirstlane(t.dom)];
+    const int di = __builtin_amdgcn_readfirstlane(t.dom);
+    const DomDev D = doms[di];
     GroupCtl *G = D.grp;
-    // the stop flag is tested once the first plane's loads are in flight
-    // (reads only: a stopped group's arrays are still allocated)
-    const bool stopped = group_stopped(*G);
+    // The stop flag is tested once the first plane's loads are in flight
+    // (reads only: a stopped group's arrays are still allocated), by wave 0
+    // for the whole workgroup: the flag can change while the workgroup's
+    // waves read it (the time increment below raises it in one workgroup
+    // of the launch, and a volume error in any), and a wave that left
+    // alone would leave its slots of the node patch unstaged for the rest.
+    // Node-state buffers of the cycle about to run (cycle n = completed + 1
+    // reads buffer (n-1)&1 and writes buffer n&1), from the completed
+    // cycle count, which nothing in this launch writes.  With ti = 0 a
+    // time-increment kernel has counted the cycle already.
+    DomCtl *const Cg = reinterpret_cast<DomCtl *>(G + 1);   // the group's DomCtl array
+    const int rank = (int)(D.ctl - Cg);
+    const int done_cycles = ti ? *gp(&D.ctl->cycle) : *gp(&D.ctl->cycle) - 1;
+    const NodeState S = (done_cycles & 1) ? D.ns[1] : D.ns[0];
+    const NodeState Snew = (done_cycles & 1) ? D.ns[0] : D.ns[1];
     const int nx = D.nx, s1 = nx + 1, s2 = s1 * s1;
     const int tid = threadIdx.x;
     const int ex = t.ex, ey = t.ey, px = ex + 1;
@@ -62,8 +76,6 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
     // lower / upper corner slots seen from the node thread:
     // own element, +x, +y, +x+y neighbours
     const int t1 = tid + 1, t2 = tid + ex, t3 = tid + ex + 1;
-    const NodeState S = ns_old(D);
-    const double dt = D.ctl->deltatime;
 
     // Node patch: slot (pi, pj) holds node (i0-1+pi, j0-1+pj) of a node
     // plane.  The element's corners 0..3 are slots e0, e0+1, e0+px+1,
@@ -123,11 +135,62 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
         fetch(k0 - 1, pf_own, pf_xt);
         fetch(k0, own1, xt1);
         fetch_elem(k0 - 1, ef);
-        if (stopped) return;
+        if (tid < 64) {
+            // this cycle's deltatime, or -1 where the workgroup does nothing
+            const bool stopped = __builtin_amdgcn_readfirstlane((int)group_stopped(*G)) != 0;
+            if (stopped) {
+                if (tid == 0) ti_dt = -1.0;
+            } else if (!ti) {
+                if (tid == 0) ti_dt = *gp(&D.ctl->ndeltatime);
+            } else {
+                // TimeIncrement (lulesh.cc:207-263) of this cycle, computed by
+                // every workgroup of the group from the completed cycle's state:
+                // lane l = subdomain l (at most 4^3 = 64), the group's MIN of
+                // their candidates, then this subdomain's clamps
+                const int ndom = *gp(&G->ndom);
+                const double stoptime = *gp(&G->stoptime);
+                const int max_cycles = *gp(&G->max_cycles);
+                const bool mine = tid < ndom;
+                const DomCtl &Cl = Cg[mine ? tid : 0];
+                double newdt = mine ? ti_candidate(*gp(&Cl.dtcourant_bits), *gp(&Cl.dthydro_bits))
+                                    : __builtin_inf();
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double o = __shfl_xor(newdt, off, 64);
+                    newdt = o < newdt ? o : newdt;
+                }
+                if (tid == 0) {
+                    const double time = *gp(&D.ctl->time), olddt = *gp(&D.ctl->deltatime);
+                    const int cyc = *gp(&D.ctl->cycle);
+                    const bool stop = ti_stop(*gp(&Cg[0].time), *gp(&Cg[0].cycle), stoptime, max_cycles);
+                    const double deltatime = stop ? -1.0 : ti_deltatime(newdt, olddt, time, cyc, stoptime);
+                    ti_dt = deltatime;
+                    if (t.pad[0] == kFirstTile) {
+                        // the subdomain's first tile publishes the running
+                        // cycle for the cycle's later kernels (no workgroup
+                        // of this launch reads the n-fields or the
+                        // descriptors' old/cur; the stop flag is read by
+                        // each workgroup's wave 0 only, above)
+                        if (stop) {
+                            if (rank == 0) G->done = 1;
+                        } else {
+                            DomCtl &C = *D.ctl;
+                            C.ndeltatime = deltatime;
+                            C.ntime = time + deltatime;
+                            C.ncycle = cyc + 1;
+                            doms[di].old = S;
+                            doms[di].cur = Snew;
+                        }
+                    }
+                }
+            }
+        }
         stage(k0 - 1, pf_own, pf_xt);
         stage(k0, own1, xt1);
     }
     __syncthreads();
+    const double dt = ti_dt;
+    if (dt < 0.0) return;   // the group has stopped, or reaches its stop condition now
 
     // a non-positive volume (lulesh.cc:1263-1265, 1340-1349) is reported
     // once, after the planes: no atomic sits on the loop's path
@@ -277,7 +340,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
                     if (fabs(vy) < Phys::u_cut) vy = 0.0;
                     double vz = Lo[5][own_sl] + az * dt;
                     if (fabs(vz) < Phys::u_cut) vz = 0.0;
-                    const NodeState N = ns_new(D);
+                    const NodeState N = Snew;
                     gst(N.xd, nd, vx);
                     gst(N.yd, nd, vy);
                     gst(N.zd, nd, vz);
@@ -299,12 +362,12 @@ __global__ __launch_bounds__(256, 2) void k_force_march(const DomDev *doms, cons
 }
 
 
-hipError_t launch_force_march(const DomDev *d, const ColTile *t, int nt, int kchunk, bool advance,
-                              bool store_f, hipStream_t s)
+hipError_t launch_force_march(DomDev *d, const ColTile *t, int nt, int kchunk, bool advance, bool store_f,
+                              bool ti, hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
     hipLaunchKernelGGL(k_force_march, dim3(nt), dim3(kMThreads), 0, s, d, t, nt, kchunk, advance ? 1 : 0,
-                       store_f ? 1 : 0);
+                       store_f ? 1 : 0, ti ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -40,6 +40,48 @@ __device__ __forceinline__ unsigned long long dbits(double v)
     return (unsigned long long)__double_as_longlong(v);
 }
 
+// TimeIncrement, lulesh.cc:207-263.  A subdomain's candidate from its
+// Courant and hydro minima of the last cycle (before the group's MIN, the
+// reference's MPI_Allreduce, lulesh.cc:227-236)...
+__device__ __forceinline__ double ti_candidate(unsigned long long dcb, unsigned long long dhb)
+{
+    double gnewdt = 1.0e+20;
+    const double dc = __longlong_as_double((long long)dcb);
+    const double dh = __longlong_as_double((long long)dhb);
+    if (dc < gnewdt) gnewdt = dc / 2.0;
+    if (dh < gnewdt) gnewdt = dh * 2.0 / 3.0;
+    return gnewdt;
+}
+
+// ... and, given the group's MIN newdt, a subdomain's new deltatime (ratio
+// clamps against its own olddt, dtmax, the approach to the stop time)
+__device__ __forceinline__ double ti_deltatime(double newdt, double olddt, double time, int cyc,
+                                               double stoptime)
+{
+    double deltatime = olddt;
+    double targetdt = stoptime - time;
+    if ((Phys::dtfixed <= 0.0) && (cyc != 0)) {
+        double dt = newdt;
+        const double ratio = dt / olddt;
+        if (ratio >= 1.0) {
+            if (ratio < Phys::deltatimemultlb) dt = olddt;
+            else if (ratio > Phys::deltatimemultub) dt = olddt * Phys::deltatimemultub;
+        }
+        if (dt > Phys::dtmax) dt = Phys::dtmax;
+        deltatime = dt;
+    }
+    if ((targetdt > deltatime) && (targetdt < (4.0 * deltatime / 3.0)))
+        targetdt = 2.0 * deltatime / 3.0;
+    if (targetdt < deltatime) deltatime = targetdt;
+    return deltatime;
+}
+
+// main's loop test (lulesh.cc:3222) on the group's subdomain 0
+__device__ __forceinline__ bool ti_stop(double time, int cyc, double stoptime, int max_cycles)
+{
+    return !(time < stoptime) || (max_cycles >= 0 && cyc >= max_cycles);
+}
+
 constexpr int kMThreads = 256;   // threads (element slots) per column tile
 constexpr int kPatch = kForcePatch;   // node slots per plane: (ex+1)*(ey+1)
 

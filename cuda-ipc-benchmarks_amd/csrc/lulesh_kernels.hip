@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_time_increment(GroupCtl *const *groups,
                      "v"(b0[4]), "v"(b0[5]), "v"(b1[0]), "v"(b1[1]), "v"(b1[2]), "v"(b1[3]), "v"(b1[4]),
                      "v"(b1[5]));
     if (stop != 0) return;
-    if (!(time < stoptime) || (max_cycles >= 0 && cyc >= max_cycles)) {
+    if (ti_stop(time, cyc, stoptime, max_cycles)) {
         if (r == 0) Gp->done = 1;
         return;
     }
@@ -97,43 +97,25 @@ __global__ __launch_bounds__(256) void k_time_increment(GroupCtl *const *groups,
         ns_unpack(&doms[d0 + r].ns[0], b0);
         ns_unpack(&doms[d0 + r].ns[1], b1);
     }
-    // each subdomain's candidate, then the MIN over the group (the
-    // reference's MPI_Allreduce(MIN), lulesh.cc:227-236; MIN is exact, so
-    // any reduction order gives the sequential result)
-    double gnewdt = 1.0e+20;
-    if (mine) {
-        const double dc = __longlong_as_double((long long)dcb);
-        const double dh = __longlong_as_double((long long)dhb);
-        if (dc < gnewdt) gnewdt = dc / 2.0;
-        if (dh < gnewdt) gnewdt = dh * 2.0 / 3.0;
-    }
-    double newdt = mine ? gnewdt : __builtin_inf();
+    // each subdomain's candidate, then the MIN over the group (MIN is
+    // exact, so any reduction order gives the sequential result)
+    double newdt = mine ? ti_candidate(dcb, dhb) : __builtin_inf();
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const double o = __shfl_xor(newdt, off, 64);
         newdt = o < newdt ? o : newdt;
     }
     if (!mine) return;
-    double deltatime = olddt;
-    double targetdt = stoptime - time;
-    if ((Phys::dtfixed <= 0.0) && (cyc != 0)) {
-        double dt = newdt;
-        const double ratio = dt / olddt;
-        if (ratio >= 1.0) {
-            if (ratio < Phys::deltatimemultlb) dt = olddt;
-            else if (ratio > Phys::deltatimemultub) dt = olddt * Phys::deltatimemultub;
-        }
-        if (dt > Phys::dtmax) dt = Phys::dtmax;
-        deltatime = dt;
-    }
-    if ((targetdt > deltatime) && (targetdt < (4.0 * deltatime / 3.0)))
-        targetdt = 2.0 * deltatime / 3.0;
-    if (targetdt < deltatime) deltatime = targetdt;
+    const double deltatime = ti_deltatime(newdt, olddt, time, cyc, stoptime);
     const int cycle = cyc + 1;
     DomCtl &d = D[r];
     d.deltatime = deltatime;
     d.time = time + deltatime;
     d.cycle = cycle;
+    // the running cycle's state, for the kernels that read the n-fields
+    d.ndeltatime = deltatime;
+    d.ntime = time + deltatime;
+    d.ncycle = cycle;
     // CalcTimeConstraintsForElems restarts from 1e20 (lulesh.cc:3019-3020)
     d.dtcourant_bits = dbits(1.0e+20);
     d.dthydro_bits = dbits(1.0e+20);
@@ -382,7 +364,7 @@ __device__ __forceinline__ void node_gather_u(const DomDev &D, int nd)
     const double fx = sx + hx, fy = sy + hy, fz = sz + hz;
     if (FUSE_UPDATE) {
         node_advance(D, ns_old(D), ns_new(D), nd, node_symm<true>(D, nd, 0, 0, 0), fx, fy, fz,
-                     D.ctl->deltatime);
+                     D.ctl->ndeltatime);
     } else {
         gp(D.fpx)[nd] = fx;
         gp(D.fpy)[nd] = fy;
@@ -444,7 +426,7 @@ __global__ __launch_bounds__(256) void k_node_gather(const DomDev *doms, const T
     const double fx = sx + hx, fy = sy + hy, fz = sz + hz;
     if (FUSE_UPDATE) {
         node_advance(D, ns_old(D), ns_new(D), nd, node_symm<false>(D, nd, i, j, k), fx, fy, fz,
-                     D.ctl->deltatime);
+                     D.ctl->ndeltatime);
     } else {
         gp(D.fpx)[nd] = fx;
         gp(D.fpy)[nd] = fy;
@@ -465,7 +447,7 @@ __global__ __launch_bounds__(256) void k_node_update(const DomDev *doms, const T
     int i = 0, j = 0, k = 0;
     if (!U) node_ijk(D.nx, nd, i, j, k);
     node_advance(D, ns_old(D), ns_new(D), nd, node_symm<U>(D, nd, i, j, k), gp(D.fx)[nd], gp(D.fy)[nd], gp(D.fz)[nd],
-                 D.ctl->deltatime);
+                 D.ctl->ndeltatime);
 }
 
 // CommSBN + the node update in one pass (device_local backend): a shared
@@ -527,7 +509,7 @@ __global__ __launch_bounds__(256) void k_node_update_direct(const DomDev *doms, 
     gp(D.fy)[nd] = fy;
     gp(D.fz)[nd] = fz;
     node_advance(D, ns_old(D), ns_new(D), nd, node_symm<false>(D, nd, i, j, k), fx, fy, fz,
-                 D.ctl->deltatime);
+                 D.ctl->ndeltatime);
 }
 
 // the update of a node copy that CommSyncPosVel touches
@@ -541,9 +523,9 @@ __device__ __forceinline__ void node_update_pv(const DomDev *doms, const DomDev 
     gp(D.fy)[nd] = fy;
     gp(D.fz)[nd] = fz;
     const unsigned symm = node_symm<false>(D, nd, i, j, k);
-    const double dt = D.ctl->deltatime;
+    const double dt = D.ctl->ndeltatime;
     const PvNode P = D.dir_pvnode[pi];
-    const int par = D.ctl->cycle & 1;   // snapshots: read the other parity, write this one
+    const int par = D.ctl->ncycle & 1;   // snapshots: read the other parity, write this one
     NodeNew v;
     if (P.fw_dom >= 0) {
         // the same physical node in the writer: same symmetry planes
@@ -728,11 +710,26 @@ __global__ __launch_bounds__(256) void k_kinematics(const DomDev *doms, const Ti
     // the stop flag is tested once the node loads are in flight (reads
     // only: a stopped group's arrays are still allocated)
     const bool stopped = group_stopped(*G);
+    if (t.pad == kFirstTile && threadIdx.x == 0) {
+        // the subdomain's first tile makes the running cycle the completed
+        // one (the other tiles read only the n-fields) and restarts the dt
+        // minima for the EOS kernel (CalcTimeConstraintsForElems restarts
+        // from 1e20, lulesh.cc:3019-3020) -- also after an error stopped
+        // the cycle, which the reference has counted too
+        DomCtl &C = *D.ctl;
+        if (C.ncycle != C.cycle) {
+            C.time = C.ntime;
+            C.deltatime = C.ndeltatime;
+            C.cycle = C.ncycle;
+            C.dtcourant_bits = dbits(1.0e+20);
+            C.dthydro_bits = dbits(1.0e+20);
+        }
+    }
     const int el = t.start + threadIdx.x;
     if (el >= D.numElem) return;
     int n[8];
     elem_nodes_of<U>(D, el, n);
-    const double dt = D.ctl->deltatime;
+    const double dt = D.ctl->ndeltatime;
 
     const NodeState S = ns_new(D);
     double x[8], y[8], z[8], xv[8], yv[8], zv[8];
@@ -1035,7 +1032,10 @@ __global__ __launch_bounds__(256) void k_qeos(const DomDev *doms, const Tile *ti
         s_in[5][tid] = qquad;
         s_in[6][tid] = qlin;
     }
-    if (stopped) return;   // uniform over the workgroup
+    // uniform over the workgroup unless another workgroup raises an error
+    // status meanwhile: the first error code is kept and the group's state
+    // past it is void, as in the reference, which exits there
+    if (stopped) return;
     __syncthreads();
 
     // --- EOS repetitions, dealt over the workgroup ---

@@ -1,7 +1,8 @@
 """Every stepping path of the library gives the oracle's state bit for bit:
 the fused force + node-update march (default for one-subdomain groups),
-the same with a separate node update, the two-pass corner-array path,
-other z-chunk depths, eager launches instead of graphs, other EOS tile
+which also computes the cycle's time increment, the same with a separate
+time-increment launch, the same with a separate node update, the two-pass
+corner-array path, other z-chunk depths, eager launches instead of graphs, other EOS tile
 sizes (the s=12 default is 64 elements per tile); node-state buffer
 alternation across separate run calls and host field writes; the forces
 kept on request."""
@@ -29,6 +30,8 @@ def _compare(g, o, what):
     {"LULESH_NO_GRAPH": "1"},
     {"LULESH_QEOS_TILE": "128"},
     {"LULESH_QEOS_TILE": "256"},
+    {"LULESH_NO_FUSED_TI": "1"},
+    {"LULESH_NO_FUSED_TI": "1", "LULESH_NO_GRAPH": "1"},
 ])
 def test_paths_bitwise(amd, oracle_mod, monkeypatch, env):
     for k, v in env.items():
@@ -82,6 +85,7 @@ def test_forces_not_kept_rejected(amd):
 
 
 @pytest.mark.parametrize("env", [{}, {"LULESH_NO_DIRECT": "1"}, {"LULESH_NO_DIRECT_POSVEL": "1"},
+                                 {"LULESH_NO_FUSED_TI": "1"},
                                  {"LULESH_FORCE_PATH": "twopass"}])
 def test_decomposed_paths_bitwise(amd, oracle_mod, monkeypatch, env):
     """2x2x2 subdomains: the direct halo steps (CommSBN summed in the node
