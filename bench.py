@@ -18,7 +18,10 @@ Default run: --steps 200 timed cycles after --warmup 20, then
 The headline line also carries the batched and served halves of the metric
 as sub-objects ("batched": the 64-problem ensemble to stoptime vs sequential
 plain runs of the same mix; "served": a Poisson stream of --serve-requests
-requests at --serve-rate per second); --no-extra skips them.
+requests at --serve-rate per second, a load the GPU keeps up with, for
+latency; "served_saturated": --serve-sat-requests requests arriving faster
+than the GPU serves them, for throughput against sequential plain runs);
+--no-extra skips them.
 
 Other modes (one JSON line each):
   --mode batched   64-problem ensemble vs sequential plain runs
@@ -479,17 +482,20 @@ def bench_multidomain(args):
     E = 8 * 45 ** 3
     paths = {}
     for path in ("direct", "exchange", "device_copy"):
+        if args.halo_path not in ("all", path):
+            continue
         progress(f"multidomain {path}")
         ms, kern = multidomain_run(args, path)
         paths[path] = {"ms_per_step": ms, "value": E / (ms * 1e-3), "kernels_ms": kern}
-    best = paths["direct"]
+    best = paths["direct" if args.halo_path == "all" else args.halo_path]
     return {
         "metric": "8 subdomains (2x2x2 of -s 45) on one GPU, zone-cycles/s incl. on-device halo exchange",
         "value": best["value"], "unit": "zone-cycles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": best["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: Sedov blast, 8-rank decomposition on one GPU",
-        "config": {"workload": "2x2x2 subdomains, -s 45 each; value = direct halo steps"},
+        "config": {"workload": "2x2x2 subdomains, -s 45 each; value = "
+                               + ("direct halo steps" if args.halo_path == "all" else args.halo_path)},
         "halo_paths": paths,
     }
 
@@ -557,6 +563,8 @@ def main():
     ap.add_argument("--mode", default="plain",
                     choices=["plain", "copy", "batched", "serve", "multidomain", "umesh"])
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--halo-path", default="all", choices=["all", "direct", "exchange", "device_copy"],
+                    help="multidomain mode: the halo paths to time")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=8.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--slots", type=int, default=64, help="serve: max resident problems")
@@ -568,6 +576,10 @@ def main():
                     help="plain mode: requests of the served sub-object")
     ap.add_argument("--serve-rate", type=float, default=12.0,
                     help="plain mode: Poisson arrivals per second of the served sub-object")
+    ap.add_argument("--serve-sat-requests", type=int, default=400,
+                    help="plain mode: requests of the saturated served sub-object (0: none)")
+    ap.add_argument("--serve-sat-rate", type=float, default=200.0,
+                    help="plain mode: arrivals per second of the saturated served sub-object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -614,20 +626,27 @@ def headline_extras(args):
                "speedup_vs_sequential": b["speedup_vs_sequential"],
                "max_origin_energy_rel_diff_vs_plain": b["max_origin_energy_rel_diff_vs_plain"],
                "cycles_match_plain": b["cycles_match_plain"]}
-    sa = argparse.Namespace(**vars(args))
-    sa.requests, sa.rate = args.serve_requests, args.serve_rate
-    sa.seq_sample = min(args.seq_sample, args.serve_requests)
-    progress(f"served stream ({sa.requests} requests at {sa.rate}/s)")
-    v = bench_serve(sa)
-    served = {"value": v["value"], "unit": v["unit"], "workload": v["config"]["workload"],
-              "requests": sa.requests, "full_stream_requests": 2000, "rate_per_s": sa.rate,
-              "slots": sa.slots, "seed": args.seed, "offered_load": v["offered_load"],
-              "latency_s": v["latency_s"], "wall_s": v["wall_s"], "failed": v["failed"],
-              "sequential_plain_sample_value": v["sequential_plain_sample"]["value"],
-              "speedup_vs_sequential": v["speedup_vs_sequential"],
-              "fifo_sequential_model_latency_s": v["fifo_sequential_model_latency_s"],
-              "max_origin_energy_rel_diff_vs_plain": v["max_origin_energy_rel_diff_vs_plain"]}
-    return {"batched": batched, "served": served}
+    def served(requests, rate):
+        sa = argparse.Namespace(**vars(args))
+        sa.requests, sa.rate = requests, rate
+        sa.seq_sample = min(args.seq_sample, requests)
+        progress(f"served stream ({sa.requests} requests at {sa.rate}/s)")
+        v = bench_serve(sa)
+        return {"value": v["value"], "unit": v["unit"], "workload": v["config"]["workload"],
+                "requests": sa.requests, "full_stream_requests": 2000, "rate_per_s": sa.rate,
+                "slots": sa.slots, "seed": args.seed, "offered_load": v["offered_load"],
+                "latency_s": v["latency_s"], "wall_s": v["wall_s"], "failed": v["failed"],
+                "sequential_plain_sample_value": v["sequential_plain_sample"]["value"],
+                "speedup_vs_sequential": v["speedup_vs_sequential"],
+                "fifo_sequential_model_latency_s": v["fifo_sequential_model_latency_s"],
+                "max_origin_energy_rel_diff_vs_plain": v["max_origin_energy_rel_diff_vs_plain"]}
+    # latency under a load the GPU keeps up with, and throughput with
+    # arrivals faster than it can serve (the served form against
+    # sequential plain runs)
+    out = {"batched": batched, "served": served(args.serve_requests, args.serve_rate)}
+    if args.serve_sat_requests > 0:
+        out["served_saturated"] = served(args.serve_sat_requests, args.serve_sat_rate)
+    return out
 
 
 if __name__ == "__main__":
