@@ -193,9 +193,11 @@ int lulesh_group_sync(lulesh_group *g, int *cycle, double *time,
 
 /* Runs `ncycles` cycles launch by launch on the group's stream with a HIP
  * event pair around each kernel class and returns the summed milliseconds:
- * ms[0] time increment, [1] element forces, [2] node gather (+ update when
- * fused), [3] node update, [4] kinematics + gradients, [5] MonoQ + EOS +
- * dt, [6] halo exchanges.  These are real cycles of the run. */
+ * ms[0] time increment (0 where the force kernel computes it), [1] element
+ * forces (+ gather, + update, + time increment on the fused path), [2] node
+ * gather (+ update) of the two-pass path, [3] node update, [4] kinematics +
+ * gradients, [5] MonoQ + EOS + dt, [6] halo exchanges.  These are real
+ * cycles of the run. */
 int lulesh_group_profile(lulesh_group *g, int ncycles, double ms[7]);
 
 /* Final-output figures of VerifyAndWriteFinalOutput

@@ -1,7 +1,12 @@
 """Unstructured-mesh (.lmesh) path on the GPU against the oracle: the
-reference's own sedov15oct.lmesh to stoptime, the block written as an
-unstructured mesh against the structured GPU path, and a renumbered mesh
-(elements and nodes shuffled) -- all bit for bit."""
+reference's own sedov15oct.lmesh to stoptime, its sedov21oct.lmesh (31 408
+elements) after 100 cycles and at the stop time against committed oracle
+digests, the block written as an unstructured mesh against the structured
+GPU path, and a renumbered mesh (elements and nodes shuffled) -- all bit
+for bit."""
+import gzip
+import hashlib
+import json
 import os
 
 import numpy as np
@@ -13,6 +18,7 @@ import umesh_util as U
 pytestmark = pytest.mark.gpu
 
 LMESH = os.path.join(GOLDEN, U.GOLDEN_LMESH)
+U21 = json.load(open(os.path.join(GOLDEN, "umesh_sedov21oct.json")))
 NODE = ["x", "y", "z", "xd", "yd", "zd", "nodalMass"]
 ELEM = ["e", "p", "q", "v", "ss", "volo", "elemMass", "delv", "vdov", "arealg", "vnew",
         "delv_xi", "delv_eta", "delv_zeta", "delx_xi", "delx_eta", "delx_zeta"]
@@ -36,6 +42,40 @@ def test_sedov15oct_to_stoptime(amd, oracle_mod):
     v = g.verify()
     assert v["origin_energy"] == o.field("e")[o.octant_corner]
     assert v["max_abs_diff"] == 0.0
+
+
+def _check21(g, rec, what):
+    rc, cyc, t, dt = g.sync()
+    assert rc == 0 and cyc == rec["cycle"], what
+    assert t == float.fromhex(rec["time"]) and dt == float.fromhex(rec["deltatime"]), what
+    for f, h in rec["sha256"].items():
+        got = hashlib.sha256(np.ascontiguousarray(g.field(f), dtype="<f8").tobytes()).hexdigest()
+        assert got == h, f"{what} {f}"
+    v = g.verify()
+    for k, h in rec["verify"].items():
+        assert v[k] == float.fromhex(h), f"{what} {k}"
+
+
+def test_sedov21oct_to_stoptime(amd, tmp_path):
+    # digests of the oracle (exact cube root) on the same mesh:
+    # tests/golden/make_umesh_golden.py
+    print("case: sedov21oct.lmesh 100 cycles, then to stoptime", flush=True)
+    path = tmp_path / "sedov21oct.lmesh"
+    path.write_bytes(gzip.decompress(open(os.path.join(GOLDEN, "sedov21oct.lmesh.gz"), "rb").read()))
+    m = amd.Mesh.read(str(path))
+    assert (m.num_elem, m.num_node) == (31408, 34446)
+    g = amd.Group.from_mesh(m)
+    m.close()
+    g.set_max_cycles(100)
+    assert g.run() == 0
+    _check21(g, U21["its100_cbrt1"], "sedov21oct 100 cycles")
+    g.set_max_cycles(-1)
+    assert g.run() == 0
+    _check21(g, U21["its-1_cbrt1"], "sedov21oct full run")
+    # the reference's arithmetic (libm cube root): origin energy within 1e-10
+    want = float.fromhex(U21["its-1_cbrt0"]["origin_energy"])
+    assert abs(g.verify()["origin_energy"] - want) <= 1e-10 * abs(want)
+    g.close()
 
 
 def test_sedov15oct_regions_and_forces(amd, oracle_mod):

@@ -10,6 +10,9 @@ against the compiled omp_4.0 reference, so the unstructured form is pinned by
 block written as an unstructured mesh reproducing the structured run bit for
 bit, and (3) invariance under node renumbering.
 """
+import gzip
+import hashlib
+import json
 import os
 
 import numpy as np
@@ -20,6 +23,8 @@ import umesh_util as U
 
 LMESH = os.path.join(GOLDEN, U.GOLDEN_LMESH)
 REF_LMESH21 = "/root/reference/LULESH/cuda/src/sedov21oct.lmesh"
+FIX21 = os.path.join(GOLDEN, "sedov21oct.lmesh.gz")
+U21 = json.load(open(os.path.join(GOLDEN, "umesh_sedov21oct.json")))
 FIELDS = ["x", "y", "z", "xd", "yd", "zd", "e", "p", "q", "v", "ss", "volo", "elemMass",
           "nodalMass", "delv", "vdov", "arealg", "delv_xi", "delv_eta", "delv_zeta"]
 
@@ -245,3 +250,35 @@ def test_corner_mesh_reproduces_block_corner(oracle_mod):
         assert_bitwise(o.field(f), U.corner_of(ref.field(f), nx + 1, m + 1), f"corner {f}")
     e = o.field("e").reshape(m, m, m)
     assert e[0, 0, 0] > 0 and not e[-1].any() and not e[:, -1].any() and not e[:, :, -1].any()
+
+
+def _unpack21(tmp_path):
+    path = tmp_path / "sedov21oct.lmesh"
+    path.write_bytes(gzip.decompress(open(FIX21, "rb").read()))
+    return str(path)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LMESH21), reason="reference tree not present")
+def test_sedov21oct_fixture_is_the_reference_file(tmp_path):
+    # tests/golden/sedov21oct.lmesh.gz holds the reference's mesh unchanged
+    assert open(_unpack21(tmp_path), "rb").read() == open(REF_LMESH21, "rb").read()
+
+
+def test_sedov21oct_oracle_100_cycles(oracle_mod, tmp_path):
+    # the golden record the GPU test compares with (make_umesh_golden.py),
+    # reproduced by the oracle from the committed fixture
+    rec = U21["its100_cbrt1"]
+    o = oracle_mod.Oracle.from_mesh(oracle_mod.read_lmesh(_unpack21(tmp_path)),
+                                    cbrt_mode=oracle_mod.CBRT_EXACT)
+    assert (o.num_elem, o.num_node) == (31408, 34446)
+    assert o.run(100) == 0 and o.cycle == rec["cycle"] == 100
+    assert o.time == float.fromhex(rec["time"]) and o.deltatime == float.fromhex(rec["deltatime"])
+    for f, h in rec["sha256"].items():
+        got = hashlib.sha256(np.ascontiguousarray(o.field(f), dtype="<f8").tobytes()).hexdigest()
+        assert got == h, f
+    # full runs: the same cycle count in both cube-root modes, and the
+    # energies within 1e-10 of each other
+    full, libm = U21["its-1_cbrt1"], U21["its-1_cbrt0"]
+    assert full["status"] == libm["status"] == 0 and full["cycle"] == libm["cycle"] == 872
+    a, b = float.fromhex(full["origin_energy"]), float.fromhex(libm["origin_energy"])
+    assert abs(a - b) <= 1e-10 * abs(b)
