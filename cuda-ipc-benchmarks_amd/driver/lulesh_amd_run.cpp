@@ -11,8 +11,15 @@
 //   -b <bal>    region balance                         (default 1)
 //   -c <cost>   region cost                            (default 1)
 //   -t <tp>     subdomains per edge; tp^3 subdomains share the GPU (default 1)
-//   -p          print progress every 100 cycles
+//   -p          print progress every cycle (one cycle per launch, as the
+//               reference's loop prints, lulesh.cc:3227-3230)
 //   -q          quiet
+//   -z <n>      stop after n cycles of this run (the reference's iteration
+//               cap, lulesh.cc:3231-3234; 0 = none)
+//   -f <n>      accepted for the reference's viz-file count; no viz output
+//   -v          refused, as in a reference build without VIZ_MESH: the
+//               Silo output needs a library this build does not have
+//   -h          usage
 //   -w <0|1>    write subdomain 0's final coordinates to xyz.asc, as the
 //               CUDA reference's write_solution does (default 1)
 //
@@ -35,13 +42,45 @@ static int parse_int(const char *s, int *out)
     return 1;
 }
 
+static void usage(const char *exe)
+{
+    // PrintCommandLineOptions, lulesh-util.cc:30-48, with this driver's own
+    // options after the reference's
+    printf("Usage: %s [opts]\n", exe);
+    printf(" where [opts] is one or more of:\n");
+    printf(" -q              : quiet mode - suppress all stdout\n");
+    printf(" -i <iterations> : number of cycles to run\n");
+    printf(" -s <size>       : length of cube mesh along side\n");
+    printf(" -r <numregions> : Number of distinct regions (def: 11)\n");
+    printf(" -b <balance>    : Load balance between regions of a domain (def: 1)\n");
+    printf(" -c <cost>       : Extra cost of more expensive regions (def: 1)\n");
+    printf(" -f <numfiles>   : Number of files to split viz dump into (no viz output here)\n");
+    printf(" -p              : Print out progress\n");
+    printf(" -v              : Output viz file (not available: needs the Silo library)\n");
+    printf(" -z <cycles>     : Stop after this many cycles of the run\n");
+    printf(" -h              : This message\n");
+    printf(" -u <file>       : Unstructured mesh (.lmesh) instead of the cube\n");
+    printf(" -t <tp>         : tp^3 subdomains on the one GPU (def: 1)\n");
+    printf(" -w <0|1>        : Write xyz.asc (def: 1)\n");
+    printf("\n\n");
+}
+
 int main(int argc, char **argv)
 {
     const auto t_app = std::chrono::steady_clock::now();
     int its = -1, nx = 30, nr = 11, bal = 1, cost = 1, tp = 1, prog = 0, quiet = 0, wsol = 1;
+    int zcap = 0, nfiles = 0;
     const char *mesh_path = nullptr;
     for (int i = 1; i < argc; ++i) {
         int *dst = nullptr;
+        if (!strcmp(argv[i], "-h")) {
+            usage(argv[0]);
+            return 0;
+        }
+        if (!strcmp(argv[i], "-v")) {
+            printf("Use of -v requires the Silo library, which this build does not have\n");
+            return 1;
+        }
         if (!strcmp(argv[i], "-u")) {
             if (i + 1 >= argc) {
                 fprintf(stderr, "file name required after -u\n");
@@ -57,10 +96,13 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "-c")) dst = &cost;
         else if (!strcmp(argv[i], "-t")) dst = &tp;
         else if (!strcmp(argv[i], "-w")) dst = &wsol;
+        else if (!strcmp(argv[i], "-z")) dst = &zcap;
+        else if (!strcmp(argv[i], "-f")) dst = &nfiles;
         else if (!strcmp(argv[i], "-p")) { prog = 1; continue; }
         else if (!strcmp(argv[i], "-q")) { quiet = 1; continue; }
         else {
-            fprintf(stderr, "unknown option %s\n", argv[i]);
+            usage(argv[0]);
+            printf("ERROR: Unknown command line argument: %s\n", argv[i]);
             return 1;
         }
         if (i + 1 >= argc || !parse_int(argv[i + 1], dst)) {
@@ -69,6 +111,7 @@ int main(int argc, char **argv)
         }
         ++i;
     }
+    (void)nfiles;   // viz files: no viz output is written
     int rc = lulesh_init(0);
     if (rc) { fprintf(stderr, "lulesh_init: %s\n", lulesh_status_string(rc)); return 1; }
     lulesh_group *g = nullptr;
@@ -87,6 +130,9 @@ int main(int argc, char **argv)
         rc = lulesh_group_create(nx, tp, nr, bal, cost, &g);
     }
     if (rc) { fprintf(stderr, "create: %s\n", lulesh_status_string(rc)); return 1; }
+    // the run starts at cycle 0, so an iteration cap of z cycles is a cycle
+    // cap of z (the reference's loop breaks after its z-th iteration)
+    if (zcap > 0) its = its < 0 ? zcap : (its < zcap ? its : zcap);
     if (its >= 0) lulesh_group_set_max_cycles(g, its);
     const int ndom = tp * tp * tp;
     if (!quiet) {
@@ -103,13 +149,13 @@ int main(int argc, char **argv)
     int cycle = 0;
     double time = 0, dt = 0;
     for (;;) {
-        rc = lulesh_group_enqueue(g, prog ? 100 : 64, nullptr);
+        rc = lulesh_group_enqueue(g, prog ? 1 : 64, nullptr);
         if (rc) break;
         const int prev = cycle;
         rc = lulesh_group_sync(g, &cycle, &time, &dt, nullptr);
         if (rc) break;
-        if (prog && !quiet) printf("cycle = %d, time = %e, dt=%e\n", cycle, time, dt);
         if (cycle == prev) break;   // stop condition reached on the device
+        if (prog && !quiet) printf("cycle = %d, time = %e, dt=%e\n", cycle, time, dt);
     }
     const double elapsed =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

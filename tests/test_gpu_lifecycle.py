@@ -63,3 +63,20 @@ def test_driver_phase_lines_and_solution(tmp_path):
                      "PHASE_COMM_FREE_ms", "PHASE_APP_TOTAL_ms"]
     assert "Final Origin Energy" in out.stdout
     assert (tmp_path / "xyz.asc").stat().st_size > 0
+
+
+def test_driver_progress_and_iteration_cap(tmp_path, oracle_mod):
+    # -p prints every cycle in the reference's format (lulesh.cc:3227-3230);
+    # -z 5 stops after five cycles (lulesh.cc:3231-3234)
+    out = subprocess.run([DRIVER, "-s", "6", "-p", "-z", "5", "-w", "0"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    prog = [ln for ln in out.stdout.splitlines() if ln.startswith("cycle = ")]
+    o = oracle_mod.Oracle(6, cbrt_mode=oracle_mod.CBRT_EXACT)
+    want = []
+    for c in range(1, 6):
+        o.run(1)   # one more cycle
+        assert o.cycle == c
+        want.append("cycle = %d, time = %e, dt=%e" % (c, o.time, o.deltatime))
+    assert prog == want
+    assert "Iteration count     =  5 " in out.stdout
