@@ -160,7 +160,7 @@ struct Tile {
 
 // Tile::pad of an element tile that is its subdomain's first (the one
 // that completes the running cycle's time state in the kinematics kernel),
-// and ColTile::pad[0] of a subdomain's first column tile (the one that
+// and ColTile::flags of a subdomain's first column tile (the one that
 // publishes the cycle's time increment in the force kernel)
 constexpr int kFirstTile = 2;
 
@@ -169,13 +169,14 @@ constexpr int kFirstTile = 2;
 constexpr unsigned kEvLast = 0x8000u, kEvMask = 0x7fffu;
 
 // column tile of the fused force kernel: (ex-1) x (ey-1) nodes from
-// (i0, j0), node planes [k0, k0 + kchunk); its ex x ey threads are the
+// (i0, j0), node planes [k0, k0 + kn); its ex x ey threads are the
 // elements of the tile plus one halo row and column on the low side
 struct ColTile {
     int dom;
     int i0, j0, k0;
     int ex, ey;
-    int pad[2];
+    int flags;   // kFirstTile or 0
+    int kn;      // node planes
 };
 
 // elemBC bits (LULESH/omp_4.0/lulesh.h:60-88)

@@ -102,38 +102,88 @@ static void tile_shape(int nx, int &ex, int &ey)
         }
 }
 
-// node planes per column tile.  The force kernel runs two 256-thread
-// workgroups per CU; a chunk of kc node planes computes kc + 1 element
-// planes, so the kernel takes about ceil(tiles / (2 * CUs)) * (kc + 1)
-// plane steps.  Pick the kc in [1, 32] that minimises that (the smaller kc
-// on a tie, for the shorter tail): small meshes get short chunks and more
-// workgroups, large ones the least recomputation that still fills the
-// chip in whole rounds.
-static int auto_kchunk(const Engine *e)
+// A column's s1 node planes are split into runs of about kc planes, either
+// evenly (max(1, s1 / kc) runs of kc or kc + 1 planes, the longer first:
+// no short remainder run) or as runs of kc with the remainder last.
+struct ZRuns {
+    int n, len, extra;   // runs; planes per run; see planes()
+    bool even;
+    int planes(int r) const
+    {
+        if (even) return len + (r < extra ? 1 : 0);
+        return r == n - 1 ? extra : len;
+    }
+};
+static ZRuns z_runs(int s1, int kc, bool even)
+{
+    kc = std::max(kc, 1);
+    if (even) {
+        const int n = std::max(1, s1 / kc);
+        return ZRuns{n, s1 / n, s1 % n, true};
+    }
+    const int n = (s1 + kc - 1) / kc;
+    return ZRuns{n, kc, s1 - kc * (n - 1), false};
+}
+
+// Model of the force kernel's time for one z-chunking, in plane steps.  A
+// workgroup over a run of k node planes takes about k + 2 plane steps (k + 1
+// element planes and its start-up); the kernel runs two workgroups per CU,
+// so with T workgroups it takes about ceil(T / (2 CUs)) rounds of the
+// longest run, and a workgroup alone on its CU (T <= CUs) runs its steps
+// about 0.72 times as long.  Ties go to more workgroups (the dispatcher
+// balances a longer queue better), then to the even split.  Fitted to the
+// force kernel's times on MI355X over every split tried at -s 32..200 (for
+// each size the model's pick is within 0.4 % of the fastest measured;
+// -s 48 runs 29.0 us with 204 workgroups of 3 planes against 32.5 us with
+// 300 of 2, -s 100 100 us with 480 of 10-11 against 110 us with 1008 of 5).
+struct ZCost {
+    double time = 0.0;
+    long tiles = 0;
+};
+static ZCost z_cost(const Engine *e, int kc, bool even, int cus)
+{
+    long tiles = 0;
+    int longest = 0;
+    for (auto *g : e->groups)
+        for (auto *d : g->doms) {
+            int ex, ey;
+            tile_shape(d->host.nx, ex, ey);
+            const int s1 = d->host.nx + 1;
+            const long cols = (long)((s1 + ex - 2) / (ex - 1)) * ((s1 + ey - 2) / (ey - 1));
+            const ZRuns z = z_runs(s1, kc, even);
+            tiles += cols * z.n;
+            longest = std::max(longest, std::max(z.planes(0), z.planes(z.n - 1)));
+        }
+    const long slots = 2L * cus;
+    ZCost c;
+    c.time = (double)((tiles + slots - 1) / slots) * (longest + 2) * (tiles <= cus ? 0.72 : 1.0);
+    c.tiles = tiles;
+    return c;
+}
+
+// the z-chunking of the engine's column tiles: the kchunk given (0 = every
+// kc in [1, 32]) with the better split by the model above
+static void choose_zruns(const Engine *e, int &kc_out, bool &even_out)
 {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const long slots = 2L * std::max(cus, 1);
-    long best_cost = -1;
-    int best = 4;
-    for (int kc = 1; kc <= 32; ++kc) {
-        long tiles = 0;
-        for (auto *g : e->groups)
-            for (auto *d : g->doms) {
-                int ex, ey;
-                tile_shape(d->host.nx, ex, ey);
-                const int s1 = d->host.nx + 1;
-                tiles += (long)((s1 + ex - 2) / (ex - 1)) * ((s1 + ey - 2) / (ey - 1)) *
-                         ((s1 + kc - 1) / kc);
+    cus = std::max(cus, 1);
+    const int lo = e->kchunk > 0 ? e->kchunk : 1, hi = e->kchunk > 0 ? e->kchunk : 32;
+    bool have = false;
+    ZCost best;
+    for (int kc = lo; kc <= hi; ++kc)
+        for (int ev = 1; ev >= 0; --ev) {
+            const ZCost c = z_cost(e, kc, ev != 0, cus);
+            const bool better = !have || c.time < best.time - 1e-9 ||
+                                (c.time < best.time + 1e-9 && c.tiles > best.tiles);
+            if (better) {
+                have = true;
+                best = c;
+                kc_out = kc;
+                even_out = ev != 0;
             }
-        const long cost = ((tiles + slots - 1) / slots) * (kc + 1);
-        if (best_cost < 0 || cost < best_cost) {
-            best_cost = cost;
-            best = kc;
         }
-    }
-    return best;
 }
 
 static int build_tiles(Engine *e)
@@ -142,18 +192,21 @@ static int build_tiles(Engine *e)
     std::vector<ColTile> ct;
     // column tiles of the force march (the block's (i,j,k) structure; an
     // unstructured engine uses the element/node tiles only)
-    const int kc = e->umesh ? 1 : e->kchunk > 0 ? e->kchunk : auto_kchunk(e);
+    int kc = 1;
+    bool even = true;
+    if (!e->umesh) choose_zruns(e, kc, even);
     for (int di = 0, gi = 0; gi < (int)e->groups.size() && !e->umesh; ++gi)
         for (auto *d : e->groups[gi]->doms) {
             const int s1 = d->host.nx + 1;
             int ex, ey;
             tile_shape(d->host.nx, ex, ey);
             if (ex * ey > 256 || (ex + 1) * (ey + 1) > kForcePatch) return LULESH_EINVAL;
-            for (int k0 = 0; k0 < s1; k0 += kc)
+            const ZRuns z = z_runs(s1, kc, even);
+            for (int r = 0, k0 = 0; r < z.n; k0 += z.planes(r), ++r)
                 for (int j0 = 0; j0 < s1; j0 += ey - 1)
                     for (int i0 = 0; i0 < s1; i0 += ex - 1)
-                        ct.push_back(ColTile{di, i0, j0, k0, ex, ey,
-                                             {(k0 | j0 | i0) == 0 ? kFirstTile : 0, 0}});
+                        ct.push_back(ColTile{di, i0, j0, k0, ex, ey, (k0 | j0 | i0) == 0 ? kFirstTile : 0,
+                                             z.planes(r)});
             ++di;
         }
     e->kchunk_used = kc;
@@ -474,8 +527,8 @@ static int enqueue_cycle(Engine *e, hipStream_t s, Prof *prof)
         STEP(0, HIPCHK(launch_time_increment(e->d_groups, ng, e->d_doms, e->d_dom0, s)));
     bool updated;   // node update done inside the force step
     if (e->march) {
-        STEP(1, HIPCHK(launch_force_march(e->d_doms, e->d_ctiles, e->n_ctiles, e->kchunk_used,
-                                          e->fuse_update, e->store_forces, e->fused_ti, s)));
+        STEP(1, HIPCHK(launch_force_march(e->d_doms, e->d_ctiles, e->n_ctiles, e->fuse_update,
+                                          e->store_forces, e->fused_ti, s)));
         updated = e->fuse_update;
     } else {
         STEP(1, HIPCHK(launch_elem_force(e->umesh, e->d_doms, e->d_etiles, e->n_etiles, e->tile_e, s)));

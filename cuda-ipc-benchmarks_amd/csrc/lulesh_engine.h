@@ -113,7 +113,9 @@ struct Engine {
     // fused force + gather (z-marching column tiles) or the two-pass
     // corner-array path
     bool march = true;
-    int kchunk = 0;            // node planes per column tile (0 = auto)
+    // node planes per column tile (0 = auto): a column's planes are split
+    // into max(1, planes / kchunk) runs of kchunk or kchunk + 1 planes
+    int kchunk = 0;
     int kchunk_used = 8;
     ColTile *d_ctiles = nullptr;
     int n_ctiles = 0;

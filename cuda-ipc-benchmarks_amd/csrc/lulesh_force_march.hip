@@ -33,7 +33,7 @@ namespace lulesh {
 // started from +0.0 unchanged.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColTile *tiles, int ntiles,
-                                                        int kchunk, int advance, int store_f, int ti)
+                                                        int advance, int store_f, int ti)
 {
     // corner forces of one phase, in pairs: lds[d][p][t] = component d of
     // element t's corners p and p+4 (the node-plane-c and node-plane-c+1
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColT
     const int ni = t.i0 + ta, nj = t.j0 + tb;
     const bool node_ok = ta < ex - 1 && tb < ey - 1 && ni <= nx && nj <= nx;
     const int k0 = t.k0;
-    const int k1 = min(t.k0 + kchunk, nx + 1);
+    const int k1 = min(t.k0 + t.kn, nx + 1);
     // lower / upper corner slots seen from the node thread:
     // own element, +x, +y, +x+y neighbours
     const int t1 = tid + 1, t2 = tid + ex, t3 = tid + ex + 1;
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColT
                     const bool stop = ti_stop(*gp(&Cg[0].time), *gp(&Cg[0].cycle), stoptime, max_cycles);
                     const double deltatime = stop ? -1.0 : ti_deltatime(newdt, olddt, time, cyc, stoptime);
                     ti_dt = deltatime;
-                    if (t.pad[0] == kFirstTile) {
+                    if (t.flags == kFirstTile) {
                         // the subdomain's first tile publishes the running
                         // cycle for the cycle's later kernels (no workgroup
                         // of this launch reads the n-fields or the
@@ -362,11 +362,11 @@ __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColT
 }
 
 
-hipError_t launch_force_march(DomDev *d, const ColTile *t, int nt, int kchunk, bool advance, bool store_f,
-                              bool ti, hipStream_t s)
+hipError_t launch_force_march(DomDev *d, const ColTile *t, int nt, bool advance, bool store_f, bool ti,
+                              hipStream_t s)
 {
     if (nt == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_force_march, dim3(nt), dim3(kMThreads), 0, s, d, t, nt, kchunk, advance ? 1 : 0,
+    hipLaunchKernelGGL(k_force_march, dim3(nt), dim3(kMThreads), 0, s, d, t, nt, advance ? 1 : 0,
                        store_f ? 1 : 0, ti ? 1 : 0);
     return hipGetLastError();
 }

@@ -11,14 +11,15 @@ hipError_t launch_time_increment(GroupCtl *const *groups, int ngroups, DomDev *d
 hipError_t launch_elem_force(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 hipError_t launch_node_gather(bool umesh, bool fuse, const DomDev *d, const Tile *t, int nt, int bs,
                               hipStream_t s);
-// fused force + ordered gather (16x16-thread column tiles, kchunk node planes)
+// fused force + ordered gather (column tiles of at most 256 threads, each
+// over its own run of node planes)
 // node slots per plane of the force kernel's LDS patch: (ex+1)*(ey+1) must fit
 constexpr int kForcePatch = 320;
 // advance: also the node update (old -> new node-state buffer) as each
 // node plane completes; store_f: write fx..fz; ti: the cycle's time
 // increment is computed in the kernel (no time-increment launch before it)
-hipError_t launch_force_march(DomDev *d, const ColTile *t, int nt, int kchunk, bool advance, bool store_f,
-                              bool ti, hipStream_t s);
+hipError_t launch_force_march(DomDev *d, const ColTile *t, int nt, bool advance, bool store_f, bool ti,
+                              hipStream_t s);
 hipError_t launch_node_update(bool umesh, const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);
 // CommSBN summed directly from the neighbours' partial forces + node update
 hipError_t launch_node_update_direct(const DomDev *d, const Tile *t, int nt, int bs, hipStream_t s);

@@ -415,7 +415,9 @@ int lulesh_batch_results(lulesh_batch *b, int *cycles, int *status);
  * tile_nodes for node kernels (64, 128, 192 or 256); 0 keeps the current
  * value. */
 int lulesh_batch_set_tiles(lulesh_batch *b, int tile_elems, int tile_nodes);
-/* Node planes per column tile of the fused force kernel (0 = automatic). */
+/* Node planes per column tile of the fused force kernel (0 = automatic): a
+ * column of P node planes is split into max(1, P / kchunk) runs of kchunk
+ * or kchunk + 1 planes. */
 int lulesh_batch_set_kchunk(lulesh_batch *b, int kchunk);
 /* Autotuner: times `probe_cycles` real cycles of the ensemble for each
  * candidate element tile {64,128,256}, node tile {128,256} and column chunk
