@@ -77,21 +77,26 @@ __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColT
     // own element, +x, +y, +x+y neighbours
     const int t1 = tid + 1, t2 = tid + ex, t3 = tid + ex + 1;
 
-    // Node patch: slot (pi, pj) holds node (i0-1+pi, j0-1+pj) of a node
-    // plane.  The element's corners 0..3 are slots e0, e0+1, e0+px+1,
-    // e0+px of its lower plane, 4..7 the same slots of the upper one.  A
+    // Node patch: node (i0-1+pi, j0-1+pj) of a node plane, pi in [0, ex],
+    // pj in [0, ey], sits in slot pj*ex + pi for pi < ex and in a last
+    // column of its own, ex*(ey+1) + pj, for pi = ex.  Rows of ex slots
+    // match the rows of ex threads, so a wave's lanes read consecutive
+    // slots (no bank conflict where a group of lanes spans a row end), save
+    // the one lane per row whose corner lies in the last column.  The
+    // element's corners 0..3 are nodes (ta, tb), (ta+1, tb), (ta+1, tb+1),
+    // (ta, tb+1) of its lower plane, 4..7 the same of the upper one.  A
     // thread loads the slot of the node it owns, (ta+1, tb+1) = corner 2
     // of its element; the first ex+ey+1 threads also load one slot of the
     // patch's row 0 / column 0.
-    const int e0 = tb * px + ta;
-    const int cs[4] = {e0, e0 + 1, e0 + px + 1, e0 + px};
-    const int own_sl = e0 + px + 1;
+    auto slot = [&](int pi, int pj) { return pi < ex ? pj * ex + pi : ex * (ey + 1) + pj; };
+    const int cs[4] = {slot(ta, tb), slot(ta + 1, tb), slot(ta + 1, tb + 1), slot(ta, tb + 1)};
+    const int own_sl = slot(ta + 1, tb + 1);
     const int own_i = ni, own_j = nj;
     const bool own_in = slot_ok && own_i <= nx && own_j <= nx;
     int xt_sl = -1, xt_i = 0, xt_j = 0;
     if (tid < ex + ey + 1) {
         const int pi = tid < px ? tid : 0, pj = tid < px ? 0 : tid - px + 1;
-        xt_sl = pj * px + pi;
+        xt_sl = slot(pi, pj);
         xt_i = t.i0 - 1 + pi;
         xt_j = t.j0 - 1 + pj;
         if (xt_i < 0 || xt_i > nx || xt_j < 0 || xt_j > nx) xt_sl = -1;
