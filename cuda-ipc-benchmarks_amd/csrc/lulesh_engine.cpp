@@ -161,13 +161,13 @@ static ZCost z_cost(const Engine *e, int kc, bool even, int cus)
     return c;
 }
 
-// the z-chunking of the engine's column tiles: the kchunk given (0 = every
+// The z-chunking of the engine's column tiles: the kchunk given (0 = every
 // kc in [1, 32]) with the better split by the model above.  An engine of
 // several groups (a batch, a server) mixes run lengths, which the model's
-// rounds do not describe: it keeps runs of kc planes with the remainder
-// last, and the model with the chunk length for the longest run, which
-// served streams ran faster with (5.21 against 5.15e9 zone-cycles/s on 400
-// saturating requests); the batch autotuner times its candidates instead.
+// rounds do not describe: there the runs are kc planes with the remainder
+// last, and kc minimises rounds x (kc + 1) (served streams ran faster so,
+// 5.21 against 5.15e9 zone-cycles/s on 400 saturating requests); the batch
+// autotuner times its candidates instead.
 static void choose_zruns(const Engine *e, int &kc_out, bool &even_out)
 {
     int dev = 0, cus = 256;
