@@ -164,10 +164,12 @@ static ZCost z_cost(const Engine *e, int kc, bool even, int cus)
 // The z-chunking of the engine's column tiles: the kchunk given (0 = every
 // kc in [1, 32]) with the better split by the model above.  An engine of
 // several groups (a batch, a server) mixes run lengths, which the model's
-// rounds do not describe: there the runs are kc planes with the remainder
-// last, and kc minimises rounds x (kc + 1) (served streams ran faster so,
-// 5.21 against 5.15e9 zone-cycles/s on 400 saturating requests); the batch
-// autotuner times its candidates instead.
+// rounds do not describe: there kc minimises rounds x (kc + 1) over runs of
+// kc planes, and each group's columns are split evenly into runs of kc or
+// kc + 1 (measured on MI355X: served streams 5.15 with the single-group
+// model, 5.25 with runs of kc and the remainder last, 5.28e9 zone-cycles/s
+// with even runs; the 64-problem ensemble 4.95 -> 5.04e9 at kc = 8); the
+// batch autotuner times its kc candidates on top.
 static void choose_zruns(const Engine *e, int &kc_out, bool &even_out)
 {
     int dev = 0, cus = 256;
@@ -179,7 +181,7 @@ static void choose_zruns(const Engine *e, int &kc_out, bool &even_out)
         const long slots = 2L * cus;
         long best_cost = -1;
         kc_out = lo;
-        even_out = false;
+        even_out = true;
         for (int kc = lo; kc <= hi; ++kc) {
             long tiles = 0;
             for (auto *g : e->groups)
