@@ -20,8 +20,9 @@ as sub-objects ("batched": the 64-problem ensemble to stoptime vs sequential
 plain runs of the same mix; "served": a Poisson stream of --serve-requests
 requests at --serve-rate per second, a load the GPU keeps up with, for
 latency; "served_saturated": --serve-sat-requests requests arriving faster
-than the GPU serves them, for throughput against sequential plain runs);
---no-extra skips them.
+than the GPU serves them, for throughput against sequential plain runs;
+"multidomain": 2x2x2 subdomains of -s 45 on the one GPU, halo exchange
+included, on each halo path); --no-extra skips them.
 
 Other modes (one JSON line each):
   --mode batched   64-problem ensemble vs sequential plain runs
@@ -646,6 +647,15 @@ def headline_extras(args):
     out = {"batched": batched, "served": served(args.serve_requests, args.serve_rate)}
     if args.serve_sat_requests > 0:
         out["served_saturated"] = served(args.serve_sat_requests, args.serve_sat_rate)
+    # eight subdomains (2x2x2 of -s 45) on the one GPU, halo exchange
+    # included, on each halo path
+    ma = argparse.Namespace(**vars(args))
+    ma.halo_path = "all"
+    md = bench_multidomain(ma)
+    out["multidomain"] = {"unit": md["unit"], "workload": md["config"]["workload"],
+                          "steps": md["steps"],
+                          "halo_paths": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"]}
+                                         for k, v in md["halo_paths"].items()}}
     return out
 
 
