@@ -255,21 +255,43 @@ static const lulesh_comm_backend_ops kDeviceLocal = {
 // ---------------------------------------------------------------------------
 // device_copy backend (the reference's Mode A, comm_ipc_common.h:192-225):
 // every message is packed into the sender's own send buffer, then copied
-// device to device into the receiver's slot -- one copy per message, as
-// COMM_SEND_MSG / COMM_SEND_CORNER do over IPC.
+// device to device into the receiver's slot, as COMM_SEND_MSG /
+// COMM_SEND_CORNER do over IPC.  The copies are queued as the messages are
+// handed over (their packs are already on the stream) and issued by the
+// next COMM_SEND_FINISH as one copy kernel for everything queued (one
+// hipMemcpyAsync per message cost 0.29 ms of a 2x2x2 -s 45 cycle: 56 copies
+// per exchange step).
 // ---------------------------------------------------------------------------
 static double *dc_pack_dest(lulesh_domain *, lulesh_domain *, size_t, void *) { return nullptr; }
 
-static int dc_send(lulesh_domain *, lulesh_domain *to, size_t offset, const double *src, size_t count,
-                   lulesh_stream_t stream, void *)
+static int dc_flush(lulesh_domain *from, lulesh_stream_t stream)
 {
-    const hipError_t e = hipMemcpyAsync(to->recv + offset, src, count * sizeof(double),
-                                        hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    CopyMsgs &q = from->g->dc_pending;
+    const hipError_t e = launch_copy_msgs(q, (hipStream_t)stream);
+    q.n = 0;
     return e == hipSuccess ? LULESH_OK : LULESH_EHIP;
 }
 
+static int dc_send(lulesh_domain *from, lulesh_domain *to, size_t offset, const double *src, size_t count,
+                   lulesh_stream_t stream, void *)
+{
+    if (count > 0xffffffffu) return LULESH_EINVAL;
+    CopyMsgs &q = from->g->dc_pending;
+    if (q.n == kMaxCopyMsgs) {
+        const int rc = dc_flush(from, stream);
+        if (rc) return rc;
+    }
+    q.dst[q.n] = to->recv + offset;
+    q.src[q.n] = src;
+    q.count[q.n] = (unsigned)count;
+    ++q.n;
+    return LULESH_OK;
+}
+
+static int dc_send_finish(lulesh_domain *from, lulesh_stream_t stream, void *) { return dc_flush(from, stream); }
+
 static const lulesh_comm_backend_ops kDeviceCopy = {
-    "device_copy", dl_alloc_recv, dl_teardown, dc_pack_dest, dl_send_finish, dl_recv_base,
+    "device_copy", dl_alloc_recv, dl_teardown, dc_pack_dest, dc_send_finish, dl_recv_base,
     nullptr, nullptr, nullptr, nullptr, dc_send, dc_send, nullptr, nullptr, nullptr, nullptr};
 
 // ---------------------------------------------------------------------------
@@ -387,6 +409,9 @@ int plan_pack(lulesh_group *g, DevPlan *p, int rank, hipStream_t s, bool gated =
 {
     const int a = rank < 0 ? 0 : p->pack_start[rank];
     const int b = rank < 0 ? p->npack : p->pack_start[rank + 1];
+    // a step begins: copies the device_copy backend queued for a step that
+    // failed before its COMM_SEND_FINISH are dropped
+    g->dc_pending.n = 0;
     HIPCHK(launch_pack(p->pack + a, b - a, p->fptr, p->nfields, gated ? g->gctl : nullptr, s));
     const lulesh_comm_backend_ops *ops = g->backend;
     const int m0 = rank < 0 ? 0 : p->send_start[rank];

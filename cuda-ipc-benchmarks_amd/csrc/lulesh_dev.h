@@ -223,4 +223,14 @@ struct UnpackSrc {
     int pad;
 };
 
+// device-to-device copies of messages (the device_copy backend's sends),
+// passed to the copy kernel by value
+constexpr int kMaxCopyMsgs = 64;
+struct CopyMsgs {
+    double *dst[kMaxCopyMsgs];
+    const double *src[kMaxCopyMsgs];
+    unsigned count[kMaxCopyMsgs];   // doubles
+    int n;
+};
+
 }  // namespace lulesh

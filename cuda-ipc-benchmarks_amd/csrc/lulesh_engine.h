@@ -86,6 +86,7 @@ struct lulesh_group {
     std::map<std::string, std::unique_ptr<lulesh::DevPlan>> plans;
     std::unique_ptr<lulesh::DirectTables> direct;   // built on first use
     lulesh::Engine *engine = nullptr;   // standalone stepping
+    lulesh::CopyMsgs dc_pending{};      // device_copy backend: sends not yet copied
     // the node state changed behind the engine's back (PosVel snapshots
     // stale) where it could not be freed at once: rebuilt at the next step
     bool engine_stale = false;

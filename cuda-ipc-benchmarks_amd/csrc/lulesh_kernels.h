@@ -43,6 +43,8 @@ hipError_t launch_apply_corner(double *const f[6], int nf, int idx, const double
 hipError_t launch_setup(const DomDev &D, bool umesh, hipStream_t s);
 hipError_t launch_hg_coef(double *coef, const double *ss, const double *mass, const double *volo,
                           const double *v, int n, hipStream_t s);
+// the messages of m, each copied device to device (one launch)
+hipError_t launch_copy_msgs(const CopyMsgs &m, hipStream_t s);
 hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s);
 hipError_t launch_probe_cbrt(const double *in, double *out, int n, hipStream_t s);
 hipError_t launch_probe_sqrt_div(const double *a, const double *b, double *out, int n, hipStream_t s);

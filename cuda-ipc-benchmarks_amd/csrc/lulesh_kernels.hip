@@ -1409,6 +1409,28 @@ __global__ __launch_bounds__(256) void k_probe_copy(const double2 *__restrict__ 
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += stride) b[t] = a[t];
 }
 
+// message i is copied by the workgroups of grid row i
+__global__ __launch_bounds__(256) void k_copy_msgs(CopyMsgs m)
+{
+    const int i = blockIdx.y;
+    double *const dst = m.dst[i];
+    const double *const src = m.src[i];
+    const unsigned n = m.count[i];
+    for (unsigned w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x)
+        dst[w] = src[w];
+}
+
+hipError_t launch_copy_msgs(const CopyMsgs &m, hipStream_t s)
+{
+    if (m.n <= 0) return hipSuccess;
+    if (m.n > kMaxCopyMsgs) return hipErrorInvalidValue;
+    unsigned longest = 0;
+    for (int i = 0; i < m.n; ++i) longest = m.count[i] > longest ? m.count[i] : longest;
+    const unsigned bx = longest == 0 ? 1u : (longest + 255u) / 256u < 64u ? (longest + 255u) / 256u : 64u;
+    hipLaunchKernelGGL(k_copy_msgs, dim3(bx, m.n), dim3(256), 0, s, m);
+    return hipGetLastError();
+}
+
 hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s)
 {
     const size_t n2 = n / 2;
