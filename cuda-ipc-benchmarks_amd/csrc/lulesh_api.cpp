@@ -1067,6 +1067,29 @@ int lulesh_group_profile(lulesh_group *g, int ncycles, double ms[7])
     return engine_profile(g->engine, ncycles, g->stream, ms);
 }
 
+int lulesh_group_force_plan(lulesh_group *g, int *workgroups, int *node_planes_max, int *paired)
+{
+    if (!g) return LULESH_EINVAL;
+    drop_stale_engine(g);
+    if (!g->engine) {
+        std::vector<lulesh_group *> one{g};
+        const int rc = engine_create(one, &g->engine);
+        if (rc) return rc;
+    }
+    int longest = 0;
+    if (g->engine->n_ctiles > 0) {
+        std::vector<ColTile> t(g->engine->n_ctiles);
+        if (hipMemcpy(t.data(), g->engine->d_ctiles, t.size() * sizeof(ColTile), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            return LULESH_EHIP;
+        for (const ColTile &c : t) longest = std::max(longest, c.kn);
+    }
+    if (workgroups) *workgroups = g->engine->n_ctiles;
+    if (node_planes_max) *node_planes_max = longest;
+    if (paired) *paired = g->engine->paired_runs ? 1 : 0;
+    return LULESH_OK;
+}
+
 int lulesh_group_verify(lulesh_group *g, int rank, double out[4])
 {
     if (!g || rank < 0 || rank >= g->ndom || !out) return LULESH_EINVAL;

@@ -216,6 +216,178 @@ static void choose_zruns(const Engine *e, int &kc_out, bool &even_out)
         }
 }
 
+// ---------------------------------------------------------------------------
+// Paired z-runs: one launch round balanced for the two workgroups of a CU.
+//
+// With at most two workgroups per CU, the dispatcher gives blocks 0..CUs-1
+// one CU each and block b + CUs the CU of block b.  The two do not progress
+// alike: the older workgroup's waves win the issue arbitration whenever
+// both are ready, so it runs almost as fast as alone while the younger
+// takes the cycles it leaves.  Measured per workgroup on MI355X at -s 100
+// (wall-clock stamps at each workgroup's start and end, runs of 10-11
+// planes): older 87.5 us, younger 107 us, alone 80 us -- relative rates
+// 1.0, 0.756 and 1.094, and the launch lasts as long as the younger ones.
+// So a round is balanced by giving the older slots longer runs: for a
+// target time T, a slot of rate v takes a run of at most floor(T v) - 2
+// node planes (a run of k planes costs about k + 2 plane steps: k + 1
+// element planes and its start-up); the columns are filled bottom-up over
+// the slots in launch order, each column's planes are then spread over its
+// slots in proportion to their rates, and T is the least for which every
+// column fits.  The number of workgroups is chosen by the pair model below
+// (a CU's finish time with its two runs), against the even split.
+// ---------------------------------------------------------------------------
+static const double kRateOlder = 1.0, kRateYounger = 0.756, kRateAlone = 1.094;
+static const double kRunStartup = 2.0;
+
+// host mirror of the kernels' blockIdx -> tile index map (xcd_tile)
+static int xcd_tile_host(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, slot = b >> 3;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+static double slot_rate(int b, int nb, int cus)
+{
+    if (b >= cus) return kRateYounger;
+    return b + cus < nb ? kRateOlder : kRateAlone;
+}
+
+// finish time (plane steps) of a CU whose older workgroup has a run of cost
+// ca and whose younger one cost cb (< 0: none): both at their paired rates
+// until one ends, the other alone after that
+static double pair_time(double ca, double cb)
+{
+    if (cb < 0.0) return ca / kRateAlone;
+    const double ta = ca / kRateOlder, tb = cb / kRateYounger;
+    if (ta <= tb) return ta + (cb - kRateYounger * ta) / kRateAlone;
+    return tb + (ca - kRateOlder * tb) / kRateAlone;
+}
+
+struct ColRef {
+    int dom, i0, j0, ex, ey, planes;
+};
+struct PairedPlan {
+    double time = -1.0;
+    std::vector<ColTile> tiles;   // in tile-index order
+};
+
+// The runs for nb workgroups and target T, or false when the columns do
+// not fit or leave a slot unused.
+static bool paired_fill(const std::vector<ColRef> &cols, int nb, int cus, double T, PairedPlan &out)
+{
+    std::vector<int> blk(nb);
+    for (int b = 0; b < nb; ++b) blk[xcd_tile_host(b, nb)] = b;
+    std::vector<ColTile> tiles(nb);
+    int t = 0;
+    for (const ColRef &c : cols) {
+        const int first = t;
+        int rem = c.planes;
+        while (rem > 0) {
+            if (t >= nb) return false;
+            const int cap = (int)floor(T * slot_rate(blk[t], nb, cus) - kRunStartup);
+            if (cap < 1) return false;
+            rem -= std::min(cap, rem);
+            ++t;
+        }
+        // spread the column's planes over its slots in proportion to their
+        // rates (equal finish times), at least one plane each
+        const int m = t - first;
+        double vsum = 0.0;
+        for (int s = first; s < t; ++s) vsum += slot_rate(blk[s], nb, cus);
+        const double tau = (c.planes + m * kRunStartup) / vsum;
+        std::vector<int> len(m);
+        std::vector<double> frac(m);
+        int tot = 0;
+        for (int s = 0; s < m; ++s) {
+            const double want = std::max(1.0, tau * slot_rate(blk[first + s], nb, cus) - kRunStartup);
+            len[s] = std::max(1, (int)floor(want));
+            frac[s] = want - floor(want);
+            tot += len[s];
+        }
+        while (tot < c.planes) {   // largest remainders first
+            int bi = 0;
+            for (int s = 1; s < m; ++s)
+                if (frac[s] > frac[bi]) bi = s;
+            ++len[bi];
+            frac[bi] = -1.0;
+            ++tot;
+        }
+        while (tot > c.planes) {   // trim the longest runs
+            int bi = 0;
+            for (int s = 1; s < m; ++s)
+                if (len[s] > len[bi]) bi = s;
+            if (len[bi] <= 1) return false;
+            --len[bi];
+            --tot;
+        }
+        for (int s = 0, k0 = 0; s < m; k0 += len[s], ++s)
+            tiles[first + s] = ColTile{c.dom, c.i0, c.j0, k0, c.ex, c.ey,
+                                       (k0 | c.j0 | c.i0) == 0 ? kFirstTile : 0, len[s]};
+    }
+    if (t != nb) return false;
+    std::vector<double> cost(nb, -1.0);
+    for (int s = 0; s < nb; ++s) cost[blk[s]] = tiles[s].kn + kRunStartup;
+    double worst = 0.0;
+    for (int b = 0; b < std::min(nb, cus); ++b)
+        worst = std::max(worst, pair_time(cost[b], b + cus < nb ? cost[b + cus] : -1.0));
+    out.time = worst;
+    out.tiles.swap(tiles);
+    return true;
+}
+
+// The best paired plan over the workgroup counts [columns, 2 CUs], or an
+// empty plan when the columns do not fit one round.
+static PairedPlan paired_plan(const std::vector<ColRef> &cols, int cus)
+{
+    PairedPlan best;
+    const int nc = (int)cols.size();
+    long planes = 0;
+    for (const ColRef &c : cols) planes += c.planes;
+    if (nc == 0 || nc > 2 * cus) return best;
+    for (int nb = nc; nb <= 2 * cus; ++nb) {
+        // least T for which the columns fit in at most nb slots
+        double lo = 0.0, hi = (double)planes + 4.0 * kRunStartup;
+        PairedPlan p;
+        for (int it = 0; it < 50; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            // feasible when the columns fit in nb slots or fewer
+            std::vector<int> blk(nb);
+            for (int b = 0; b < nb; ++b) blk[xcd_tile_host(b, nb)] = b;
+            int t = 0;
+            bool ok = true;
+            for (const ColRef &c : cols) {
+                int rem = c.planes;
+                while (rem > 0 && ok) {
+                    if (t >= nb) { ok = false; break; }
+                    const int cap = (int)floor(mid * slot_rate(blk[t], nb, cus) - kRunStartup);
+                    if (cap < 1) { ok = false; break; }
+                    rem -= std::min(cap, rem);
+                    ++t;
+                }
+                if (!ok) break;
+            }
+            if (ok) hi = mid;
+            else lo = mid;
+        }
+        if (paired_fill(cols, nb, cus, hi, p) && (best.time < 0.0 || p.time < best.time - 1e-9)) best = p;
+    }
+    return best;
+}
+
+// pair-model time of a plan given as tiles in tile-index order (one round)
+static double plan_pair_time(const std::vector<ColTile> &tiles, int cus)
+{
+    const int nb = (int)tiles.size();
+    if (nb > 2 * cus) return -1.0;
+    std::vector<double> cost(nb, -1.0);
+    for (int b = 0; b < nb; ++b) cost[b] = tiles[xcd_tile_host(b, nb)].kn + kRunStartup;
+    double worst = 0.0;
+    for (int b = 0; b < std::min(nb, cus); ++b)
+        worst = std::max(worst, pair_time(cost[b], b + cus < nb ? cost[b + cus] : -1.0));
+    return worst;
+}
+
 static int build_tiles(Engine *e)
 {
     std::vector<Tile> et, nt;
@@ -225,6 +397,7 @@ static int build_tiles(Engine *e)
     int kc = 1;
     bool even = true;
     if (!e->umesh) choose_zruns(e, kc, even);
+    std::vector<ColRef> cols;
     for (int di = 0, gi = 0; gi < (int)e->groups.size() && !e->umesh; ++gi)
         for (auto *d : e->groups[gi]->doms) {
             const int s1 = d->host.nx + 1;
@@ -237,8 +410,27 @@ static int build_tiles(Engine *e)
                     for (int i0 = 0; i0 < s1; i0 += ex - 1)
                         ct.push_back(ColTile{di, i0, j0, k0, ex, ey, (k0 | j0 | i0) == 0 ? kFirstTile : 0,
                                              z.planes(r)});
+            for (int j0 = 0; j0 < s1; j0 += ey - 1)
+                for (int i0 = 0; i0 < s1; i0 += ex - 1) cols.push_back(ColRef{di, i0, j0, ex, ey, s1});
             ++di;
         }
+    e->paired_runs = false;
+    {
+        const char *pr = getenv("LULESH_PAIRED_RUNS");
+        const bool want = !(pr && atoi(pr) == 0);
+        if (want && !e->umesh && e->groups.size() == 1 && e->kchunk == 0 && !ct.empty()) {
+            int cus = 256, dev = 0;
+            if (hipGetDevice(&dev) == hipSuccess)
+                (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            cus = std::max(cus, 1);
+            const PairedPlan p = paired_plan(cols, cus);
+            const double t_even = plan_pair_time(ct, cus);
+            if (p.time > 0.0 && t_even > 0.0 && p.time < t_even - 1e-9) {
+                ct = p.tiles;
+                e->paired_runs = true;
+            }
+        }
+    }
     e->kchunk_used = kc;
     if (e->d_ctiles) (void)hipFree(e->d_ctiles);
     e->d_ctiles = nullptr;

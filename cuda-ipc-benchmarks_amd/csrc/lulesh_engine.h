@@ -118,6 +118,7 @@ struct Engine {
     // into max(1, planes / kchunk) runs of kchunk or kchunk + 1 planes
     int kchunk = 0;
     int kchunk_used = 8;
+    bool paired_runs = false;   // force-march runs balanced for the CU pairs (one round)
     ColTile *d_ctiles = nullptr;
     int n_ctiles = 0;
     // node update done by the force march as each node plane completes

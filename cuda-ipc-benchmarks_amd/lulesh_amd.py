@@ -59,6 +59,7 @@ SIGNATURES = {
     "lulesh_group_sync": (_i, [_vp, _ip, _dp, _dp, _vp]),
     "lulesh_group_verify": (_i, [_vp, _i, _dp]),
     "lulesh_group_profile": (_i, [_vp, _i, _dp]),
+    "lulesh_group_force_plan": (_i, [_vp, _ip, _ip, _ip]),
     "lulesh_batch_profile": (_i, [_vp, _i, _dp]),
     "lulesh_domain_rank": (_i, [_vp]),
     "lulesh_domain_sizes": (_i, [_vp, _ip, _ip, _ip, _ip]),
@@ -420,6 +421,12 @@ class Group:
         ms = np.zeros(7, dtype=np.float64)
         _chk(load().lulesh_group_profile(self.h, ncycles, _p(ms)), "profile")
         return ms
+
+    def force_plan(self):
+        """(workgroups, longest run in node planes, paired) of the force kernel."""
+        v = [ctypes.c_int() for _ in range(3)]
+        _chk(load().lulesh_group_force_plan(self.h, *[ctypes.byref(x) for x in v]), "force_plan")
+        return v[0].value, v[1].value, bool(v[2].value)
 
     def exchange(self, msg_type, fields):
         ids = np.array([FIELDS[f] for f in fields], dtype=np.int32)

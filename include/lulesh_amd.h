@@ -200,6 +200,14 @@ int lulesh_group_sync(lulesh_group *g, int *cycle, double *time,
  * cycles of the run. */
 int lulesh_group_profile(lulesh_group *g, int ncycles, double ms[7]);
 
+/* Launch shape of the group's fused force kernel (a diagnostic; no
+ * counterpart in the reference, whose force kernels are per element and
+ * per node): the number of column-run workgroups, the longest run in node
+ * planes, and whether the runs are the paired plan (longer runs for the
+ * first workgroup of each CU, shorter for the second; see the engine) or
+ * the even split.  An unstructured group reports 0 workgroups. */
+int lulesh_group_force_plan(lulesh_group *g, int *workgroups, int *node_planes_max, int *paired);
+
 /* Final-output figures of VerifyAndWriteFinalOutput
  * (LULESH/omp_4.0/lulesh-util.cc:177-224) for subdomain `rank`:
  * out[0] origin energy e(0), out[1] MaxAbsDiff, out[2] TotalAbsDiff,

@@ -135,3 +135,42 @@ def test_decomposed_inconsistent_copies(amd, oracle_mod, monkeypatch, env):
     for r in range(8):
         for f in NODE + ["e", "p", "q", "v"]:
             assert_bitwise(g.field(f, r), o.field(f, r), f"{env} rank {r} {f}")
+
+
+def test_paired_runs_bitwise(amd, oracle_mod, monkeypatch):
+    """The force kernel's column runs are split by the paired plan (longer
+    runs for the first workgroup of each CU) where it is faster, by the
+    even split otherwise: the split decides which workgroup completes which
+    node planes, never the order of any sum, so both give the oracle's
+    state bit for bit."""
+    o = oracle_mod.Oracle(64, cbrt_mode=oracle_mod.CBRT_EXACT)
+    o.run(25)
+    states = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("LULESH_PAIRED_RUNS", flag)
+        g = amd.Group(64)
+        wg, longest, paired = g.force_plan()
+        assert paired == (flag == "1"), (wg, longest, paired)
+        g.set_max_cycles(25)
+        assert g.run() == 0
+        _compare(g, o, f"paired={flag}")
+        states[flag] = (wg, longest)
+        g.close()
+    o.close()
+    assert states["1"] != states["0"]
+
+
+def test_paired_runs_s100_against_even_split(amd, monkeypatch):
+    """-s 100 (the headline size) on the paired plan and on the even split:
+    every node and element field bit-identical after 60 cycles."""
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("LULESH_PAIRED_RUNS", flag)
+        g = amd.Group(100)
+        assert g.force_plan()[2] == (flag == "1")
+        g.set_max_cycles(60)
+        assert g.run() == 0
+        out[flag] = {f: g.field(f).copy() for f in NODE + ELEM}
+        g.close()
+    for f in NODE + ELEM:
+        assert_bitwise(out["1"][f], out["0"][f], f"s100 paired vs even {f}")
