@@ -35,10 +35,13 @@ namespace lulesh {
 __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColTile *tiles, int ntiles,
                                                         int advance, int store_f, int ti)
 {
-    // corner forces of one phase, in pairs: lds[d][p][t] = component d of
+    // corner forces of one phase, in pairs: lds[d][s][t] = component d of
     // element t's corners p and p+4 (the node-plane-c and node-plane-c+1
-    // corners one node gathers from the same element), one 16-byte access
-    __shared__ double2 lds[3][4][kMThreads];
+    // corners one node gathers from the same element), one 16-byte access,
+    // for p = 0, 1, 3 in slots s = 0, 1, 2.  Corners 2 and 6 belong to the
+    // node the element's own thread gathers: that pair stays in registers.
+    __shared__ double2 lds[3][3][kMThreads];
+    double2 own_c[3];
     __shared__ double sn[2][6][kPatch];          // node planes (x y z xd yd zd), by plane parity
     __shared__ double ti_dt;                     // this cycle's deltatime, or -1: the group stops
     const ColTile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -228,15 +231,23 @@ __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColT
             node_normals16(x, y, z, bx, by, bz);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                lds[0][q][tid] = make_double2(-(sig16 * bx[q]), -(sig16 * bx[q + 4]));
-                lds[1][q][tid] = make_double2(-(sig16 * by[q]), -(sig16 * by[q + 4]));
-                lds[2][q][tid] = make_double2(-(sig16 * bz[q]), -(sig16 * bz[q + 4]));
+                const double2 vx = make_double2(-(sig16 * bx[q]), -(sig16 * bx[q + 4]));
+                const double2 vy = make_double2(-(sig16 * by[q]), -(sig16 * by[q + 4]));
+                const double2 vz = make_double2(-(sig16 * bz[q]), -(sig16 * bz[q + 4]));
+                if (q == 2) {
+                    own_c[0] = vx; own_c[1] = vy; own_c[2] = vz;
+                } else {
+                    const int sl = q == 3 ? 2 : q;
+                    lds[0][sl][tid] = vx; lds[1][sl][tid] = vy; lds[2][sl][tid] = vz;
+                }
             }
         } else {
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
+            for (int d = 0; d < 3; ++d) {
+                own_c[d] = make_double2(0.0, 0.0);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) lds[d][q][tid] = make_double2(0.0, 0.0);
+                for (int q = 0; q < 3; ++q) lds[d][q][tid] = make_double2(0.0, 0.0);
+            }
         }
         __syncthreads();
         // the gathers between barriers are short and latency-bound: they
@@ -246,7 +257,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColT
         if (node_ok) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double2 p2 = lds[d][2][tid], p3 = lds[d][3][t1], p1 = lds[d][1][t2], p0 = lds[d][0][t3];
+                const double2 p2 = own_c[d], p3 = lds[d][2][t1], p1 = lds[d][1][t2], p0 = lds[d][0][t3];
                 if (c >= k0) {
                     double v = ps[d];
                     v += p2.x;
@@ -290,14 +301,18 @@ __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColT
                     vl[q + 4] = Up[3 + d][cs[q]];
                 }
                 hg_force(H, vl, h);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) lds[d][q][tid] = make_double2(h[q], h[q + 4]);
+                own_c[d] = make_double2(h[2], h[6]);
+                lds[d][0][tid] = make_double2(h[0], h[4]);
+                lds[d][1][tid] = make_double2(h[1], h[5]);
+                lds[d][2][tid] = make_double2(h[3], h[7]);
             }
         } else {
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
+            for (int d = 0; d < 3; ++d) {
+                own_c[d] = make_double2(0.0, 0.0);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) lds[d][q][tid] = make_double2(0.0, 0.0);
+                for (int q = 0; q < 3; ++q) lds[d][q][tid] = make_double2(0.0, 0.0);
+            }
         }
         __syncthreads();
         __builtin_amdgcn_s_setprio(2);
@@ -305,7 +320,7 @@ __global__ __launch_bounds__(256, 2) void k_force_march(DomDev *doms, const ColT
             double f[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double2 p2 = lds[d][2][tid], p3 = lds[d][3][t1], p1 = lds[d][1][t2], p0 = lds[d][0][t3];
+                const double2 p2 = own_c[d], p3 = lds[d][2][t1], p1 = lds[d][1][t2], p0 = lds[d][0][t3];
                 if (c >= k0) {
                     double v = ph[d];
                     v += p2.x;

@@ -98,20 +98,23 @@ __device__ __forceinline__ void hg_modes(const double x[8], const double y[8], c
 {
     // the four modes' coordinate sums first, so x..z can die before the
     // volume derivatives and the mode vectors are live together
+    // the twelve sums advance together (corner by corner), so that no add
+    // waits on the one just issued; each is still summed left to right
     double hmx[4], hmy[4], hmz[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        double hx = x[0] * hg_gamma(m, 0), hy = y[0] * hg_gamma(m, 0), hz = z[0] * hg_gamma(m, 0);
-#pragma unroll
-        for (int c = 1; c < 8; ++c) {
-            hx = hx + x[c] * hg_gamma(m, c);
-            hy = hy + y[c] * hg_gamma(m, c);
-            hz = hz + z[c] * hg_gamma(m, c);
-        }
-        hmx[m] = hx;
-        hmy[m] = hy;
-        hmz[m] = hz;
+        hmx[m] = x[0] * hg_gamma(m, 0);
+        hmy[m] = y[0] * hg_gamma(m, 0);
+        hmz[m] = z[0] * hg_gamma(m, 0);
     }
+#pragma unroll
+    for (int c = 1; c < 8; ++c)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            hmx[m] = hmx[m] + x[c] * hg_gamma(m, c);
+            hmy[m] = hmy[m] + y[c] * hg_gamma(m, c);
+            hmz[m] = hmz[m] + z[c] * hg_gamma(m, c);
+        }
     double dvdx[8], dvdy[8], dvdz[8];
     volume_derivs(x, y, z, dvdx, dvdy, dvdz);
     const double volinv = rcp_pos(determ);   // volo * v: positive unless the volume error is raised
@@ -132,17 +135,24 @@ __device__ __forceinline__ double hg_coefficient(double ss, double mass, double 
 
 __device__ __forceinline__ void hg_force(const HgModes &H, const double vl[8], double out[8])
 {
+    // the four mode sums, then the eight corner sums, each advanced in
+    // lockstep with its siblings (the same left-to-right sums)
     double h[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        double acc = H.g[0][m] * vl[0];
+    for (int m = 0; m < 4; ++m) h[m] = H.g[0][m] * vl[0];
 #pragma unroll
-        for (int c = 1; c < 8; ++c) acc = acc + H.g[c][m] * vl[c];
-        h[m] = acc;
-    }
+    for (int c = 1; c < 8; ++c)
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
-        out[c] = H.coefficient * (H.g[c][0] * h[0] + H.g[c][1] * h[1] + H.g[c][2] * h[2] + H.g[c][3] * h[3]);
+        for (int m = 0; m < 4; ++m) h[m] = h[m] + H.g[c][m] * vl[c];
+    double acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = H.g[c][0] * h[0];
+#pragma unroll
+    for (int m = 1; m < 4; ++m)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = acc[c] + H.g[c][m] * h[m];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) out[c] = H.coefficient * acc[c];
 }
 
 }  // namespace lulesh
