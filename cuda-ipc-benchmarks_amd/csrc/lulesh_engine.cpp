@@ -272,35 +272,58 @@ struct PairedPlan {
     std::vector<ColTile> tiles;   // in tile-index order
 };
 
+// Slots handed to the columns for target T: the slots in launch order
+// (tile-index order, each XCD's chunk in turn), the columns in round-robin
+// order, so that each XCD works on a z-slab of every column at once (the
+// column tiles' halo rows and columns then come from its own L2), each
+// slot taking up to floor(T v) - 2 planes of its column.  Returns the
+// slots used, or -1 when the columns do not fit in nb.
+static int paired_assign(const std::vector<ColRef> &cols, int nb, int cus, double T, const std::vector<int> &blk,
+                         std::vector<std::vector<int>> *slots_of)
+{
+    const int nc = (int)cols.size();
+    std::vector<int> rem(nc);
+    for (int c = 0; c < nc; ++c) rem[c] = cols[c].planes;
+    if (slots_of) slots_of->assign(nc, std::vector<int>());
+    int t = 0, active = nc, ci = 0;
+    while (active > 0) {
+        while (rem[ci] == 0) ci = (ci + 1) % nc;
+        if (t >= nb) return -1;
+        const int cap = (int)floor(T * slot_rate(blk[t], nb, cus) - kRunStartup);
+        if (cap < 1) return -1;
+        const int take = std::min(cap, rem[ci]);
+        rem[ci] -= take;
+        if (rem[ci] == 0) --active;
+        if (slots_of) (*slots_of)[ci].push_back(t);
+        ++t;
+        ci = (ci + 1) % nc;
+    }
+    return t;
+}
+
 // The runs for nb workgroups and target T, or false when the columns do
-// not fit or leave a slot unused.
+// not fit or leave a slot unused: each column's planes are spread over its
+// slots in proportion to their rates (equal finish times), at least one
+// plane each, bottom-up in launch order.
 static bool paired_fill(const std::vector<ColRef> &cols, int nb, int cus, double T, PairedPlan &out)
 {
     std::vector<int> blk(nb);
     for (int b = 0; b < nb; ++b) blk[xcd_tile_host(b, nb)] = b;
+    std::vector<std::vector<int>> slots_of;
+    if (paired_assign(cols, nb, cus, T, blk, &slots_of) != nb) return false;
     std::vector<ColTile> tiles(nb);
-    int t = 0;
-    for (const ColRef &c : cols) {
-        const int first = t;
-        int rem = c.planes;
-        while (rem > 0) {
-            if (t >= nb) return false;
-            const int cap = (int)floor(T * slot_rate(blk[t], nb, cus) - kRunStartup);
-            if (cap < 1) return false;
-            rem -= std::min(cap, rem);
-            ++t;
-        }
-        // spread the column's planes over its slots in proportion to their
-        // rates (equal finish times), at least one plane each
-        const int m = t - first;
+    for (size_t ci = 0; ci < cols.size(); ++ci) {
+        const ColRef &c = cols[ci];
+        const std::vector<int> &sl = slots_of[ci];
+        const int m = (int)sl.size();
         double vsum = 0.0;
-        for (int s = first; s < t; ++s) vsum += slot_rate(blk[s], nb, cus);
+        for (int s : sl) vsum += slot_rate(blk[s], nb, cus);
         const double tau = (c.planes + m * kRunStartup) / vsum;
         std::vector<int> len(m);
         std::vector<double> frac(m);
         int tot = 0;
         for (int s = 0; s < m; ++s) {
-            const double want = std::max(1.0, tau * slot_rate(blk[first + s], nb, cus) - kRunStartup);
+            const double want = std::max(1.0, tau * slot_rate(blk[sl[s]], nb, cus) - kRunStartup);
             len[s] = std::max(1, (int)floor(want));
             frac[s] = want - floor(want);
             tot += len[s];
@@ -322,10 +345,9 @@ static bool paired_fill(const std::vector<ColRef> &cols, int nb, int cus, double
             --tot;
         }
         for (int s = 0, k0 = 0; s < m; k0 += len[s], ++s)
-            tiles[first + s] = ColTile{c.dom, c.i0, c.j0, k0, c.ex, c.ey,
-                                       (k0 | c.j0 | c.i0) == 0 ? kFirstTile : 0, len[s]};
+            tiles[sl[s]] = ColTile{c.dom, c.i0, c.j0, k0, c.ex, c.ey, (k0 | c.j0 | c.i0) == 0 ? kFirstTile : 0,
+                                   len[s]};
     }
-    if (t != nb) return false;
     std::vector<double> cost(nb, -1.0);
     for (int s = 0; s < nb; ++s) cost[blk[s]] = tiles[s].kn + kRunStartup;
     double worst = 0.0;
@@ -346,30 +368,16 @@ static PairedPlan paired_plan(const std::vector<ColRef> &cols, int cus)
     for (const ColRef &c : cols) planes += c.planes;
     if (nc == 0 || nc > 2 * cus) return best;
     for (int nb = nc; nb <= 2 * cus; ++nb) {
+        std::vector<int> blk(nb);
+        for (int b = 0; b < nb; ++b) blk[xcd_tile_host(b, nb)] = b;
         // least T for which the columns fit in at most nb slots
         double lo = 0.0, hi = (double)planes + 4.0 * kRunStartup;
-        PairedPlan p;
         for (int it = 0; it < 50; ++it) {
             const double mid = 0.5 * (lo + hi);
-            // feasible when the columns fit in nb slots or fewer
-            std::vector<int> blk(nb);
-            for (int b = 0; b < nb; ++b) blk[xcd_tile_host(b, nb)] = b;
-            int t = 0;
-            bool ok = true;
-            for (const ColRef &c : cols) {
-                int rem = c.planes;
-                while (rem > 0 && ok) {
-                    if (t >= nb) { ok = false; break; }
-                    const int cap = (int)floor(mid * slot_rate(blk[t], nb, cus) - kRunStartup);
-                    if (cap < 1) { ok = false; break; }
-                    rem -= std::min(cap, rem);
-                    ++t;
-                }
-                if (!ok) break;
-            }
-            if (ok) hi = mid;
+            if (paired_assign(cols, nb, cus, mid, blk, nullptr) >= 0) hi = mid;
             else lo = mid;
         }
+        PairedPlan p;
         if (paired_fill(cols, nb, cus, hi, p) && (best.time < 0.0 || p.time < best.time - 1e-9)) best = p;
     }
     return best;
