@@ -334,14 +334,19 @@ def run_sequential(mix, idx):
     zc, t, res = 0, 0.0, {}
     for i in idx:
         g = lulesh_amd.Group(int(nx[i]), 1, int(nr[i]), int(b[i]), int(c[i]))
+        # the engine is built and its graph captured by a first, untimed
+        # chunk of real cycles, as the batched and served forms' are before
+        # their timed regions; the timed part is the rest of the run
+        g.enqueue(64)
+        rc0, c0, _, _ = g.sync()
         t0 = time.perf_counter()
-        rc = g.run(64)
+        rc = g.run(64) if rc0 == 0 else rc0
         t += time.perf_counter() - t0
         if rc != 0:
             raise SystemExit(f"sequential run {i} failed: {rc}")
         cyc = g.sync()[1]
         res[int(i)] = (cyc, g.verify()["origin_energy"])
-        zc += int(nx[i]) ** 3 * cyc
+        zc += int(nx[i]) ** 3 * (cyc - c0)
         g.close()
     return zc, t, res
 

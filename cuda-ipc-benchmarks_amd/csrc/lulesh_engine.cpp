@@ -26,6 +26,8 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <map>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <vector>
@@ -276,110 +278,164 @@ struct PairedPlan {
 // (tile-index order, each XCD's chunk in turn), the columns in round-robin
 // order, so that each XCD works on a z-slab of every column at once (the
 // column tiles' halo rows and columns then come from its own L2), each
-// slot taking up to floor(T v) - 2 planes of its column.  Returns the
-// slots used, or -1 when the columns do not fit in nb.
+// slot taking up to floor(T v) - 2 planes of its column.  Records each
+// slot's column when col_of is given.  Returns the slots used, or -1 when
+// the columns do not fit in nb.
 static int paired_assign(const std::vector<ColRef> &cols, int nb, int cus, double T, const std::vector<int> &blk,
-                         std::vector<std::vector<int>> *slots_of)
+                         std::vector<int> *col_of)
 {
     const int nc = (int)cols.size();
-    std::vector<int> rem(nc);
+    // the three capacities (slot classes: older, younger, alone)
+    const int cap_o = (int)floor(T * kRateOlder - kRunStartup);
+    const int cap_y = (int)floor(T * kRateYounger - kRunStartup);
+    const int cap_a = (int)floor(T * kRateAlone - kRunStartup);
+    thread_local std::vector<int> rem;
+    rem.resize(nc);
     for (int c = 0; c < nc; ++c) rem[c] = cols[c].planes;
-    if (slots_of) slots_of->assign(nc, std::vector<int>());
+    if (col_of) col_of->assign(nb, -1);
     int t = 0, active = nc, ci = 0;
     while (active > 0) {
-        while (rem[ci] == 0) ci = (ci + 1) % nc;
+        while (rem[ci] == 0) ci = ci + 1 == nc ? 0 : ci + 1;
         if (t >= nb) return -1;
-        const int cap = (int)floor(T * slot_rate(blk[t], nb, cus) - kRunStartup);
+        const int b = blk[t];
+        const int cap = b >= cus ? cap_y : (b + cus < nb ? cap_o : cap_a);
         if (cap < 1) return -1;
         const int take = std::min(cap, rem[ci]);
         rem[ci] -= take;
         if (rem[ci] == 0) --active;
-        if (slots_of) (*slots_of)[ci].push_back(t);
+        if (col_of) (*col_of)[t] = ci;
         ++t;
-        ci = (ci + 1) % nc;
+        ci = ci + 1 == nc ? 0 : ci + 1;
     }
     return t;
 }
 
-// The runs for nb workgroups and target T, or false when the columns do
-// not fit or leave a slot unused: each column's planes are spread over its
-// slots in proportion to their rates (equal finish times), at least one
-// plane each, bottom-up in launch order.
-static bool paired_fill(const std::vector<ColRef> &cols, int nb, int cus, double T, PairedPlan &out)
+// The pair-model time of the runs for nb workgroups and target T (-1 when
+// the columns do not fit or leave a slot unused), and the tiles when asked:
+// each column's planes are spread over its slots in proportion to their
+// rates (equal finish times), at least one plane each, bottom-up in launch
+// order.
+static double paired_fill(const std::vector<ColRef> &cols, int nb, int cus, double T, const std::vector<int> &blk,
+                          std::vector<ColTile> *tiles)
 {
-    std::vector<int> blk(nb);
-    for (int b = 0; b < nb; ++b) blk[xcd_tile_host(b, nb)] = b;
-    std::vector<std::vector<int>> slots_of;
-    if (paired_assign(cols, nb, cus, T, blk, &slots_of) != nb) return false;
-    std::vector<ColTile> tiles(nb);
-    for (size_t ci = 0; ci < cols.size(); ++ci) {
-        const ColRef &c = cols[ci];
-        const std::vector<int> &sl = slots_of[ci];
-        const int m = (int)sl.size();
+    thread_local std::vector<int> col_of, start, order, len;
+    thread_local std::vector<double> frac, cost;
+    if (paired_assign(cols, nb, cus, T, blk, &col_of) != nb) return -1.0;
+    const int nc = (int)cols.size();
+    // each column's slots, in launch order (counting sort by column)
+    start.assign(nc + 1, 0);
+    for (int t = 0; t < nb; ++t) ++start[col_of[t] + 1];
+    for (int c = 0; c < nc; ++c) start[c + 1] += start[c];
+    order.resize(nb);
+    {
+        thread_local std::vector<int> pos;
+        pos.assign(start.begin(), start.end() - 1);
+        for (int t = 0; t < nb; ++t) order[pos[col_of[t]]++] = t;
+    }
+    len.assign(nb, 0);
+    frac.assign(nb, 0.0);
+    for (int c = 0; c < nc; ++c) {
+        const int f = start[c], m = start[c + 1] - f, planes = cols[c].planes;
         double vsum = 0.0;
-        for (int s : sl) vsum += slot_rate(blk[s], nb, cus);
-        const double tau = (c.planes + m * kRunStartup) / vsum;
-        std::vector<int> len(m);
-        std::vector<double> frac(m);
+        for (int s = f; s < f + m; ++s) vsum += slot_rate(blk[order[s]], nb, cus);
+        const double tau = (planes + m * kRunStartup) / vsum;
         int tot = 0;
-        for (int s = 0; s < m; ++s) {
-            const double want = std::max(1.0, tau * slot_rate(blk[sl[s]], nb, cus) - kRunStartup);
+        for (int s = f; s < f + m; ++s) {
+            const double want = std::max(1.0, tau * slot_rate(blk[order[s]], nb, cus) - kRunStartup);
             len[s] = std::max(1, (int)floor(want));
             frac[s] = want - floor(want);
             tot += len[s];
         }
-        while (tot < c.planes) {   // largest remainders first
-            int bi = 0;
-            for (int s = 1; s < m; ++s)
+        while (tot < planes) {   // largest remainders first
+            int bi = f;
+            for (int s = f + 1; s < f + m; ++s)
                 if (frac[s] > frac[bi]) bi = s;
             ++len[bi];
             frac[bi] = -1.0;
             ++tot;
         }
-        while (tot > c.planes) {   // trim the longest runs
-            int bi = 0;
-            for (int s = 1; s < m; ++s)
+        while (tot > planes) {   // trim the longest runs
+            int bi = f;
+            for (int s = f + 1; s < f + m; ++s)
                 if (len[s] > len[bi]) bi = s;
-            if (len[bi] <= 1) return false;
+            if (len[bi] <= 1) return -1.0;
             --len[bi];
             --tot;
         }
-        for (int s = 0, k0 = 0; s < m; k0 += len[s], ++s)
-            tiles[sl[s]] = ColTile{c.dom, c.i0, c.j0, k0, c.ex, c.ey, (k0 | c.j0 | c.i0) == 0 ? kFirstTile : 0,
-                                   len[s]};
     }
-    std::vector<double> cost(nb, -1.0);
-    for (int s = 0; s < nb; ++s) cost[blk[s]] = tiles[s].kn + kRunStartup;
+    cost.assign(nb, -1.0);
+    for (int s = 0; s < nb; ++s) cost[blk[order[s]]] = len[s] + kRunStartup;
     double worst = 0.0;
     for (int b = 0; b < std::min(nb, cus); ++b)
         worst = std::max(worst, pair_time(cost[b], b + cus < nb ? cost[b + cus] : -1.0));
-    out.time = worst;
-    out.tiles.swap(tiles);
-    return true;
+    if (tiles) {
+        tiles->assign(nb, ColTile{});
+        for (int c = 0; c < nc; ++c) {
+            const ColRef &cr = cols[c];
+            for (int s = start[c], k0 = 0; s < start[c + 1]; k0 += len[s], ++s)
+                (*tiles)[order[s]] = ColTile{cr.dom, cr.i0, cr.j0, k0, cr.ex, cr.ey,
+                                             (k0 | cr.j0 | cr.i0) == 0 ? kFirstTile : 0, len[s]};
+        }
+    }
+    return worst;
+}
+
+// The least T for which the columns fit in nb slots (bisection; a column
+// alone on the slowest slot bounds it).
+static double paired_target(const std::vector<ColRef> &cols, int nb, int cus, int longest,
+                            const std::vector<int> &blk)
+{
+    double lo = 0.0, hi = (longest + kRunStartup) / kRateYounger + 1.0;
+    for (int it = 0; it < 16; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (paired_assign(cols, nb, cus, mid, blk, nullptr) >= 0) hi = mid;
+        else lo = mid;
+    }
+    return hi;
 }
 
 // The best paired plan over the workgroup counts [columns, 2 CUs], or an
-// empty plan when the columns do not fit one round.
+// empty plan when the columns do not fit one round.  Plans are kept per
+// column list: a process that builds engines for the same problem shape
+// searches once.
 static PairedPlan paired_plan(const std::vector<ColRef> &cols, int cus)
 {
     PairedPlan best;
     const int nc = (int)cols.size();
-    long planes = 0;
-    for (const ColRef &c : cols) planes += c.planes;
+    int longest = 0;
+    for (const ColRef &c : cols) longest = std::max(longest, c.planes);
     if (nc == 0 || nc > 2 * cus) return best;
-    for (int nb = nc; nb <= 2 * cus; ++nb) {
-        std::vector<int> blk(nb);
-        for (int b = 0; b < nb; ++b) blk[xcd_tile_host(b, nb)] = b;
-        // least T for which the columns fit in at most nb slots
-        double lo = 0.0, hi = (double)planes + 4.0 * kRunStartup;
-        for (int it = 0; it < 50; ++it) {
-            const double mid = 0.5 * (lo + hi);
-            if (paired_assign(cols, nb, cus, mid, blk, nullptr) >= 0) hi = mid;
-            else lo = mid;
-        }
-        PairedPlan p;
-        if (paired_fill(cols, nb, cus, hi, p) && (best.time < 0.0 || p.time < best.time - 1e-9)) best = p;
+    std::vector<int> key{cus};
+    for (const ColRef &c : cols) key.insert(key.end(), {c.dom, c.i0, c.j0, c.ex, c.ey, c.planes});
+    static std::mutex mu;
+    static std::map<std::vector<int>, PairedPlan> cache;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
     }
+    std::vector<int> blk;
+    int best_nb = -1;
+    double best_T = 0.0;
+    for (int nb = nc; nb <= 2 * cus; ++nb) {
+        blk.resize(nb);
+        for (int b = 0; b < nb; ++b) blk[xcd_tile_host(b, nb)] = b;
+        const double T = paired_target(cols, nb, cus, longest, blk);
+        const double t = paired_fill(cols, nb, cus, T, blk, nullptr);
+        if (t > 0.0 && (best.time < 0.0 || t < best.time - 1e-9)) {
+            best.time = t;
+            best_nb = nb;
+            best_T = T;
+        }
+    }
+    if (best_nb > 0) {
+        blk.resize(best_nb);
+        for (int b = 0; b < best_nb; ++b) blk[xcd_tile_host(b, best_nb)] = b;
+        paired_fill(cols, best_nb, cus, best_T, blk, &best.tiles);
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() > 256) cache.clear();
+    cache[key] = best;
     return best;
 }
 
