@@ -841,7 +841,13 @@ __device__ __forceinline__ EosOut eos_rep(double vnewc, double e_old, double del
     const double c1s = 2.0 / 3.0, sixth = 1.0 / 6.0;
     const double compression = eos_compression(vnewc);
     const double vchalf = vnewc - delvc * .5;
-    double compHalfStep = 1. / vchalf - 1.;
+    // 1 / vchalf as the range-limited reciprocal: its value is used only
+    // while eosvmin < vnewc < eosvmax, where vnewc is the kinematics' vnew
+    // and delvc = vnew - v of the same cycle, so vchalf is (vnew + v) / 2
+    // to rounding, with v (the last cycle's clamped volume) at least
+    // eosvmin: far inside rcp_pos's range.  Outside that window the value
+    // is replaced just below.
+    double compHalfStep = rcp_pos(vchalf) - 1.;
     if (vnewc <= Phys::eosvmin) compHalfStep = compression;
     if (vnewc >= Phys::eosvmax) { p_old = 0.; compHalfStep = 0.; }
     const double work = 0.;
@@ -853,7 +859,8 @@ __device__ __forceinline__ EosOut eos_rep(double vnewc, double e_old, double del
     if (fabs(pHalfStep) < Phys::p_cut) pHalfStep = 0.0;
     if (vnewc >= Phys::eosvmax) pHalfStep = 0.0;
     pHalfStep = raise_to(pHalfStep, Phys::pmin);
-    const double vhalf = 1. / (1. + compHalfStep);
+    // 1 + compHalfStep is about 1 / vchalf, about 1 / eosvmin, or 1
+    const double vhalf = rcp_pos(1. + compHalfStep);
     double q_new;
     if (delvc > 0.) {
         q_new = 0.;
