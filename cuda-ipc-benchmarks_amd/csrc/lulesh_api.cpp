@@ -1579,6 +1579,50 @@ double lulesh_probe_copy(size_t n, int reps)
     return ms / reps;
 }
 
+int lulesh_probe_fp64(int waves_per_simd, int one_chain, int iters, double *tflops, double *clock_mhz)
+{
+    if (waves_per_simd < 1 || waves_per_simd > 8 || (one_chain != 0 && one_chain != 1) || iters < 1 ||
+        iters > 1000000 || !tflops || !clock_mhz)
+        return LULESH_EINVAL;
+    int dev = 0, cus = 0, wall_khz = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev));
+    if (cus < 1 || wall_khz < 1) return LULESH_EHIP;
+    // w workgroups of four waves per CU: w waves on each of its four SIMDs
+    const int grid = cus * waves_per_simd;
+    double *out = nullptr;
+    unsigned long long *clk = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<unsigned long long> h((size_t)2 * grid);
+    float ms = 0.f;
+    hipError_t e = hipMalloc((void **)&out, (size_t)grid * 256 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&clk, h.size() * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = launch_probe_fp64(out, clk, grid, iters, one_chain != 0, 0);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, 0);
+    if (e == hipSuccess) e = launch_probe_fp64(out, clk, grid, iters, one_chain != 0, 0);
+    if (e == hipSuccess) e = hipEventRecord(e1, 0);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), clk, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (out) (void)hipFree(out);
+    if (clk) (void)hipFree(clk);
+    if (e != hipSuccess) return LULESH_EHIP;
+    double cycles = 0.0, ticks = 0.0;
+    for (int b = 0; b < grid; ++b) {
+        cycles += (double)h[2 * b];
+        ticks += (double)h[2 * b + 1];
+    }
+    if (ms <= 0.f || ticks <= 0.0) return LULESH_EHIP;
+    *tflops = (double)grid * 256 * iters * 16 / (ms * 1e-3) / 1e12;
+    *clock_mhz = cycles / ticks * wall_khz / 1000.0;
+    return LULESH_OK;
+}
+
 int lulesh_probe_range_math(const double *a, double *out2, int n)
 {
     if (n < 0 || (n > 0 && (!a || !out2))) return LULESH_EINVAL;

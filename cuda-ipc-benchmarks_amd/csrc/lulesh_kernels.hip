@@ -1416,6 +1416,43 @@ __global__ __launch_bounds__(256) void k_probe_copy(const double2 *__restrict__ 
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += stride) b[t] = a[t];
 }
 
+// FP64 issue rate: 16 multiplies and adds per loop trip and lane, as eight
+// independent chains or (ONE) as a single chain where each operation reads
+// the one before (no memory traffic, no FMA); thread 0 of each workgroup
+// records the shader cycles and the 100 MHz real-time ticks its loop took
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_probe_fp64(double *out, int iters, unsigned long long *clk)
+{
+    double a[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) a[c] = 1.0 + 1e-9 * threadIdx.x + 0.125 * c;
+    const double m = 0.9999999999, s = 1e-13;
+    const unsigned long long t0 = __builtin_readcyclecounter(), r0 = wall_clock64();
+    for (int i = 0; i < iters; ++i) {
+        if (ONE) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                a[0] = a[0] * m;
+                a[0] = a[0] + s;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) a[c] = a[c] * m;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) a[c] = a[c] + s;
+        }
+    }
+    const unsigned long long t1 = __builtin_readcyclecounter(), r1 = wall_clock64();
+    if (threadIdx.x == 0) {
+        clk[2 * blockIdx.x] = t1 - t0;
+        clk[2 * blockIdx.x + 1] = r1 - r0;
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) t += a[c];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = t;
+}
+
 // message i is copied by the workgroups of grid row i
 __global__ __launch_bounds__(256) void k_copy_msgs(CopyMsgs m)
 {
@@ -1442,6 +1479,16 @@ hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s
 {
     const size_t n2 = n / 2;
     hipLaunchKernelGGL(k_probe_copy, dim3(4096), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n2);
+    return hipGetLastError();
+}
+
+// out: grid x 256 doubles, clk: 2 x grid counters
+hipError_t launch_probe_fp64(double *out, unsigned long long *clk, int grid, int iters, bool one_chain,
+                             hipStream_t s)
+{
+    if (grid < 1 || iters < 1) return hipErrorInvalidValue;
+    if (one_chain) hipLaunchKernelGGL(k_probe_fp64<true>, dim3(grid), dim3(256), 0, s, out, iters, clk);
+    else hipLaunchKernelGGL(k_probe_fp64<false>, dim3(grid), dim3(256), 0, s, out, iters, clk);
     return hipGetLastError();
 }
 

@@ -109,6 +109,7 @@ SIGNATURES = {
     "lulesh_probe_copy": (_d, [_sz, _i]),
     "lulesh_probe_sqrt_div": (_i, [_dp, _dp, _dp, _i]),
     "lulesh_probe_range_math": (_i, [_dp, _dp, _i]),
+    "lulesh_probe_fp64": (_i, [_i, _i, _i, _dp, _dp]),
 }
 
 _lib = None
@@ -592,6 +593,18 @@ def probe_copy_ms(n_doubles, reps=20):
     if ms < 0:
         raise LuleshError(ENOMEM, "probe_copy")
     return ms
+
+
+def probe_fp64(waves_per_simd, iters=20000, one_chain=False):
+    """(TFLOP/s, MHz): the FP64 rate 16 multiplies and adds per loop trip
+    and lane reach, as eight independent chains or as one dependent chain,
+    with `waves_per_simd` waves on every SIMD and no memory traffic, and the
+    mean shader clock over their loops."""
+    init()
+    tf, mhz = ctypes.c_double(), ctypes.c_double()
+    _chk(load().lulesh_probe_fp64(int(waves_per_simd), 1 if one_chain else 0, int(iters), ctypes.byref(tf),
+                                  ctypes.byref(mhz)), "probe_fp64")
+    return tf.value, mhz.value
 
 
 def probe_sqrt_div(a, b):

@@ -481,6 +481,14 @@ int lulesh_probe_sqrt_div(const double *a, const double *b, double *out2, int n)
  * kinematics kernels form them where the physics bounds the argument
  * (sqrt: 2^-767 <= a[i] < inf; 1/x: 2^-767 <= |a[i]| < 2^1021) */
 int lulesh_probe_range_math(const double *a, double *out2, int n);
+/* FP64 issue ceiling (a diagnostic with no counterpart in the reference):
+ * a kernel of 16 FP64 multiplies and adds per loop trip and lane, as eight
+ * independent chains (one_chain 0) or one dependent chain (one_chain 1),
+ * with waves_per_simd (1-8) waves on every SIMD and no memory traffic, runs
+ * `iters` (1-10^6) loop trips; reports the FP64 rate it reached in TFLOP/s
+ * and the mean shader clock over its loops in MHz.  The measured-FP64
+ * yardstick of the profiling runs, beside lulesh_probe_copy. */
+int lulesh_probe_fp64(int waves_per_simd, int one_chain, int iters, double *tflops, double *clock_mhz);
 
 #ifdef __cplusplus
 }

@@ -80,6 +80,25 @@ def test_probe_range_math(amd):
     assert_bitwise(r, 1.0 / -a[:1000], "rcp_pos negative")
 
 
+def test_probe_fp64_issue_rate(amd):
+    # the FP64 yardstick: a sane rate under the no-FMA half of the vector
+    # peak (78.6 TFLOP/s counts an FMA as two), a plausible shader clock,
+    # and more waves per SIMD never issuing slower
+    rates = {}
+    for w in (1, 2, 8):
+        tf, mhz = amd.probe_fp64(w, 5000)
+        assert 1.0 < tf < 39.3 * 1.05, (w, tf)
+        assert 500.0 < mhz < 3500.0, (w, mhz)
+        rates[w] = tf
+    assert rates[8] > rates[1] * 0.95, rates
+    tf, mhz = amd.probe_fp64(2, 5000, one_chain=True)
+    assert 1.0 < tf < 39.3 * 1.05 and 500.0 < mhz < 3500.0, (tf, mhz)
+    with pytest.raises(amd.LuleshError):
+        amd.probe_fp64(0)
+    with pytest.raises(amd.LuleshError):
+        amd.probe_fp64(9)
+
+
 def _compare_state(g, o, what):
     for f in NODE:
         assert_bitwise(g.field(f), o.field(f), f"{what} {f}")
