@@ -914,6 +914,14 @@ int lulesh_init(int device)
     return scratch_base(&sc);
 }
 
+int lulesh_device_cus(void)
+{
+    int dev = 0, cus = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return cus > 0 ? cus : LULESH_EHIP;
+}
+
 const char *lulesh_status_string(int status)
 {
     switch (status) {

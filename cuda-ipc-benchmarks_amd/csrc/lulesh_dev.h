@@ -21,6 +21,8 @@
 namespace lulesh {
 
 constexpr int kCornerComps = 48;     // 6 components x 8 corners
+// node slots per plane of the force kernel's LDS patch: (ex+1)*(ey+1) must fit
+constexpr int kForcePatch = 320;
 
 struct IdxSpan {          // [first, first + count) of a list
     int first;

@@ -13,8 +13,6 @@ hipError_t launch_node_gather(bool umesh, bool fuse, const DomDev *d, const Tile
                               hipStream_t s);
 // fused force + ordered gather (column tiles of at most 256 threads, each
 // over its own run of node planes)
-// node slots per plane of the force kernel's LDS patch: (ex+1)*(ey+1) must fit
-constexpr int kForcePatch = 320;
 // advance: also the node update (old -> new node-state buffer) as each
 // node plane completes; store_f: write fx..fz; ti: the cycle's time
 // increment is computed in the kernel (no time-increment launch before it)

@@ -8,6 +8,7 @@
 
 #include "lulesh_amd.h"
 #include "lulesh_comm.h"
+#include "lulesh_plan.h"
 #include "lulesh_setup.h"
 
 using namespace lulesh;
@@ -50,6 +51,31 @@ int lulesh_region_layout(int nx, int tp, int rank, int nr, int balance, int cost
         if (region) memcpy(region, h.region.data(), h.region.size() * sizeof(int));
         if (rep) memcpy(rep, h.elemRep.data(), h.elemRep.size() * sizeof(int));
         return h.numElem;
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
+    }
+}
+
+int lulesh_force_plan_of(int nx, int tp, int cus, int *kchunk_used, int *paired, int *tiles, size_t n)
+{
+    if (!decomp_ok(nx, tp, 0) || cus < 1 || cus > 65536) return LULESH_EINVAL;
+    try {
+        ForcePlanInput in;
+        in.group_nx.assign(1, std::vector<int>((size_t)tp * tp * tp, nx));
+        in.cus = cus;
+        in.paired_allowed = paired_runs_wanted();
+        ForcePlan fp;
+        const int rc = plan_force_columns(in, fp);
+        if (rc) return rc;
+        if (kchunk_used) *kchunk_used = fp.kchunk_used;
+        if (paired) *paired = fp.paired ? 1 : 0;
+        if (tiles)
+            for (size_t t = 0; t < fp.tiles.size() && t < n; ++t) {
+                const ColTile &c = fp.tiles[t];
+                const int row[8] = {c.dom, c.i0, c.j0, c.k0, c.ex, c.ey, c.flags, c.kn};
+                memcpy(tiles + 8 * t, row, sizeof(row));
+            }
+        return (int)fp.tiles.size();
     } catch (const std::bad_alloc &) {
         return LULESH_ENOMEM;
     }

@@ -32,6 +32,7 @@ _pvp = ctypes.POINTER(ctypes.c_void_p)
 # name -> (restype, argtypes); every extern "C" symbol of the header
 SIGNATURES = {
     "lulesh_init": (_i, [_i]),
+    "lulesh_device_cus": (_i, []),
     "lulesh_status_string": (ctypes.c_char_p, [_i]),
     "lulesh_version": (ctypes.c_char_p, []),
     "lulesh_group_create": (_i, [_i, _i, _i, _i, _i, _pvp]),
@@ -79,6 +80,7 @@ SIGNATURES = {
     "lulesh_shm_recv_offset": (_i, [_i, _i, _i, _i, _i, _i, _i, _i, _i]),
     "lulesh_comm_buf_size_of": (_i, [_i, _i, _i]),
     "lulesh_region_layout": (_i, [_i, _i, _i, _i, _i, _i, _ip, _ip, _sz]),
+    "lulesh_force_plan_of": (_i, [_i, _i, _i, _ip, _ip, _ip, _sz]),
     "lulesh_domain_get_eos_table": (_i, [_vp, ctypes.POINTER(ctypes.c_uint16), _sz]),
     "lulesh_device_local_backend": (_vp, []),
     "lulesh_device_copy_backend": (_vp, []),
@@ -576,6 +578,27 @@ def region_layout(nx, tp=1, rank=0, nr=11, balance=1, cost=1):
     _chk(load().lulesh_region_layout(nx, tp, rank, nr, balance, cost, _p(reg, ctypes.c_int),
                                      _p(rep, ctypes.c_int), ne), "region_layout")
     return reg, rep
+
+
+def device_cus():
+    """CUs of the current device (the engine's planning input)."""
+    init()
+    return _chk(load().lulesh_device_cus(), "device_cus")
+
+
+def force_plan_of(nx, tp=1, cus=256):
+    """(kchunk_used, paired, tiles): the force kernel's column tiles of a
+    group of tp^3 subdomains of nx^3 elements on a device of `cus` CUs, as
+    the engine plans them (host only).  tiles is an (n, 8) int32 array of
+    subdomain, i0, j0, k0, ex, ey, flags, kn in launch order."""
+    L = load()
+    n = _chk(L.lulesh_force_plan_of(nx, tp, cus, None, None, None, 0), "force_plan_of")
+    tiles = np.zeros((max(n, 1), 8), dtype=np.int32)
+    kc, pr = ctypes.c_int(), ctypes.c_int()
+    m = _chk(L.lulesh_force_plan_of(nx, tp, cus, ctypes.byref(kc), ctypes.byref(pr), _p(tiles, ctypes.c_int),
+                                    n), "force_plan_of")
+    assert m == n
+    return kc.value, bool(pr.value), tiles[:n]
 
 
 def probe_cbrt(x):

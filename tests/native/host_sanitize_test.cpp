@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "lulesh_amd.h"
@@ -129,6 +130,39 @@ static void setup_and_plans()
         }
 }
 
+// the force kernel's column planner over sizes, decompositions and CU
+// counts: every node plane of every column in exactly one run
+static void force_plans()
+{
+    std::vector<int> t;
+    for (int cus : {1, 8, 80, 256, 304})
+        for (int tp = 1; tp <= 2; ++tp)
+            for (int nx : {1, 2, 3, 5, 8, 13, 16, 31, 45, 48, 64, 100, 130}) {
+                const int n = lulesh_force_plan_of(nx, tp, cus, nullptr, nullptr, nullptr, 0);
+                EXPECT(n > 0);
+                if (n <= 0) continue;
+                t.assign((size_t)8 * n, -1);
+                int kc = 0, paired = -1;
+                EXPECT(lulesh_force_plan_of(nx, tp, cus, &kc, &paired, t.data(), n) == n);
+                EXPECT(kc >= 1 && (paired == 0 || paired == 1));
+                const int s1 = nx + 1;
+                std::vector<int> covered((size_t)tp * tp * tp * s1 * s1 * s1, 0);
+                for (int i = 0; i < n; ++i) {
+                    const int *r = &t[(size_t)8 * i];
+                    EXPECT(r[0] >= 0 && r[0] < tp * tp * tp && r[7] >= 1 && r[3] + r[7] <= s1);
+                    if (!(r[0] >= 0 && r[0] < tp * tp * tp && r[7] >= 1 && r[3] >= 0 && r[3] + r[7] <= s1)) continue;
+                    for (int k = r[3]; k < r[3] + r[7]; ++k)
+                        for (int j = r[2]; j < std::min(s1, r[2] + r[5] - 1); ++j)
+                            for (int i2 = r[1]; i2 < std::min(s1, r[1] + r[4] - 1); ++i2)
+                                ++covered[(((size_t)r[0] * s1 + k) * s1 + j) * s1 + i2];
+                }
+                for (int c : covered) EXPECT(c == 1);
+            }
+    EXPECT(lulesh_force_plan_of(0, 1, 256, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
+    EXPECT(lulesh_force_plan_of(8, 5, 256, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
+    EXPECT(lulesh_force_plan_of(8, 1, 0, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
+}
+
 static void oracle_runs()
 {
     lo_group *g = lo_create(4, 1, 11, 1, 1, LO_CBRT_EXACT);
@@ -155,6 +189,7 @@ int main(int argc, char **argv)
 {
     meshes(argc, argv);
     setup_and_plans();
+    force_plans();
     oracle_runs();
     printf("host_sanitize_test: %s\n", failures ? "FAILED" : "clean");
     return failures ? 1 : 0;

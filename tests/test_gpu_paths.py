@@ -160,6 +160,18 @@ def test_paired_runs_bitwise(amd, oracle_mod, monkeypatch):
     assert states["1"] != states["0"]
 
 
+@pytest.mark.parametrize("nx,tp", [(8, 1), (30, 1), (45, 2), (48, 1), (64, 1), (100, 1), (150, 1)])
+def test_engine_plan_is_the_host_plan(amd, nx, tp):
+    """The engine's force-kernel launch is the plan the host-only planner
+    gives for this device's CU count (whose every-plane-once invariants the
+    CPU tests check over all sizes): same workgroup count, longest run and
+    choice of the paired plan."""
+    kc, paired, t = amd.force_plan_of(nx, tp, amd.device_cus())
+    g = amd.Group(nx, tp)
+    assert g.force_plan() == (len(t), int(t[:, 7].max()), paired)
+    g.close()
+
+
 def test_paired_runs_s100_against_even_split(amd, monkeypatch):
     """-s 100 (the headline size) on the paired plan and on the even split:
     every node and element field bit-identical after 60 cycles."""

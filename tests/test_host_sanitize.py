@@ -1,10 +1,11 @@
 """The library's host-only code (the .lmesh reader, set-up, region layout,
-exchange plans) and the oracle, built with AddressSanitizer and
-UndefinedBehaviorSanitizer (tests/native/host_sanitize_test, g++ and gcc
-with -fsanitize=address,undefined; CPU only) and run on the reference's own
-mesh plus malformed files: truncated mid-record, headers claiming huge
-counts, node ids out of range, garbage.  Clean = exit status 0 and no
-sanitizer report."""
+exchange plans, the force kernel's column planner) and the oracle, built
+with AddressSanitizer and UndefinedBehaviorSanitizer
+(tests/native/host_sanitize_test, g++ and gcc with
+-fsanitize=address,undefined; CPU only) and run on the reference's own mesh
+plus malformed files: truncated mid-record, headers claiming huge counts,
+node ids out of range, garbage.  Clean = exit status 0 and no sanitizer
+report."""
 import os
 import subprocess
 
