@@ -56,14 +56,19 @@ int lulesh_region_layout(int nx, int tp, int rank, int nr, int balance, int cost
     }
 }
 
-int lulesh_force_plan_of(int nx, int tp, int cus, int *kchunk_used, int *paired, int *tiles, size_t n)
+int lulesh_force_plan_of(int ngroups, const int *nx, const int *tp, int kchunk, int cus, int *kchunk_used,
+                         int *paired, int *tiles, size_t n)
 {
-    if (!decomp_ok(nx, tp, 0) || cus < 1 || cus > 65536) return LULESH_EINVAL;
+    if (ngroups < 1 || ngroups > 65536 || !nx || !tp || kchunk < 0 || kchunk > 4096 || cus < 1 || cus > 65536)
+        return LULESH_EINVAL;
+    for (int g = 0; g < ngroups; ++g)
+        if (!decomp_ok(nx[g], tp[g], 0)) return LULESH_EINVAL;
     try {
         ForcePlanInput in;
-        in.group_nx.assign(1, std::vector<int>((size_t)tp * tp * tp, nx));
+        for (int g = 0; g < ngroups; ++g) in.group_nx.emplace_back((size_t)tp[g] * tp[g] * tp[g], nx[g]);
+        in.kchunk = kchunk;
         in.cus = cus;
-        in.paired_allowed = paired_runs_wanted();
+        in.paired_allowed = paired_runs_wanted() && ngroups == 1 && kchunk == 0;
         ForcePlan fp;
         const int rc = plan_force_columns(in, fp);
         if (rc) return rc;

@@ -80,7 +80,7 @@ SIGNATURES = {
     "lulesh_shm_recv_offset": (_i, [_i, _i, _i, _i, _i, _i, _i, _i, _i]),
     "lulesh_comm_buf_size_of": (_i, [_i, _i, _i]),
     "lulesh_region_layout": (_i, [_i, _i, _i, _i, _i, _i, _ip, _ip, _sz]),
-    "lulesh_force_plan_of": (_i, [_i, _i, _i, _ip, _ip, _ip, _sz]),
+    "lulesh_force_plan_of": (_i, [_i, _ip, _ip, _i, _i, _ip, _ip, _ip, _sz]),
     "lulesh_domain_get_eos_table": (_i, [_vp, ctypes.POINTER(ctypes.c_uint16), _sz]),
     "lulesh_device_local_backend": (_vp, []),
     "lulesh_device_copy_backend": (_vp, []),
@@ -586,17 +586,23 @@ def device_cus():
     return _chk(load().lulesh_device_cus(), "device_cus")
 
 
-def force_plan_of(nx, tp=1, cus=256):
-    """(kchunk_used, paired, tiles): the force kernel's column tiles of a
-    group of tp^3 subdomains of nx^3 elements on a device of `cus` CUs, as
-    the engine plans them (host only).  tiles is an (n, 8) int32 array of
-    subdomain, i0, j0, k0, ex, ey, flags, kn in launch order."""
+def force_plan_of(nx, tp=1, cus=256, kchunk=0):
+    """(kchunk_used, paired, tiles): the force kernel's column tiles of an
+    engine over groups of tp^3 subdomains of nx^3 elements (nx, tp: ints for
+    one group, or sequences, one entry per group) on a device of `cus` CUs,
+    as the engine plans them (host only).  tiles is an (n, 8) int32 array of
+    subdomain (numbered over the groups), i0, j0, k0, ex, ey, flags, kn in
+    launch order."""
     L = load()
-    n = _chk(L.lulesh_force_plan_of(nx, tp, cus, None, None, None, 0), "force_plan_of")
+    nxs = np.atleast_1d(np.asarray(nx, dtype=np.int32))
+    tps = np.broadcast_to(np.asarray(tp, dtype=np.int32), nxs.shape).copy()
+    ng = int(nxs.size)
+    a, b = _p(nxs, ctypes.c_int), _p(tps, ctypes.c_int)
+    n = _chk(L.lulesh_force_plan_of(ng, a, b, kchunk, cus, None, None, None, 0), "force_plan_of")
     tiles = np.zeros((max(n, 1), 8), dtype=np.int32)
     kc, pr = ctypes.c_int(), ctypes.c_int()
-    m = _chk(L.lulesh_force_plan_of(nx, tp, cus, ctypes.byref(kc), ctypes.byref(pr), _p(tiles, ctypes.c_int),
-                                    n), "force_plan_of")
+    m = _chk(L.lulesh_force_plan_of(ng, a, b, kchunk, cus, ctypes.byref(kc), ctypes.byref(pr),
+                                    _p(tiles, ctypes.c_int), n), "force_plan_of")
     assert m == n
     return kc.value, bool(pr.value), tiles[:n]
 

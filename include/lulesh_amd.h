@@ -293,15 +293,18 @@ int lulesh_comm_buf_size_of(int nx, int tp, int rank);
  * capacity of each array (nx^3 needed).  Returns nx^3. */
 int lulesh_region_layout(int nx, int tp, int rank, int nr, int balance, int cost,
                          int *region, int *rep, size_t n);
-/* The force kernel's column tiles that a group of tp^3 subdomains of nx^3
- * elements gets on a device of `cus` CUs, host only (no counterpart in the
- * reference; for tests and tuning without a GPU): the same planner the
+/* The force kernel's column tiles that an engine stepping `ngroups`
+ * groups (group g: tp[g]^3 subdomains of nx[g]^3 elements) gets on a device
+ * of `cus` CUs with z-chunk `kchunk` (0: chosen), host only (no counterpart
+ * in the reference; for tests and tuning without a GPU): the planner the
  * engine runs.  Writes up to n tiles of 8 ints each, in launch order --
- * subdomain, i0, j0, k0 (first node column, row, plane), ex, ey (element
- * slots), flags (2: the subdomain's first tile) and kn (node planes of the
- * run) -- and the z-chunk chosen and whether the paired plan was taken.
- * Returns the number of tiles (write none with tiles = NULL). */
-int lulesh_force_plan_of(int nx, int tp, int cus, int *kchunk_used, int *paired, int *tiles, size_t n);
+ * subdomain (numbered over the groups in order), i0, j0, k0 (first node
+ * column, row, plane), ex, ey (element slots), flags (2: the subdomain's
+ * first tile) and kn (node planes of the run) -- and the z-chunk used and
+ * whether the paired plan was taken.  Returns the number of tiles (write
+ * none with tiles = NULL). */
+int lulesh_force_plan_of(int ngroups, const int *nx, const int *tp, int kchunk, int cus, int *kchunk_used,
+                         int *paired, int *tiles, size_t n);
 
 /* Backend hook table: the run-time form of the compile-time backend API of
  * LULESH/cuda/src/comm/comm_backend.h:7-22 (dispatch :140-173).  The first

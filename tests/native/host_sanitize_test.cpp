@@ -138,12 +138,12 @@ static void force_plans()
     for (int cus : {1, 8, 80, 256, 304})
         for (int tp = 1; tp <= 2; ++tp)
             for (int nx : {1, 2, 3, 5, 8, 13, 16, 31, 45, 48, 64, 100, 130}) {
-                const int n = lulesh_force_plan_of(nx, tp, cus, nullptr, nullptr, nullptr, 0);
+                const int n = lulesh_force_plan_of(1, &nx, &tp, 0, cus, nullptr, nullptr, nullptr, 0);
                 EXPECT(n > 0);
                 if (n <= 0) continue;
                 t.assign((size_t)8 * n, -1);
                 int kc = 0, paired = -1;
-                EXPECT(lulesh_force_plan_of(nx, tp, cus, &kc, &paired, t.data(), n) == n);
+                EXPECT(lulesh_force_plan_of(1, &nx, &tp, 0, cus, &kc, &paired, t.data(), n) == n);
                 EXPECT(kc >= 1 && (paired == 0 || paired == 1));
                 const int s1 = nx + 1;
                 std::vector<int> covered((size_t)tp * tp * tp * s1 * s1 * s1, 0);
@@ -158,9 +158,23 @@ static void force_plans()
                 }
                 for (int c : covered) EXPECT(c == 1);
             }
-    EXPECT(lulesh_force_plan_of(0, 1, 256, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
-    EXPECT(lulesh_force_plan_of(8, 5, 256, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
-    EXPECT(lulesh_force_plan_of(8, 1, 0, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
+    const int z = 0, five = 5, eight = 8, one = 1;
+    EXPECT(lulesh_force_plan_of(1, &z, &one, 0, 256, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
+    EXPECT(lulesh_force_plan_of(1, &eight, &five, 0, 256, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
+    EXPECT(lulesh_force_plan_of(1, &eight, &one, 0, 0, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
+    EXPECT(lulesh_force_plan_of(0, &eight, &one, 0, 256, nullptr, nullptr, nullptr, 0) == LULESH_EINVAL);
+    // an ensemble of several groups, the z-chunk chosen and given
+    const int gnx[5] = {16, 33, 64, 20, 48}, gtp[5] = {1, 1, 1, 2, 1};
+    for (int kc : {0, 4, 8}) {
+        const int n = lulesh_force_plan_of(5, gnx, gtp, kc, 256, nullptr, nullptr, nullptr, 0);
+        EXPECT(n > 0);
+        if (n > 0) {
+            t.assign((size_t)8 * n, -1);
+            int used = 0, paired = -1;
+            EXPECT(lulesh_force_plan_of(5, gnx, gtp, kc, 256, &used, &paired, t.data(), n) == n);
+            EXPECT(paired == 0 && used >= 1 && (kc == 0 || used == kc));
+        }
+    }
 }
 
 static void oracle_runs()
