@@ -309,7 +309,7 @@ def bench_plain(args):
             "bound": "fp64 valu", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": tf / FP64_PEAK_TFLOPS, "flops_per_launch": flops,
             "valu_issue_busy": profiled_value("valu_busy", dom, nx, fu)}
-        # what any code can issue at the force kernel's occupancy (230
+        # what any code can issue at the force kernel's occupancy (232
         # VGPRs: two waves per SIMD), measured now on this GPU: eight
         # independent FP64 multiply / add chains per lane, no memory traffic
         ceil_tf, mhz = lulesh_amd.probe_fp64(2)
