@@ -700,7 +700,10 @@ int group_create(int nx, int tp, int nr, int balance, int cost, lulesh_group **o
         const auto t0 = std::chrono::steady_clock::now();
         rc = g->backend->alloc_recv(d, (size_t)d->host.comBufSize, &d->recv, g->backend_ctx);
         if (rc) return rc;
-        HIPCHK(hipDeviceSynchronize());
+        // a backend's set-up is done on return or ordered on the null
+        // stream (hipMalloc + hipMemset); the library's streams do not
+        // block on it, so groups stepping meanwhile are not waited for
+        HIPCHK(hipStreamSynchronize(nullptr));
         g->phase_setup_ms = std::max(
             g->phase_setup_ms, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     }

@@ -147,6 +147,14 @@ struct GroupCtl {
     int ndom;
 };
 
+// what a host poll needs of a group after an engine's chunk of cycles: its
+// loop state and the first subdomain's completed-cycle time state, copied
+// by the chunk's last kernel into pinned host memory (one entry per group)
+struct CtlSnap {
+    int done, status, cycle, pad;
+    double time, deltatime;
+};
+
 // element or node tile: [start, start + blockDim) of subdomain dom; for
 // element tiles eoff/ecount locate the tile's EOS evaluations in the
 // subdomain's evaluation table.  A node-update tile with pad = 1 covers

@@ -202,8 +202,9 @@ static int build_tiles(Engine *e)
     return LULESH_OK;
 }
 
-int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
+int engine_create(const std::vector<lulesh_group *> &groups, Engine **out, const std::vector<int> *cycles)
 {
+    if (cycles && cycles->size() != groups.size()) return LULESH_EINVAL;
     Engine *e = new Engine();
     e->groups = groups;
     const char *nti = getenv("LULESH_NO_FUSED_TI");
@@ -226,18 +227,25 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
             return LULESH_EINVAL;
         }
     if (e->umesh) e->march = false;
-    for (auto *g : groups) {
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        lulesh_group *g = groups[gi];
         if (g->ndom > 1) e->fused = false;
         // node-state buffers as of the group's last counted cycle
-        GroupCtl G;
-        DomCtl D0;
-        int rc0 = group_read_ctl(g, &G, &D0, g->stream);
-        if (rc0) { engine_destroy(e); return rc0; }
+        int cycle;
+        if (cycles) {
+            cycle = (*cycles)[gi];
+        } else {
+            GroupCtl G;
+            DomCtl D0;
+            const int rc0 = group_read_ctl(g, &G, &D0, g->stream);
+            if (rc0) { engine_destroy(e); return rc0; }
+            cycle = D0.cycle;
+        }
         dom0.push_back((int)doms.size());
         for (auto *d : g->doms) {
             DomDev D = d->dev;
-            D.cur = D.ns[D0.cycle & 1];
-            D.old = D.ns[(D0.cycle & 1) ^ 1];
+            D.cur = D.ns[cycle & 1];
+            D.old = D.ns[(cycle & 1) ^ 1];
             doms.push_back(D);
         }
         gptr.push_back(g->gctl);
@@ -309,7 +317,9 @@ int engine_create(const std::vector<lulesh_group *> &groups, Engine **out)
     }
     if (hipMalloc((void **)&e->d_doms, std::max<size_t>(1, doms.size()) * sizeof(DomDev)) != hipSuccess ||
         hipMalloc((void **)&e->d_groups, std::max<size_t>(1, gptr.size()) * sizeof(GroupCtl *)) != hipSuccess ||
-        hipMalloc((void **)&e->d_dom0, std::max<size_t>(1, dom0.size()) * sizeof(int)) != hipSuccess) {
+        hipMalloc((void **)&e->d_dom0, std::max<size_t>(1, dom0.size()) * sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **)&e->h_snap, std::max<size_t>(1, gptr.size()) * sizeof(CtlSnap),
+                      hipHostMallocCoherent) != hipSuccess) {
         engine_destroy(e);
         return LULESH_ENOMEM;
     }
@@ -347,6 +357,8 @@ void engine_destroy(Engine *e)
     if (e->d_ntiles) (void)hipFree(e->d_ntiles);
     if (e->d_ctiles) (void)hipFree(e->d_ctiles);
     if (e->d_utiles) (void)hipFree(e->d_utiles);
+    if (e->snap_pending) (void)hipStreamSynchronize(e->snap_stream);
+    if (e->h_snap) (void)hipHostFree(e->h_snap);
     delete e;
 }
 
@@ -482,6 +494,9 @@ int engine_enqueue(Engine *e, int ncycles, hipStream_t s)
             const int rc = engine_enqueue_cycle(e, s);
             if (rc) return rc;
         }
+        e->snap_stream = s;
+        e->snap_pending = true;
+        HIPCHK(launch_ctl_snapshot(e->d_groups, (int)e->groups.size(), e->h_snap, s));
         return LULESH_OK;
     }
     for (size_t i = 0; i < e->graphs.size(); ++i) {
@@ -489,6 +504,8 @@ int engine_enqueue(Engine *e, int ncycles, hipStream_t s)
             Engine::CachedGraph c = e->graphs[i];
             e->graphs.erase(e->graphs.begin() + (long)i);
             e->graphs.push_back(c);
+            e->snap_stream = s;
+            e->snap_pending = true;
             HIPCHK(hipGraphLaunch(c.exec, s));
             return LULESH_OK;
         }
@@ -500,6 +517,8 @@ int engine_enqueue(Engine *e, int ncycles, hipStream_t s)
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = LULESH_OK;
     for (int c = 0; c < ncycles && rc == LULESH_OK; ++c) rc = engine_enqueue_cycle(e, s);
+    if (rc == LULESH_OK && launch_ctl_snapshot(e->d_groups, (int)e->groups.size(), e->h_snap, s) != hipSuccess)
+        rc = LULESH_EHIP;
     Engine::CachedGraph c;
     c.ncycles = ncycles;
     c.stream = s;
@@ -518,6 +537,8 @@ int engine_enqueue(Engine *e, int ncycles, hipStream_t s)
         HIPCHK(ie);
     }
     e->graphs.push_back(c);
+    e->snap_stream = s;
+    e->snap_pending = true;
     HIPCHK(hipGraphLaunch(c.exec, s));
     return LULESH_OK;
 }
@@ -805,6 +826,7 @@ struct Request {
     int nx, nr, balance, cost;
     double arrival;
     size_t bytes;
+    int cycle = 0;   // completed cycles as last seen (a new group has run none)
 };
 
 struct Result {
@@ -926,53 +948,76 @@ static int server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream)
     if (!s) return LULESH_EINVAL;
     (void)stream;
     if (cycles <= 0) cycles = 16;
-    int rc = server_admit(s);
-    if (rc) return rc;
+    int rc;
     if (s->active.empty()) {
-        if (!s->queue.empty()) {
-            const double wait = s->queue.front().arrival - s->now();
-            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+        // nothing on the device to overlap with: admit now
+        rc = server_admit(s);
+        if (rc) return rc;
+        if (s->active.empty()) {
+            if (!s->queue.empty()) {
+                const double wait = s->queue.front().arrival - s->now();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+            }
+            return 0;
         }
-        return 0;
     }
     if (s->dirty) {
+        // the members' completed cycles are known: from the last chunk's
+        // snapshot, or 0 for a group admitted since
+        std::vector<lulesh_group *> gs;
+        std::vector<int> cyc;
+        for (auto &a : s->active) {
+            gs.push_back(a.second);
+            cyc.push_back(a.first.cycle);
+        }
         if (s->engine) engine_destroy(s->engine);
         s->engine = nullptr;
-        std::vector<lulesh_group *> gs;
-        for (auto &a : s->active) gs.push_back(a.second);
-        rc = engine_create(gs, &s->engine);
+        rc = engine_create(gs, &s->engine, &cyc);
         if (rc) return rc;
         s->dirty = false;
     }
     rc = engine_enqueue(s->engine, cycles, s->stream);
     if (rc) return rc;
+    // the engine's members are the first entries of the active list, in
+    // order (every change of the list rebuilds the engine before the next
+    // chunk); requests admitted while the chunk runs (their set-up is host
+    // work and kernels on their own streams) join at the next chunk
+    const size_t nmem = s->engine->groups.size();
+    rc = server_admit(s);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s->stream));
+    // the chunk's snapshot: one entry per engine member
+    const CtlSnap *snap = s->engine->h_snap;
     int completed = 0;
-    for (size_t i = 0; i < s->active.size();) {
-        lulesh_group *g = s->active[i].second;
-        GroupCtl G;
-        DomCtl D0;
-        rc = group_read_ctl(g, &G, &D0, s->stream);
-        if (rc) return rc;
-        if (G.done || G.status) {
-            Result &res = s->results[s->active[i].first.id];
-            res.status = G.status;
-            res.cycles = D0.cycle;
+    std::vector<std::pair<Request, lulesh_group *>> keep;
+    keep.reserve(s->active.size());
+    for (size_t i = 0; i < s->active.size(); ++i) {
+        auto &a = s->active[i];
+        if (i >= nmem) {
+            keep.push_back(a);
+            continue;
+        }
+        const CtlSnap &cs = snap[i];
+        a.first.cycle = cs.cycle;
+        if (cs.done || cs.status) {
+            Result &res = s->results[a.first.id];
+            res.status = cs.status;
+            res.cycles = cs.cycle;
             double v[4] = {0, 0, 0, 0};
-            lulesh_group_verify(g, 0, v);
+            lulesh_group_verify(a.second, 0, v);
             res.energy = v[0];
             res.tad = v[2];
             res.t_done = s->now();
-            s->in_use -= s->active[i].first.bytes;
-            group_free(g);
-            s->active.erase(s->active.begin() + i);
+            s->in_use -= a.first.bytes;
+            group_free(a.second);
             s->dirty = true;
             ++completed;
             ++s->done;
         } else {
-            ++i;
+            keep.push_back(a);
         }
     }
+    s->active.swap(keep);
     return completed;
 }
 

@@ -149,9 +149,19 @@ struct Engine {
     };
     std::vector<CachedGraph> graphs;
     bool use_graph = true;
+    // pinned host memory, one entry per member group (engine order): each
+    // engine_enqueue ends with a kernel that writes the members' loop
+    // state here, valid once that enqueue's stream has completed
+    CtlSnap *h_snap = nullptr;
+    hipStream_t snap_stream = nullptr;   // of the last enqueue (drained before h_snap is freed)
+    bool snap_pending = false;
 };
 
-int engine_create(const std::vector<lulesh_group *> &groups, Engine **out);
+// cycles, when given: each member's completed cycle count (engine order),
+// as a snapshot of a finished stream shows it, instead of reading every
+// member's control block
+int engine_create(const std::vector<lulesh_group *> &groups, Engine **out,
+                  const std::vector<int> *cycles = nullptr);
 void engine_destroy(Engine *e);
 int engine_set_tiles(Engine *e, int tile_e, int tile_n);
 int engine_set_kchunk(Engine *e, int kchunk);

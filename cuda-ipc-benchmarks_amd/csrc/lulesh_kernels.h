@@ -45,6 +45,9 @@ hipError_t launch_hg_coef(double *coef, const double *ss, const double *mass, co
 hipError_t launch_copy_msgs(const CopyMsgs &m, hipStream_t s);
 hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s);
 hipError_t launch_probe_cbrt(const double *in, double *out, int n, hipStream_t s);
+// each group's CtlSnap (groups: device array of GroupCtl pointers; out:
+// pinned host memory, ngroups entries)
+hipError_t launch_ctl_snapshot(GroupCtl *const *groups, int ngroups, CtlSnap *out, hipStream_t s);
 hipError_t launch_probe_fp64(double *out, unsigned long long *clk, int grid, int iters, bool one_chain,
                              hipStream_t s);
 hipError_t launch_probe_sqrt_div(const double *a, const double *b, double *out, int n, hipStream_t s);

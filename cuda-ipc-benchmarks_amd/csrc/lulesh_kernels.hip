@@ -1416,6 +1416,25 @@ __global__ __launch_bounds__(256) void k_probe_copy(const double2 *__restrict__ 
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += stride) b[t] = a[t];
 }
 
+// the loop state of every group of an engine after its chunk of cycles,
+// into pinned host memory: the host reads it after the stream completes
+// instead of copying each group's control block
+__global__ __launch_bounds__(64) void k_ctl_snapshot(GroupCtl *const *groups, int ngroups, CtlSnap *out)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const GroupCtl &G = *groups[g];
+    const DomCtl &C = G.dctl[0];
+    CtlSnap v;
+    v.done = G.done;
+    v.status = G.status;
+    v.cycle = C.cycle;
+    v.pad = 0;
+    v.time = C.time;
+    v.deltatime = C.deltatime;
+    out[g] = v;
+}
+
 // FP64 issue rate: 16 multiplies and adds per loop trip and lane, as eight
 // independent chains or (ONE) as a single chain where each operation reads
 // the one before (no memory traffic, no FMA); thread 0 of each workgroup
@@ -1479,6 +1498,13 @@ hipError_t launch_probe_copy(const double *a, double *b, size_t n, hipStream_t s
 {
     const size_t n2 = n / 2;
     hipLaunchKernelGGL(k_probe_copy, dim3(4096), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n2);
+    return hipGetLastError();
+}
+
+hipError_t launch_ctl_snapshot(GroupCtl *const *groups, int ngroups, CtlSnap *out, hipStream_t s)
+{
+    if (ngroups < 1) return hipSuccess;
+    hipLaunchKernelGGL(k_ctl_snapshot, dim3((ngroups + 63) / 64), dim3(64), 0, s, groups, ngroups, out);
     return hipGetLastError();
 }
 

@@ -85,20 +85,34 @@ struct ZCost {
     double time = 0.0;
     long tiles = 0;
 };
-static ZCost z_cost(const ForcePlanInput &in, int kc, bool even)
+// a subdomain's node planes, tile shape and node columns, computed once per
+// plan (the chunk candidates below all reuse them)
+struct DomShape {
+    int s1, ex, ey;
+    long cols;
+};
+static std::vector<DomShape> dom_shapes(const ForcePlanInput &in)
+{
+    std::vector<DomShape> v;
+    for (const auto &g : in.group_nx)
+        for (const int nx : g) {
+            DomShape d;
+            d.s1 = nx + 1;
+            tile_shape(nx, d.ex, d.ey);
+            d.cols = (long)((d.s1 + d.ex - 2) / (d.ex - 1)) * ((d.s1 + d.ey - 2) / (d.ey - 1));
+            v.push_back(d);
+        }
+    return v;
+}
+static ZCost z_cost(const ForcePlanInput &in, const std::vector<DomShape> &shapes, int kc, bool even)
 {
     long tiles = 0;
     int longest = 0;
-    for (const auto &g : in.group_nx)
-        for (const int nx : g) {
-            int ex, ey;
-            tile_shape(nx, ex, ey);
-            const int s1 = nx + 1;
-            const long cols = (long)((s1 + ex - 2) / (ex - 1)) * ((s1 + ey - 2) / (ey - 1));
-            const ZRuns z = z_runs(s1, kc, even);
-            tiles += cols * z.n;
-            longest = std::max(longest, std::max(z.planes(0), z.planes(z.n - 1)));
-        }
+    for (const DomShape &d : shapes) {
+        const ZRuns z = z_runs(d.s1, kc, even);
+        tiles += d.cols * z.n;
+        longest = std::max(longest, std::max(z.planes(0), z.planes(z.n - 1)));
+    }
     const long slots = 2L * in.cus;
     ZCost c;
     c.time = (double)((tiles + slots - 1) / slots) * (longest + 2) * (tiles <= in.cus ? 0.72 : 1.0);
@@ -115,7 +129,8 @@ static ZCost z_cost(const ForcePlanInput &in, int kc, bool even)
 // model, 5.25 with runs of kc and the remainder last, 5.28e9 zone-cycles/s
 // with even runs; the 64-problem ensemble 4.95 -> 5.04e9 at kc = 8); the
 // batch autotuner times its kc candidates on top.
-static void choose_zruns(const ForcePlanInput &in, int &kc_out, bool &even_out)
+static void choose_zruns(const ForcePlanInput &in, const std::vector<DomShape> &shapes, int &kc_out,
+                         bool &even_out)
 {
     const int lo = in.kchunk > 0 ? in.kchunk : 1, hi = in.kchunk > 0 ? in.kchunk : 32;
     if (in.group_nx.size() > 1) {
@@ -125,14 +140,7 @@ static void choose_zruns(const ForcePlanInput &in, int &kc_out, bool &even_out)
         even_out = true;
         for (int kc = lo; kc <= hi; ++kc) {
             long tiles = 0;
-            for (const auto &g : in.group_nx)
-                for (const int nx : g) {
-                    int ex, ey;
-                    tile_shape(nx, ex, ey);
-                    const int s1 = nx + 1;
-                    tiles += (long)((s1 + ex - 2) / (ex - 1)) * ((s1 + ey - 2) / (ey - 1)) *
-                             ((s1 + kc - 1) / kc);
-                }
+            for (const DomShape &d : shapes) tiles += d.cols * ((d.s1 + kc - 1) / kc);
             const long cost = ((tiles + slots - 1) / slots) * (kc + 1);
             if (best_cost < 0 || cost < best_cost) {
                 best_cost = cost;
@@ -145,7 +153,7 @@ static void choose_zruns(const ForcePlanInput &in, int &kc_out, bool &even_out)
     ZCost best;
     for (int kc = lo; kc <= hi; ++kc)
         for (int ev = 1; ev >= 0; --ev) {
-            const ZCost c = z_cost(in, kc, ev != 0);
+            const ZCost c = z_cost(in, shapes, kc, ev != 0);
             const bool better = !have || c.time < best.time - 1e-9 ||
                                 (c.time < best.time + 1e-9 && c.tiles > best.tiles);
             if (better) {
@@ -401,28 +409,25 @@ int plan_force_columns(const ForcePlanInput &in, ForcePlan &out)
 {
     out = ForcePlan{};
     if (in.cus < 1) return LULESH_EINVAL;
+    const std::vector<DomShape> shapes = dom_shapes(in);
+    for (const DomShape &d : shapes)
+        if (d.ex * d.ey > 256 || (d.ex + 1) * (d.ey + 1) > kForcePatch) return LULESH_EINVAL;
     int kc = 1;
     bool even = true;
-    choose_zruns(in, kc, even);
+    choose_zruns(in, shapes, kc, even);
     std::vector<ColTile> &ct = out.tiles;
     std::vector<ColRef> cols;
-    int di = 0;
-    for (const auto &g : in.group_nx)
-        for (const int nx : g) {
-            const int s1 = nx + 1;
-            int ex, ey;
-            tile_shape(nx, ex, ey);
-            if (ex * ey > 256 || (ex + 1) * (ey + 1) > kForcePatch) return LULESH_EINVAL;
-            const ZRuns z = z_runs(s1, kc, even);
-            for (int r = 0, k0 = 0; r < z.n; k0 += z.planes(r), ++r)
-                for (int j0 = 0; j0 < s1; j0 += ey - 1)
-                    for (int i0 = 0; i0 < s1; i0 += ex - 1)
-                        ct.push_back(ColTile{di, i0, j0, k0, ex, ey, (k0 | j0 | i0) == 0 ? kFirstTile : 0,
-                                             z.planes(r)});
+    for (int di = 0; di < (int)shapes.size(); ++di) {
+        const int s1 = shapes[di].s1, ex = shapes[di].ex, ey = shapes[di].ey;
+        const ZRuns z = z_runs(s1, kc, even);
+        for (int r = 0, k0 = 0; r < z.n; k0 += z.planes(r), ++r)
             for (int j0 = 0; j0 < s1; j0 += ey - 1)
-                for (int i0 = 0; i0 < s1; i0 += ex - 1) cols.push_back(ColRef{di, i0, j0, ex, ey, s1});
-            ++di;
-        }
+                for (int i0 = 0; i0 < s1; i0 += ex - 1)
+                    ct.push_back(ColTile{di, i0, j0, k0, ex, ey, (k0 | j0 | i0) == 0 ? kFirstTile : 0,
+                                         z.planes(r)});
+        for (int j0 = 0; j0 < s1; j0 += ey - 1)
+            for (int i0 = 0; i0 < s1; i0 += ex - 1) cols.push_back(ColRef{di, i0, j0, ex, ey, s1});
+    }
     out.kchunk_used = kc;
     if (in.paired_allowed && !ct.empty()) {
         const PairedPlan p = paired_plan(cols, in.cus);

@@ -323,7 +323,8 @@ int lulesh_force_plan_of(int ngroups, const int *nx, const int *tp, int kchunk, 
  * message into the receiver's slot). */
 typedef struct lulesh_comm_backend_ops {
     const char *name;
-    /* commAllocRecv: receive buffer of comBufSize doubles for subdomain d */
+    /* commAllocRecv: receive buffer of comBufSize doubles for subdomain d,
+     * ready when the call returns or once the null stream completes */
     int (*alloc_recv)(lulesh_domain *d, size_t comBufSize, double **recv, void *ctx);
     /* commTeardown */
     int (*teardown)(lulesh_domain *d, double *recv, void *ctx);
