@@ -30,6 +30,7 @@
 #include <mutex>
 #include <new>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "lulesh_amd.h"
@@ -645,27 +646,63 @@ int lulesh_batch_group(lulesh_batch *b, int i, lulesh_group **out)
     return LULESH_OK;
 }
 
-// the members still running, and the first error any member raised
-static int batch_poll(lulesh_batch *b, std::vector<lulesh_group *> &running, int *first_err)
+// a member's loop state as the last poll saw it
+struct MemberState {
+    int done = 0, status = 0, cycle = 0;
+};
+
+// What a lulesh_batch_run call knows of the members: their states in
+// b->groups order, and where each member sits in that order.
+struct BatchView {
+    std::vector<MemberState> st;
+    std::unordered_map<const lulesh_group *, size_t> idx;
+    const MemberState &of(const lulesh_group *g) const { return st[idx.at(g)]; }
+};
+
+// The members still running, and the first error any member raised.
+// With `snap`, the engine's members are read from the snapshot its last
+// chunk wrote (the batch's stream has completed it) and the others keep the
+// state last seen: during lulesh_batch_run nothing steps a member the
+// engine no longer holds.  Without it (the start of a run: host calls may
+// have changed any member since) every member's control block is read.
+static int batch_poll(lulesh_batch *b, BatchView &v, bool snap, std::vector<lulesh_group *> &running,
+                      int *first_err)
 {
-    // every group's GroupCtl sits at the head of its own allocation
     running.clear();
     *first_err = 0;
     HIPCHK(hipStreamSynchronize(b->stream));
-    for (auto *g : b->groups) {
-        GroupCtl G;
-        HIPCHK(hipMemcpy(&G, g->gctl, sizeof(G), hipMemcpyDeviceToHost));
-        if (G.status && !*first_err) *first_err = G.status;
-        if (!G.done && !G.status) running.push_back(g);
+    if (snap) {
+        const std::vector<lulesh_group *> &eg = b->engine->groups;
+        for (size_t k = 0; k < eg.size(); ++k) {
+            const CtlSnap &c = b->engine->h_snap[k];
+            v.st[v.idx.at(eg[k])] = MemberState{c.done, c.status, c.cycle};
+        }
+    } else {
+        for (size_t i = 0; i < b->groups.size(); ++i) {
+            GroupCtl G;
+            DomCtl D0;
+            const int rc = group_read_ctl(b->groups[i], &G, &D0, b->stream);
+            if (rc) return rc;
+            v.st[i] = MemberState{G.done, G.status, D0.cycle};
+        }
+    }
+    for (size_t i = 0; i < b->groups.size(); ++i) {
+        const MemberState &m = v.st[i];
+        if (m.status && !*first_err) *first_err = m.status;
+        if (!m.done && !m.status) running.push_back(b->groups[i]);
     }
     return LULESH_OK;
 }
 
-// an engine over `members` (z elements) in place of the current one
-static int batch_rebuild(lulesh_batch *b, const std::vector<lulesh_group *> &members, long long z)
+// an engine over `members` (z elements) in place of the current one; the
+// members' cycle counts are the poll's
+static int batch_rebuild(lulesh_batch *b, const std::vector<lulesh_group *> &members, long long z,
+                         const BatchView &v)
 {
+    std::vector<int> cycles;
+    for (auto *g : members) cycles.push_back(v.of(g).cycle);
     Engine *e = nullptr;
-    int rc = engine_create(members, &e);
+    int rc = engine_create(members, &e, &cycles);
     if (rc == LULESH_OK && (b->tile_e || b->tile_n)) rc = engine_set_tiles(e, b->tile_e, b->tile_n);
     if (rc) {
         if (e) engine_destroy(e);
@@ -685,47 +722,59 @@ static int batch_rebuild(lulesh_batch *b, const std::vector<lulesh_group *> &mem
 // (measured on the 64-problem ensemble: 5.22e9 zone-cycles/s at 5/8, 5.33
 // at 6/8, 5.41 at 7/8, 5.29 rebuilding whenever a member finishes).
 // Results do not depend on the tiling.
-static int batch_shrink(lulesh_batch *b, const std::vector<lulesh_group *> &running)
+static int batch_shrink(lulesh_batch *b, const std::vector<lulesh_group *> &running, const BatchView &v)
 {
     long long z = 0;
     for (auto *g : running) z += zones_of(g);
     if (running.empty() || z * 8 >= b->engine_zones * 7) return LULESH_OK;
-    return batch_rebuild(b, running, z);
+    return batch_rebuild(b, running, z, v);
 }
 
 // a member re-armed after the engine dropped it (a raised cycle cap or stop
 // time): the engine covers every running member again
-static int batch_cover(lulesh_batch *b, const std::vector<lulesh_group *> &running)
+static int batch_cover(lulesh_batch *b, const std::vector<lulesh_group *> &running, const BatchView &v)
 {
     for (auto *g : running)
         if (std::find(b->engine_groups.begin(), b->engine_groups.end(), g) == b->engine_groups.end()) {
             long long z = 0;
             for (auto *h : running) z += zones_of(h);
-            return batch_rebuild(b, running, z);
+            return batch_rebuild(b, running, z, v);
         }
     return LULESH_OK;
 }
 
-int lulesh_batch_run(lulesh_batch *b, int cycles_per_sync, lulesh_stream_t stream)
+static int batch_run(lulesh_batch *b, int cycles_per_sync)
 {
-    if (!b) return LULESH_EINVAL;
-    (void)stream;
     if (cycles_per_sync <= 0) cycles_per_sync = 32;
+    BatchView v;
+    v.st.resize(b->groups.size());
+    for (size_t i = 0; i < b->groups.size(); ++i) v.idx[b->groups[i]] = i;
     std::vector<lulesh_group *> running;
     int err0;
-    int rc0 = batch_poll(b, running, &err0);
-    if (rc0 == LULESH_OK) rc0 = batch_cover(b, running);
+    int rc0 = batch_poll(b, v, false, running, &err0);
+    if (rc0 == LULESH_OK) rc0 = batch_cover(b, running, v);
     if (rc0) return rc0;
     if (running.empty()) return err0;
     for (;;) {
         int rc = engine_enqueue(b->engine, cycles_per_sync, b->stream);
         if (rc) return rc;
         int err;
-        rc = batch_poll(b, running, &err);
+        rc = batch_poll(b, v, true, running, &err);
         if (rc) return rc;
         if (running.empty()) return err;
-        rc = batch_shrink(b, running);
+        rc = batch_shrink(b, running, v);
         if (rc) return rc;
+    }
+}
+
+int lulesh_batch_run(lulesh_batch *b, int cycles_per_sync, lulesh_stream_t stream)
+{
+    if (!b) return LULESH_EINVAL;
+    (void)stream;
+    try {
+        return batch_run(b, cycles_per_sync);
+    } catch (const std::bad_alloc &) {
+        return LULESH_ENOMEM;
     }
 }
 
