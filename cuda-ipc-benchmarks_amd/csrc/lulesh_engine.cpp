@@ -1034,8 +1034,9 @@ static int server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream)
     // chunk); requests admitted while the chunk runs (their set-up is host
     // work and kernels on their own streams) join at the next chunk
     const size_t nmem = s->engine->groups.size();
-    rc = server_admit(s);
-    if (rc) return rc;
+    // an admission error is reported once this chunk's results are taken:
+    // the next rebuild needs every member's cycle count from its snapshot
+    const int admit_rc = server_admit(s);
     HIPCHK(hipStreamSynchronize(s->stream));
     // the chunk's snapshot: one entry per engine member
     const CtlSnap *snap = s->engine->h_snap;
@@ -1069,7 +1070,7 @@ static int server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream)
         }
     }
     s->active.swap(keep);
-    return completed;
+    return admit_rc ? admit_rc : completed;
 }
 
 int lulesh_server_poll(lulesh_server *s, int cycles, lulesh_stream_t stream)
