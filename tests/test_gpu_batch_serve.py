@@ -63,6 +63,36 @@ def test_batch_tiny_problems_vs_oracle(amd, oracle_mod):
         o.close()
 
 
+def test_batch_members_rearmed_between_runs(amd):
+    """Members stop at their caps.  In the first run the two small members
+    stop at cycle 10 and the engine shrinks to the large one mid-run (its
+    cycle count taken from a chunk's snapshot); raising a stopped member's
+    cap through its group handle makes the next run rebuild over it (its
+    cycle count read afresh by the run's first poll); raising the batch's
+    cap resumes all three.  Every state equals a plain run to the same
+    cycle."""
+    print("case: batch members re-armed between runs", flush=True)
+    mix = MIX[:3]                          # s 6, 9, 12
+    nx, nr, bal, cost = (list(v) for v in zip(*mix))
+    b = amd.Batch(nx, nr, bal, cost)
+    b.set_max_cycles(10)
+    b.group(2).set_max_cycles(20)
+    assert b.run(4) == 0
+    assert list(b.results()[0]) == [10, 10, 20]
+    b.group(0).set_max_cycles(27)          # a member the engine no longer holds
+    assert b.run(4) == 0
+    assert list(b.results()[0]) == [27, 10, 20]
+    b.set_max_cycles(33)                   # every member
+    assert b.run(4) == 0
+    assert list(b.results()[0]) == [33, 33, 33]
+    for i, (s, r, ba, c) in enumerate(mix):
+        g = plain(amd, s, r, ba, c, 33)
+        for f in FIELDS:
+            assert_bitwise(b.group(i).field(f), g.field(f), f"re-armed problem {i} (s={s}) {f}")
+        g.close()
+    b.close()
+
+
 @pytest.mark.parametrize("tiles,kchunk", [((64, 128), 4), ((256, 256), 1), ((128, 64), 0)])
 def test_batch_tiles_do_not_change_results(amd, tiles, kchunk):
     print(f"case: batch tiles {tiles} kchunk {kchunk}", flush=True)
